@@ -1,0 +1,162 @@
+/*
+ * iss_hip.h — C-ABI of the MI355X (gfx950) Cooper-Frye sampler.
+ *
+ * The reference (chunshen1987/iSS) has no C-ABI: its FSSW sampler is a C++
+ * class driven by `iSS::generate_samples` (src/iSS.cpp:130-165).  Each entry
+ * point below replaces one step of that class; the comment names the
+ * reference code it stands in for.  Plain pointers and sizes only; all host
+ * buffers are caller-owned, device memory is owned by the context.  Every
+ * call is synchronous at return (one internal HIP stream per context) and a
+ * context must be used by one host thread at a time, as the reference's single
+ * shared RNG makes an `iSS` instance single-threaded (src/iSS.h:27).
+ *
+ * Status: 0 = OK, negative = error; `iss_last_error` gives the message.  No
+ * entry point calls exit(); the C++ drop-in layer maps errors to the
+ * reference's message-then-exit behaviour only where the reference does.
+ */
+#ifndef ISS_HIP_H_
+#define ISS_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISS_OK 0
+#define ISS_ERR_ARG (-1)
+#define ISS_ERR_HIP (-2)
+#define ISS_ERR_STATE (-3)
+#define ISS_ERR_UNSUPPORTED (-4)
+#define ISS_ERR_NOMEM (-5)
+#define ISS_ERR_RANGE (-6)
+
+typedef struct iss_ctx iss_ctx;
+
+/* iSS_Hadron, same field order and size (src/data_struct.h:79-84). */
+typedef struct {
+    int32_t pid;
+    float mass;
+    float E, px, py, pz;
+    float t, x, y, z;
+} iss_hadron;
+
+/* Species in sampling order (the caller sorts by mass as
+ * src/FSSW.cpp:152-162 does).  sign: -1 Bose-Einstein, +1 Fermi-Dirac,
+ * 0 Boltzmann (src/readindata.cpp:1103-1113). */
+typedef struct {
+    int32_t n;
+    const int32_t* pid;
+    const double* mass;
+    const int32_t* degeneracy;
+    const int32_t* baryon;
+    const int32_t* strange;
+    const int32_t* charge;
+    const int32_t* sign;
+} iss_species_desc;
+
+/* delta-f switches and coefficient tables (src/FSSW.cpp:72-83, 175-195).
+ * bulk_kind: 1 RTA polynomial, 11 OSU 14-moment (T, muB) table, 20 WSU
+ * 22-moment NEoS table, 21 WSU Chapman-Enskog NEoS table; kinds 0, 2, 3, 4
+ * contribute nothing on the FSSW path (src/FSSW.cpp:663-692, 1916-1923). */
+typedef struct {
+    int32_t include_shear;
+    int32_t include_bulk;
+    int32_t bulk_kind;
+    int32_t include_diffusion;     /* must be 0 */
+    /* kind 11: grid of nT x nmu, values [iT * nmu + imu] */
+    int32_t k11_nT, k11_nmu;
+    double k11_T0, k11_mu0, k11_dT, k11_dmu;
+    const double* k11_c0;
+    const double* k11_c1;
+    const double* k11_c2;
+    /* kinds 20/21: 200 x 200 rows of neos_ncol (5 CE, 8 22-moment) */
+    int32_t neos_ncol;
+    const double* neos;
+    /* keep FermionMomentumSampler::CDF_2's missing alternating sign
+     * (src/FermionMomentumSampler.cpp:80-93); 1 = as the reference */
+    int32_t fermion_cdf2_quirk;
+} iss_deltaf_desc;
+
+/* Hadron-resonance-gas EoS used to regulate T, mu, P from (e, n_B)
+ * (src/readindata.cpp:768-801, 1249-1309).  rows: n_rows x 7
+ * {e, nB, P, T, muB, muS, muQ}; iEOS 9/91: 1-D in e, 12/14: 200 nB per e. */
+typedef struct {
+    int32_t iEOS;
+    int64_t n_rows;
+    const double* rows;
+} iss_eos_desc;
+
+typedef struct {
+    int32_t precision;       /* 32 or 64 */
+    int32_t hydro_mode;      /* 2: 3+1D; otherwise boost invariant */
+    double y_LB, y_RB;       /* rapidity window (src/FSSW.cpp:950-958) */
+} iss_yield_opts;
+
+typedef struct {
+    uint64_t seed;
+    int64_t event0;          /* index of the first event (counter-based RNG) */
+    int32_t n_events;
+    int32_t proposal;        /* 0 plain rejection, 1 learned mixture */
+    int32_t sampling_model;  /* 30 Poisson (src/FSSW.cpp:293-298) */
+    int32_t hydro_mode;
+    double y_LB, y_RB;
+    int32_t max_trials;      /* maximum_impatience (src/FSSW.cpp:1872) */
+    int32_t precision;       /* yield precision used if yields are stale */
+} iss_sample_opts;
+
+typedef struct {
+    int64_t n_hadrons;
+    int64_t n_trials;        /* outer rejection trials */
+    int64_t n_inner;         /* |p| proposals */
+    int64_t n_failed;        /* slots that exhausted max_trials */
+    int64_t n_range_errors;  /* m/T - mu/T outside the |p| tables */
+    double ms_yield;         /* device time of the last yield pass */
+    double ms_sample;        /* device time of the last sampling pass */
+    double ms_sample_kernel; /* device time of the last sampling kernel alone */
+    double ms_yield_kernel;  /* device time of the last yield kernel alone */
+} iss_stats;
+
+/* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */
+int iss_ctx_create(int device, iss_ctx** out);
+int iss_ctx_destroy(iss_ctx* ctx);
+const char* iss_last_error(const iss_ctx* ctx);
+
+/* ---- tables: FSSW ctor (src/FSSW.cpp:43-201) ---- */
+int iss_set_species(iss_ctx* ctx, const iss_species_desc* species);
+int iss_set_deltaf(iss_ctx* ctx, const iss_deltaf_desc* deltaf);
+
+/* ---- surfaces ----
+ * Lab-frame MUSIC cells, 34 floats each in FO_surf order (src/data_struct.h:53-63):
+ * regulated (src/readindata.cpp:768-842, EoS optional) and boosted to the fluid
+ * rest frame on the device (src/iSS.cpp:170-293); cells with u.dsigma < 0 are
+ * dropped (src/iSS.cpp:226). */
+int iss_set_surface_lab(iss_ctx* ctx, int64_t n, const float* cells34, const iss_eos_desc* eos);
+/* Cells already in the rest frame, 28 floats each in FO_surf_LRF order
+ * (src/data_struct.h:66-76). */
+int iss_set_surface_lrf(iss_ctx* ctx, int64_t n, const float* cells28);
+int64_t iss_num_cells(const iss_ctx* ctx);
+int iss_get_surface_lrf(iss_ctx* ctx, float* cells28);
+
+/* ---- yields: calculate_dN_dxtdy_for_one_particle_species for all species
+ * (src/FSSW.cpp:565-715, 719-848).  species_totals [n_species] FP64 sums of
+ * the clamped per-cell yields; per_cell [n_cells][n_species] in the working
+ * precision (float or double), may be NULL. ---- */
+int iss_compute_yields(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals,
+                       void* per_cell);
+
+/* ---- sampling: sample_using_dN_dxtdy_4all_particles_conventional
+ * (src/FSSW.cpp:873-1071) with sample_momemtum_from_a_fluid_cell,
+ * the boost and add_one_sampled_particle (src/FSSW.cpp:1852-2010). ---- */
+int iss_sample(iss_ctx* ctx, const iss_sample_opts* opts, int64_t* n_hadrons);
+/* hadrons per event of the last iss_sample (FSSW::get_number_of_particles) */
+int iss_get_event_counts(iss_ctx* ctx, int64_t* counts);
+/* all hadrons, grouped by event in event order (FSSW::Hadron_list) */
+int iss_get_hadrons(iss_ctx* ctx, iss_hadron* dst, int64_t capacity);
+int iss_get_stats(const iss_ctx* ctx, iss_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISS_HIP_H_ */

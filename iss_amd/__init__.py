@@ -1,0 +1,241 @@
+"""iss_amd: a gfx950-native Cooper-Frye sampler with iSS's FSSW behaviour.
+
+This module is a ctypes binding of the C-ABI declared in include/iss_hip.h
+(the library ``libiss_hip.so`` built next to this file).  It exists for the
+tests and the benchmark; the drop-in for iSS users is the C++ ``iSS`` class in
+``iss_amd/csrc/host``.  There is no CPU fallback: every call goes to the HIP
+library and fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libiss_hip.so")
+TABLE_DIR = os.path.join(_HERE, "tables")
+
+HADRON_DTYPE = np.dtype(
+    [("pid", "<i4"), ("mass", "<f4"), ("E", "<f4"), ("px", "<f4"), ("py", "<f4"),
+     ("pz", "<f4"), ("t", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
+
+# field order of a rest-frame cell record (FO_surf_LRF, src/data_struct.h:66-76)
+LRF_FIELDS = ["tau", "xpt", "ypt", "eta", "da0", "da1", "da2", "da3", "u0", "u1", "u2", "u3",
+              "Edec", "Tdec", "Pdec", "Bn", "muB", "muS", "muQ", "bulkPi",
+              "pixx", "pixy", "pixz", "piyy", "piyz", "qx", "qy", "qz"]
+# field order of a lab-frame cell record (FO_surf, src/data_struct.h:53-63)
+LAB_FIELDS = ["tau", "xpt", "ypt", "eta", "da0", "da1", "da2", "da3", "u0", "u1", "u2", "u3",
+              "Edec", "Tdec", "Pdec", "Bn", "muB", "muS", "muQ",
+              "pi00", "pi01", "pi02", "pi03", "pi11", "pi12", "pi13", "pi22", "pi23", "pi33",
+              "bulkPi", "qmu0", "qmu1", "qmu2", "qmu3"]
+
+
+class ISSError(RuntimeError):
+    pass
+
+
+class _Species(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pid", C.c_void_p), ("mass", C.c_void_p),
+                ("degeneracy", C.c_void_p), ("baryon", C.c_void_p), ("strange", C.c_void_p),
+                ("charge", C.c_void_p), ("sign", C.c_void_p)]
+
+
+class _Deltaf(C.Structure):
+    _fields_ = [("include_shear", C.c_int32), ("include_bulk", C.c_int32),
+                ("bulk_kind", C.c_int32), ("include_diffusion", C.c_int32),
+                ("k11_nT", C.c_int32), ("k11_nmu", C.c_int32),
+                ("k11_T0", C.c_double), ("k11_mu0", C.c_double),
+                ("k11_dT", C.c_double), ("k11_dmu", C.c_double),
+                ("k11_c0", C.c_void_p), ("k11_c1", C.c_void_p), ("k11_c2", C.c_void_p),
+                ("neos_ncol", C.c_int32), ("neos", C.c_void_p),
+                ("fermion_cdf2_quirk", C.c_int32)]
+
+
+class _Eos(C.Structure):
+    _fields_ = [("iEOS", C.c_int32), ("n_rows", C.c_int64), ("rows", C.c_void_p)]
+
+
+class _YieldOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("hydro_mode", C.c_int32),
+                ("y_LB", C.c_double), ("y_RB", C.c_double)]
+
+
+class _SampleOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("event0", C.c_int64), ("n_events", C.c_int32),
+                ("proposal", C.c_int32), ("sampling_model", C.c_int32),
+                ("hydro_mode", C.c_int32), ("y_LB", C.c_double), ("y_RB", C.c_double),
+                ("max_trials", C.c_int32), ("precision", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_hadrons", C.c_int64), ("n_trials", C.c_int64), ("n_inner", C.c_int64),
+                ("n_failed", C.c_int64), ("n_range_errors", C.c_int64),
+                ("ms_yield", C.c_double), ("ms_sample", C.c_double),
+                ("ms_sample_kernel", C.c_double), ("ms_yield_kernel", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol include/iss_hip.h declares, with its ctypes signature
+_SIGNATURES = {
+    "iss_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "iss_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "iss_last_error": (C.c_char_p, [C.c_void_p]),
+    "iss_set_species": (C.c_int, [C.c_void_p, C.POINTER(_Species)]),
+    "iss_set_deltaf": (C.c_int, [C.c_void_p, C.POINTER(_Deltaf)]),
+    "iss_set_surface_lab": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(_Eos)]),
+    "iss_set_surface_lrf": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "iss_num_cells": (C.c_int64, [C.c_void_p]),
+    "iss_get_surface_lrf": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "iss_compute_yields": (C.c_int, [C.c_void_p, C.POINTER(_YieldOpts), C.c_void_p, C.c_void_p]),
+    "iss_sample": (C.c_int, [C.c_void_p, C.POINTER(_SampleOpts), C.POINTER(C.c_int64)]),
+    "iss_get_event_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "iss_get_hadrons": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "iss_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libiss_hip.so; raise if it is not built (there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ISSError(f"HIP library {path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context of the sampler (wraps iss_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.iss_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise ISSError(f"iss_ctx_create failed with status {rc}")
+        self.h = h
+        self.n_species = 0
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.iss_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.iss_last_error(self.h)
+            raise ISSError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    # ---------------------------------------------------------------- tables
+    def set_species(self, sp: dict):
+        arrs = {k: np.ascontiguousarray(sp[k], dtype=np.float64 if k == "mass" else np.int32)
+                for k in ("pid", "mass", "g", "B", "S", "Q", "sign")}
+        d = _Species(len(arrs["pid"]), _ptr(arrs["pid"]), _ptr(arrs["mass"]), _ptr(arrs["g"]),
+                     _ptr(arrs["B"]), _ptr(arrs["S"]), _ptr(arrs["Q"]), _ptr(arrs["sign"]))
+        self._check(self.lib.iss_set_species(self.h, C.byref(d)), "iss_set_species")
+        self.n_species = len(arrs["pid"])
+        self.species = arrs
+
+    def set_deltaf(self, shear=0, bulk=0, bulk_kind=1, k11=None, neos=None,
+                   fermion_cdf2_quirk=1):
+        d = _Deltaf()
+        d.include_shear, d.include_bulk, d.bulk_kind = int(shear), int(bulk), int(bulk_kind)
+        d.include_diffusion = 0
+        d.fermion_cdf2_quirk = int(fermion_cdf2_quirk)
+        keep = []
+        if k11 is not None:
+            d.k11_nT, d.k11_nmu = k11["nT"], k11["nmu"]
+            d.k11_T0, d.k11_mu0, d.k11_dT, d.k11_dmu = k11["T0"], k11["mu0"], k11["dT"], k11["dmu"]
+            for nm in ("c0", "c1", "c2"):
+                a = np.ascontiguousarray(k11[nm], dtype=np.float64)
+                keep.append(a)
+                setattr(d, "k11_" + nm, _ptr(a))
+        if neos is not None:
+            a = np.ascontiguousarray(neos, dtype=np.float64)
+            keep.append(a)
+            d.neos_ncol = a.shape[1]
+            d.neos = _ptr(a)
+        self._check(self.lib.iss_set_deltaf(self.h, C.byref(d)), "iss_set_deltaf")
+
+    # -------------------------------------------------------------- surfaces
+    def set_surface_lrf(self, cells28: np.ndarray):
+        a = np.ascontiguousarray(cells28, dtype=np.float32).reshape(-1, 28)
+        self._check(self.lib.iss_set_surface_lrf(self.h, a.shape[0], _ptr(a)), "iss_set_surface_lrf")
+
+    def set_surface_lab(self, cells34: np.ndarray, eos: dict | None = None):
+        a = np.ascontiguousarray(cells34, dtype=np.float32).reshape(-1, 34)
+        e = None
+        if eos is not None:
+            rows = np.ascontiguousarray(eos["rows"], dtype=np.float64)
+            self._keep = [rows]
+            e = C.byref(_Eos(int(eos["iEOS"]), rows.shape[0], _ptr(rows)))
+        self._check(self.lib.iss_set_surface_lab(self.h, a.shape[0], _ptr(a), e), "iss_set_surface_lab")
+
+    def num_cells(self) -> int:
+        return int(self.lib.iss_num_cells(self.h))
+
+    def get_surface_lrf(self) -> np.ndarray:
+        out = np.zeros((self.num_cells(), 28), dtype=np.float32)
+        self._check(self.lib.iss_get_surface_lrf(self.h, _ptr(out)), "iss_get_surface_lrf")
+        return out
+
+    # ---------------------------------------------------------------- yields
+    def compute_yields(self, precision=32, per_cell=False, hydro_mode=2, y_LB=-5.0, y_RB=5.0):
+        tot = np.zeros(self.n_species, dtype=np.float64)
+        pc = None
+        if per_cell:
+            pc = np.zeros((self.num_cells(), self.n_species),
+                          dtype=np.float64 if precision == 64 else np.float32)
+        o = _YieldOpts(precision, hydro_mode, y_LB, y_RB)
+        self._check(self.lib.iss_compute_yields(self.h, C.byref(o), _ptr(tot), _ptr(pc)),
+                    "iss_compute_yields")
+        return (tot, pc) if per_cell else tot
+
+    # -------------------------------------------------------------- sampling
+    def sample(self, n_events, seed=1, event0=0, hydro_mode=2, y_LB=-5.0, y_RB=5.0,
+               max_trials=5000, precision=32, proposal=0, sampling_model=30) -> int:
+        o = _SampleOpts(seed, event0, n_events, proposal, sampling_model, hydro_mode, y_LB, y_RB,
+                        max_trials, precision)
+        n = C.c_int64()
+        self._check(self.lib.iss_sample(self.h, C.byref(o), C.byref(n)), "iss_sample")
+        self.n_events = n_events
+        return int(n.value)
+
+    def event_counts(self) -> np.ndarray:
+        out = np.zeros(self.n_events, dtype=np.int64)
+        self._check(self.lib.iss_get_event_counts(self.h, _ptr(out)), "iss_get_event_counts")
+        return out
+
+    def hadrons(self) -> np.ndarray:
+        n = self.stats().n_hadrons
+        out = np.zeros(n, dtype=HADRON_DTYPE)
+        self._check(self.lib.iss_get_hadrons(self.h, _ptr(out), n), "iss_get_hadrons")
+        return out
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._check(self.lib.iss_get_stats(self.h, C.byref(s)), "iss_get_stats")
+        return s

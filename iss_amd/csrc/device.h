@@ -1,0 +1,242 @@
+// Device helpers for the gfx950 Cooper-Frye sampler: counter-based RNG,
+// exact Poisson / binomial variates, a keyed permutation, wave64 scans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace issdev {
+
+constexpr int kWave = 64;
+constexpr double kHbarC = 0.197327053;  // GeV fm (src/data_struct.h:9)
+constexpr double kPi = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random numbers:
+// as easy as 1, 2, 3"): 10 rounds of the published multipliers and Weyl keys.
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// uniform in (0, 1) with 24 random bits, exactly representable in float
+__device__ __forceinline__ float u24(uint32_t x) {
+    return (static_cast<float>(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+// uniform in (0, 1) with 32 random bits, in double
+__device__ __forceinline__ double u32d(uint32_t x) {
+    return (static_cast<double>(x) + 0.5) * (1.0 / 4294967296.0);
+}
+// uniform in (0, 1) with 53 random bits from two words
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+    const uint64_t v = (static_cast<uint64_t>(a) << 21) ^ static_cast<uint64_t>(b >> 11);
+    return (static_cast<double>(v & ((1ull << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// A stream of uniforms for one (key, c0, c1, c2) identity: draw n uses the
+// Philox block (c0, c1, c2, n / 4).  Counter-based: any draw of any stream is
+// reproducible from its identity alone.
+struct Stream {
+    uint32_t k0, k1, c0, c1, c2;
+    uint32_t blk;
+    u32x4 buf;
+    int used;
+    __device__ Stream(uint64_t seed, uint32_t a, uint32_t b, uint32_t c)
+        : k0(static_cast<uint32_t>(seed)), k1(static_cast<uint32_t>(seed >> 32)),
+          c0(a), c1(b), c2(c), blk(0), used(4) {}
+    __device__ __forceinline__ uint32_t next_u32() {
+        if (used == 4) {
+            buf = philox4x32(u32x4{c0, c1, c2, blk++}, k0, k1);
+            used = 0;
+        }
+        const uint32_t v = (used == 0) ? buf.x : (used == 1 ? buf.y : (used == 2 ? buf.z : buf.w));
+        used++;
+        return v;
+    }
+    __device__ __forceinline__ double uniform() {
+        const uint32_t a = next_u32();
+        const uint32_t b = next_u32();
+        return u53(a, b);
+    }
+};
+
+// --------------------------------------------------------------- variates
+// Poisson(lambda), exact.  lambda < 10: inversion by sequential search;
+// otherwise PTRS, the transformed rejection with squeeze of W. Hoermann,
+// "The transformed rejection method for generating Poisson random
+// variables", Insurance: Math. Econ. 12 (1993) 39-45.
+__device__ inline int64_t poisson(double lam, Stream& s) {
+    if (!(lam > 0.)) return 0;
+    if (lam < 10.) {
+        const double u = s.uniform();
+        double p = exp(-lam), F = p;
+        int64_t k = 0;
+        while (u > F && k < 1000) {
+            k++;
+            p *= lam / static_cast<double>(k);
+            F += p;
+        }
+        return k;
+    }
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.);
+    for (int it = 0; it < 10000; it++) {
+        const double U = s.uniform() - 0.5;
+        const double V = s.uniform();
+        const double us = 0.5 - fabs(U);
+        const double kd = floor((2. * a / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return static_cast<int64_t>(kd);
+        if (kd < 0. || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <=
+            -lam + kd * loglam - lgamma(kd + 1.))
+            return static_cast<int64_t>(kd);
+    }
+    return static_cast<int64_t>(floor(lam + 0.5));  // unreachable in practice
+}
+
+// Binomial(n, p), exact.  n p < 10 (after p -> min(p, 1-p)): inversion;
+// otherwise BTRS of W. Hoermann, "The generation of binomial random variates",
+// J. Statist. Comput. Simul. 46 (1993) 101-110.
+__device__ inline int64_t binomial(int64_t n, double p, Stream& s) {
+    if (n <= 0 || !(p > 0.)) return 0;
+    if (p >= 1.) return n;
+    const bool flip = p > 0.5;
+    const double pp = flip ? 1. - p : p;
+    const double q = 1. - pp;
+    const double nd = static_cast<double>(n);
+    int64_t k;
+    if (nd * pp < 10.) {
+        const double u = s.uniform();
+        const double r = pp / q;
+        double f = exp(nd * log(q)), F = f;
+        k = 0;
+        while (u > F && k < n) {
+            f *= r * static_cast<double>(n - k) / static_cast<double>(k + 1);
+            k++;
+            F += f;
+        }
+    } else {
+        const double spq = sqrt(nd * pp * q);
+        const double b = 1.15 + 2.53 * spq;
+        const double a = -0.0873 + 0.0248 * b + 0.01 * pp;
+        const double c = nd * pp + 0.5;
+        const double alpha = (2.83 + 5.1 / b) * spq;
+        const double vr = 0.92 - 4.2 / b;
+        const double lpq = log(pp / q);
+        const double m = floor((nd + 1.) * pp);
+        const double h = lgamma(m + 1.) + lgamma(nd - m + 1.);
+        k = static_cast<int64_t>(floor(nd * pp + 0.5));
+        for (int it = 0; it < 10000; it++) {
+            const double u = s.uniform() - 0.5;
+            double v = s.uniform();
+            const double us = 0.5 - fabs(u);
+            const double kd = floor((2. * a / us + b) * u + c);
+            if (kd < 0. || kd > nd) continue;
+            if (us >= 0.07 && v <= vr) { k = static_cast<int64_t>(kd); break; }
+            v = log(v * alpha / (a / (us * us) + b));
+            if (v <= h - lgamma(kd + 1.) - lgamma(nd - kd + 1.) + (kd - m) * lpq) {
+                k = static_cast<int64_t>(kd);
+                break;
+            }
+        }
+    }
+    return flip ? n - k : k;
+}
+
+// ------------------------------------------------------------ permutation
+// A keyed pseudo-random permutation of [0, n): an unbalanced Feistel network
+// on the smallest power-of-two domain >= n, six rounds, with cycle walking.
+struct Permutation {
+    uint64_t n;
+    int lbits, rbits;
+    uint32_t key0, key1;
+    __host__ __device__ static int bits_for(uint64_t n) {
+        int b = 1;
+        while ((1ull << b) < n) b++;
+        return b < 2 ? 2 : b;
+    }
+    __host__ __device__ void init(uint64_t n_in, uint64_t seed) {
+        n = n_in;
+        const int b = bits_for(n_in);
+        lbits = (b + 1) / 2;
+        rbits = b / 2;
+        key0 = static_cast<uint32_t>(seed * 0x9E3779B97F4A7C15ull >> 32);
+        key1 = static_cast<uint32_t>(seed);
+    }
+    __device__ __forceinline__ static uint32_t mix(uint32_t x, uint32_t k) {
+        x ^= k;
+        x ^= x >> 16;
+        x *= 0x7FEB352Du;
+        x ^= x >> 15;
+        x *= 0x846CA68Bu;
+        x ^= x >> 16;
+        return x;
+    }
+    __device__ __forceinline__ uint64_t once(uint64_t v) const {
+        uint64_t L = v >> rbits, R = v & ((1ull << rbits) - 1);
+        int lw = lbits, rw = rbits;
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            const uint32_t f = mix(static_cast<uint32_t>(R) ^ (static_cast<uint32_t>(R >> 32) * 0x9E3779B9u),
+                                   key0 + 0x632BE5ABu * r) ^ key1;
+            const uint64_t nl = R;
+            const uint64_t nr = (L ^ static_cast<uint64_t>(f)) & ((1ull << lw) - 1);
+            L = nl;
+            R = nr;
+            const int t = lw;
+            lw = rw;
+            rw = t;
+        }
+        return (L << rw) | R;
+    }
+    __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
+        uint64_t x = once(v);
+        while (x >= n) x = once(x);
+        return x;
+    }
+};
+
+// ------------------------------------------------------------- wave scans
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T o = __shfl_up(v, d, kWave);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// first index i in [lo, hi) with a[i] > x (a sorted ascending); hi if none
+template <typename T, typename I>
+__device__ __forceinline__ I upper_bound(const T* a, I lo, I hi, T x) {
+    while (lo < hi) {
+        const I mid = lo + (hi - lo) / 2;
+        if (a[mid] > x) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+}  // namespace issdev

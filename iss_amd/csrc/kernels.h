@@ -1,0 +1,98 @@
+// Device data layout and kernels of the gfx950 Cooper-Frye sampler.
+//
+// HBM layout (one context):
+//   cells   float  [kCellFields][Nc]  rest-frame surface, field-major (SoA)
+//   coef    double [6][Nc]            per-cell delta-f coefficients
+//   cdf     double [Nc][Ns]           per-cell inclusive species prefix of the
+//                                     clamped yields (the species-selection CDF)
+//   ctot    double [Nc]               per-cell total yield (last cdf entry)
+//   ktab    double [3][kKLen]         K1, K2, K3 at x = 0.5 + 0.05 i
+//   ktabf   float2 [3][kKLen]         (K_i, K_{i+1} - K_i) for the FP32 pass
+//   ptab    double [6][3][2001]       |p|-sampler CDF_0/1/2 tables
+#pragma once
+
+#include "device.h"
+
+namespace issdev {
+
+// field index in the rest-frame cell record (FO_surf_LRF order,
+// src/data_struct.h:66-76)
+enum CellField {
+    F_TAU = 0, F_X, F_Y, F_ETA,
+    F_DA0, F_DA1, F_DA2, F_DA3,
+    F_U0, F_U1, F_U2, F_U3,
+    F_E, F_T, F_P, F_NB, F_MUB, F_MUS, F_MUQ, F_PI,
+    F_PIXX, F_PIXY, F_PIXZ, F_PIYY, F_PIYZ,
+    F_QX, F_QY, F_QZ,
+    kCellFields
+};
+
+// special-function table (src/FSSW.cpp:1609-1615)
+constexpr double kKx0 = 0.5, kKdx = 0.05, kKxmax = 400.;
+constexpr int kKLen = 7991;
+constexpr int kKLds = 1024;      // rows staged in LDS (args up to 51.65)
+
+constexpr int kMaxSpecies = 512;
+constexpr int kMaxChunks = kMaxSpecies / kWave;
+constexpr int kPTabLen = 2001;
+
+enum BulkMode { BULK_NONE = 0, BULK_RTA = 1, BULK_14MOM = 11, BULK_22MOM = 20, BULK_CE = 21 };
+
+struct SpeciesDev {
+    const double* mass;
+    const int* g;
+    const int* B;
+    const int* S;
+    const int* Q;
+    const int* sign;
+    const int* pid;
+};
+
+struct PTabDev {
+    // per table: first E~, step, length, m0, truncation, fermion flag
+    double Et0[6], dEt[6], m0[6];
+    int len[6], trunc[6], fermion[6];
+    int quirk;
+    const double* C;  // [6][3][kPTabLen]
+};
+
+struct YieldArgs {
+    const float* cells;
+    const double* coef;
+    int64_t Nc;
+    int Ns;
+    SpeciesDev sp;
+    const double* ktab;
+    const float2* ktabf;
+    int bulk;        // BulkMode, BULK_NONE when bulk delta f is off
+    double* cdf;
+    double* ctot;
+    void* raw;       // optional per-(cell, species) yields in working precision
+    double* partial; // [gridDim.x][Ns]
+};
+
+struct SampleArgs {
+    const float* cells;
+    const double* coef;
+    int64_t Nc;
+    int Ns;
+    SpeciesDev sp;
+    PTabDev pt;
+    const double* cdf;
+    const double* ctot;
+    const int64_t* off;      // [Nc + 1] exclusive prefix of per-cell counts
+    const int64_t* evoff;    // [E + 1]
+    int E;
+    int64_t K;               // hadrons in this pass
+    uint64_t seed;
+    uint32_t chunk;
+    Permutation perm;
+    int shear, bulk, neos;   // neos: -1 none, 1 CE, 0 22-moment
+    int boost_invariant;
+    float yLB, yRB;
+    int max_trials;
+    void* out;               // iss_hadron[K]
+    unsigned long long* stats;  // trials, inner, failed, range errors
+};
+
+}  // namespace issdev

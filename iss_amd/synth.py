@@ -1,0 +1,157 @@
+"""Synthetic freeze-out surfaces for the benchmark and tests.
+
+Configurations (all seeded, generated in memory):
+  C1  plain ideal, boost-invariant constant-tau surface: tau = 10 fm on a
+      40 x 50 (x, y) grid, dx = dy = 0.3 fm, radial flow u_r = gamma v with
+      v = 0.6 tanh(r / 7 fm), T = 0.150 GeV, mu_B = 0.
+  C2  viscous 3+1D: N cells with tau in [0.6, 15] fm, eta_s in [-5, 5],
+      T in [0.135, 0.160] GeV, mu_B in [0, 0.3] GeV, |u_perp| <= 3, random
+      traceless transverse pi^{mu nu} with |pi|/(e+P) <= 0.3, bulk
+      Pi/(e+P) in [-0.2, 0], normalisation such that <hadrons/event> ~ 1e4.
+  C5  as C2 with 2,000,000 cells and mu_B up to 0.45 GeV.
+
+Cells are returned as rest-frame records (28 floats, FO_surf_LRF order) or,
+for C1, as lab-frame MUSIC records (34 floats, FO_surf order).  e, P, n_B are
+those of an ideal hadron-resonance gas of the same species list.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import kn
+
+HBARC = 0.197327053
+
+
+def hrg_thermo(T, muB, sp, nterms=10):
+    """(e [GeV/fm^3], P [GeV/fm^3], n_B [1/fm^3]) of an ideal BE/FD gas of the
+    species `sp` (mass, g, B, sign) as Bessel series; T, muB arrays."""
+    T = np.asarray(T, dtype=np.float64)
+    muB = np.asarray(muB, dtype=np.float64)
+    e = np.zeros_like(T)
+    P = np.zeros_like(T)
+    nB = np.zeros_like(T)
+    for m, g, B, s in zip(sp["mass"], sp["g"], sp["B"], sp["sign"]):
+        nt = nterms if m < 0.7 else 1
+        lam = np.exp(B * muB / T)
+        for n in range(1, nt + 1):
+            th = (-s) ** (n - 1) if s != 0 else (1.0 if n == 1 else 0.0)
+            if th == 0.0:
+                continue
+            x = n * m / T
+            f = th * lam ** n
+            k2 = kn(2, x)
+            k1 = kn(1, x)
+            pn = g / (2 * np.pi ** 2) * m * m * T * T / (n * n) * k2
+            P += f * pn
+            e += f * g / (2 * np.pi ** 2) * (3 * m * m * T * T / (n * n) * k2 + m ** 3 * T / n * k1)
+            nB += B * f * g / (2 * np.pi ** 2) * m * m * T / n * k2
+    c3 = HBARC ** 3
+    return e / c3, P / c3, nB / c3
+
+
+def hrg_thermo_fast(T, muB, sp):
+    """hrg_thermo on a 64 x 64 (T, mu_B) grid with bilinear interpolation for
+    large surfaces (relative interpolation error ~1e-5)."""
+    T = np.asarray(T, dtype=np.float64)
+    muB = np.asarray(muB, dtype=np.float64)
+    if T.size < 5000:
+        return hrg_thermo(T, muB, sp)
+    t0, t1 = T.min(), T.max() + 1e-9
+    m0, m1 = muB.min(), muB.max() + 1e-9
+    nt = nm = 64
+    tg = np.linspace(t0, t1, nt)
+    mg = np.linspace(m0, m1, nm)
+    TT, MM = np.meshgrid(tg, mg, indexing="ij")
+    grids = hrg_thermo(TT.ravel(), MM.ravel(), sp)
+    ft = (T - t0) / (t1 - t0) * (nt - 1)
+    fm = (muB - m0) / (m1 - m0) * (nm - 1)
+    it = np.clip(ft.astype(int), 0, nt - 2)
+    im = np.clip(fm.astype(int), 0, nm - 2)
+    a, b = ft - it, fm - im
+    out = []
+    for g in grids:
+        g = g.reshape(nt, nm)
+        out.append((1 - a) * (1 - b) * g[it, im] + a * (1 - b) * g[it + 1, im]
+                   + (1 - a) * b * g[it, im + 1] + a * b * g[it + 1, im + 1])
+    return tuple(out)
+
+
+def _lrf_shear(rng, n, epp, frac_max=0.3):
+    """5 independent components of a traceless symmetric rest-frame pi with
+    sqrt(pi:pi)/(e+P) uniform in [0, frac_max]."""
+    comp = rng.normal(size=(n, 5))
+    xx, xy, xz, yy, yz = comp.T
+    zz = -xx - yy
+    norm = np.sqrt(xx ** 2 + yy ** 2 + zz ** 2 + 2 * (xy ** 2 + xz ** 2 + yz ** 2))
+    scale = rng.uniform(0, frac_max, n) * epp / np.maximum(norm, 1e-12)
+    return comp * scale[:, None]
+
+
+def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=True, shear=True):
+    """C2-style rest-frame surface (28 floats per cell) and the area scale."""
+    rng = np.random.default_rng(seed)
+    n = int(n_cells)
+    c = np.zeros((n, 28), dtype=np.float64)
+    tau = rng.uniform(0.6, 15.0, n)
+    eta = rng.uniform(-5.0, 5.0, n)
+    c[:, 0] = tau
+    c[:, 1] = rng.uniform(-10, 10, n)
+    c[:, 2] = rng.uniform(-10, 10, n)
+    c[:, 3] = eta
+    T = rng.uniform(0.135, 0.160, n)
+    muB = rng.uniform(0.0, muB_max, n)
+    # dsigma in the rest frame: A (1, a), |a| <= 0.3 uniform in the ball
+    a = rng.normal(size=(n, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    a *= 0.3 * rng.uniform(0, 1, n)[:, None] ** (1 / 3)
+    # flow: |u_perp| uniform in [0, 3], u^eta = 0 -> (t, z) components
+    ur = rng.uniform(0, 3.0, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    ux, uy = ur * np.cos(ph), ur * np.sin(ph)
+    utau = np.sqrt(1 + ux ** 2 + uy ** 2)
+    c[:, 8] = utau * np.cosh(eta)
+    c[:, 9] = ux
+    c[:, 10] = uy
+    c[:, 11] = utau * np.sinh(eta)
+    e, P, nB = hrg_thermo_fast(T, muB, sp)
+    c[:, 12], c[:, 13], c[:, 14], c[:, 15], c[:, 16] = e, T, P, nB, muB
+    epp = e + P
+    if bulk:
+        c[:, 19] = -rng.uniform(0, 0.2, n) * epp
+    if shear:
+        c[:, 20:25] = _lrf_shear(rng, n, epp)
+    # area: Poisson mean per cell ~ n_eq * A; choose A for the target
+    dens = P / T  # Boltzmann number density [1/fm^3]
+    w = rng.uniform(0.5, 1.5, n)
+    A = target_hadrons / float(np.sum(dens * w))
+    c[:, 4] = A * w
+    c[:, 5:8] = (A * w)[:, None] * a
+    return c.astype(np.float32), A
+
+
+def ideal_binv_lab(sp=None):
+    """C1 lab-frame boost-invariant surface (34 floats per cell)."""
+    nx, ny, d = 40, 50, 0.3
+    xs = (np.arange(nx) - 19.5) * d
+    ys = (np.arange(ny) - 24.5) * d
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    X, Y = X.ravel(), Y.ravel()
+    n = X.size
+    r = np.hypot(X, Y)
+    v = 0.6 * np.tanh(r / 7.0)
+    gam = 1 / np.sqrt(1 - v ** 2)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cx = np.where(r > 0, X / r, 0.0)
+        cy = np.where(r > 0, Y / r, 0.0)
+    c = np.zeros((n, 34), dtype=np.float64)
+    c[:, 0] = 10.0
+    c[:, 1], c[:, 2] = X, Y
+    c[:, 4] = d * d * 1.0  # dsigma_tau = dx dy deta, times tau in the transform
+    c[:, 8] = gam
+    c[:, 9], c[:, 10] = gam * v * cx, gam * v * cy
+    T = np.full(n, 0.150)
+    c[:, 13] = T
+    if sp is not None:
+        e, P, nB = hrg_thermo(T, np.zeros(n), sp)
+        c[:, 12], c[:, 14] = e, P
+    return c.astype(np.float32)

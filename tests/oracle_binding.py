@@ -1,0 +1,222 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference
+checker (oracle/_ref/libissref.so).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libissref.so")
+TABLE_DIR = os.path.join(ROOT, "iss_amd", "tables")
+FIXTURES = os.path.join(ROOT, "tests", "golden", "ref_fixtures")
+
+_o = None
+
+
+def lib():
+    global _o
+    if _o is None:
+        o = C.CDLL(ORACLE_LIB)
+        v, i, d, s, L = C.c_void_p, C.c_int, C.c_double, C.c_char_p, C.c_long
+        sig = {
+            "orc_create": (v, [s, i, i, i, i, i, i, i, d, d, d, i]),
+            "orc_destroy": (None, [v]),
+            "orc_num_species": (i, [v]),
+            "orc_num_particles": (i, [v]),
+            "orc_species": (None, [v] + [v] * 7),
+            "orc_particles": (None, [v] + [v] * 7),
+            "orc_set_cells_lrf": (None, [v, L, v]),
+            "orc_num_cells": (L, [v]),
+            "orc_get_cells_lrf": (None, [v, v]),
+            "orc_read_surface": (L, [v, s, i, i, i, i, i, s, i]),
+            "orc_get_cells_lab": (None, [v, v]),
+            "orc_surface_tmunu": (None, [v, v, v]),
+            "orc_transform": (L, [L, v, v, v]),
+            "orc_regulate": (None, [L, v]),
+            "orc_yields": (None, [v, v]),
+            "orc_time_yields": (d, [v, L, L, i]),
+            "orc_sample": (L, [v, C.c_uint, i]),
+            "orc_num_failed": (L, [v]),
+            "orc_event_counts": (None, [v, v]),
+            "orc_hadrons": (None, [v, v]),
+            "orc_tmunu_samples": (None, [v, d, v, v]),
+            "orc_bessel_kn": (d, [i, d]),
+            "orc_kn_table": (d, [v, i, d]),
+            "orc_psample": (i, [v, C.c_uint, d, d, d, i, i, v]),
+            "orc_cdf_pick": (None, [v, L, C.c_uint, L, v]),
+            "orc_neos_coeff": (None, [v, d, d, v]),
+            "orc_k11_coeff": (None, [v, d, d, v]),
+            "orc_k1_coeff": (None, [v, d, v]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(o, k)
+            f.restype = r
+            f.argtypes = a
+        _o = o
+    return _o
+
+
+HADRON_DTYPE = np.dtype(
+    [("pid", "<i4"), ("mass", "<f4"), ("E", "<f4"), ("px", "<f4"), ("py", "<f4"),
+     ("pz", "<f4"), ("t", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    def __init__(self, afterburner=1, qs=1, shear=0, bulk=0, bulk_kind=1, hydro_mode=2,
+                 model=30, para1=0.16, y_LB=-5.0, y_RB=5.0, fd_quirk=1, table_dir=TABLE_DIR):
+        self.o = lib()
+        self.h = self.o.orc_create(table_dir.encode(), afterburner, qs, shear, bulk, bulk_kind,
+                                   hydro_mode, model, para1, y_LB, y_RB, fd_quirk)
+        if not self.h:
+            raise RuntimeError("oracle setup failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.o.orc_destroy(self.h)
+            self.h = None
+
+    def species(self):
+        n = self.o.orc_num_species(self.h)
+        out = {k: np.zeros(n, dtype=np.int32) for k in ("pid", "g", "B", "S", "Q", "sign")}
+        out["mass"] = np.zeros(n)
+        self.o.orc_species(self.h, *[_p(out[k]) for k in ("pid", "mass", "g", "B", "S", "Q", "sign")])
+        return out
+
+    def particles(self):
+        n = self.o.orc_num_particles(self.h)
+        out = {k: np.zeros(n, dtype=np.int32) for k in ("pid", "g", "B", "S", "Q", "sign")}
+        out["mass"] = np.zeros(n)
+        self.o.orc_particles(self.h, *[_p(out[k]) for k in ("pid", "mass", "g", "B", "S", "Q", "sign")])
+        return out
+
+    def set_cells_lrf(self, cells):
+        a = np.ascontiguousarray(cells, dtype=np.float32).reshape(-1, 28)
+        self._cells = a
+        self.o.orc_set_cells_lrf(self.h, a.shape[0], _p(a))
+
+    def cells_lrf(self):
+        n = self.o.orc_num_cells(self.h)
+        a = np.zeros((n, 28), dtype=np.float32)
+        self.o.orc_get_cells_lrf(self.h, _p(a))
+        return a
+
+    def read_surface(self, path, binary=0, boost_inv=0, bulk=0, rhob=0, diff=0, eos_file="", iEOS=0):
+        return self.o.orc_read_surface(self.h, path.encode(), binary, boost_inv, bulk, rhob, diff,
+                                       eos_file.encode(), iEOS)
+
+    def yields(self):
+        n = self.o.orc_num_cells(self.h)
+        ns = self.o.orc_num_species(self.h)
+        out = np.zeros((n, ns))
+        self.o.orc_yields(self.h, _p(out))
+        return out
+
+    def sample(self, seed, nev):
+        tot = self.o.orc_sample(self.h, seed, nev)
+        counts = np.zeros(nev, dtype=np.int64)
+        self.o.orc_event_counts(self.h, _p(counts))
+        had = np.zeros(tot, dtype=HADRON_DTYPE)
+        self.o.orc_hadrons(self.h, _p(had))
+        return counts, had
+
+    def surface_tmunu(self):
+        T = np.zeros(16)
+        Q = np.zeros(3)
+        self.o.orc_surface_tmunu(self.h, _p(T), _p(Q))
+        return T, Q
+
+    def psample(self, seed, m, T, mu, sign, n):
+        out = np.zeros(n)
+        rc = self.o.orc_psample(self.h, seed, m, T, mu, sign, n, _p(out))
+        if rc != 0:
+            raise RuntimeError("oracle |p| sampler out of range")
+        return out
+
+
+def kn(n, x):
+    return lib().orc_bessel_kn(n, x)
+
+
+def transform(lab34):
+    a = np.ascontiguousarray(lab34, dtype=np.float32).reshape(-1, 34)
+    out = np.zeros((a.shape[0], 28), dtype=np.float32)
+    kept = np.zeros(a.shape[0], dtype=np.int64)
+    k = lib().orc_transform(a.shape[0], _p(a), _p(out), _p(kept))
+    return out[:k], kept[:k]
+
+
+def regulate(lab34):
+    a = np.array(lab34, dtype=np.float32).reshape(-1, 34)
+    lib().orc_regulate(a.shape[0], _p(a))
+    return a
+
+
+def cdf_pick(vals, seed, ndraw):
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    out = np.zeros(ndraw, dtype=np.int64)
+    lib().orc_cdf_pick(_p(v), v.size, seed, ndraw, _p(out))
+    return out
+
+
+# ------------------------------------------------------------ reference build
+_r = None
+
+
+def ref_lib():
+    """The reference's own GSL-free sources, when they were built here."""
+    global _r
+    if _r is None:
+        if not os.path.exists(REF_LIB):
+            return None
+        r = C.CDLL(REF_LIB)
+        v, i, d, s, L = C.c_void_p, C.c_int, C.c_double, C.c_char_p, C.c_long
+        r.ref_psample.argtypes = [L, d, d, d, i, i, v]
+        r.ref_cdf_pick.argtypes = [v, L, L, L, v]
+        r.ref_read_surface.argtypes = [s, s, s, i, i, i, i, i, i]
+        r.ref_read_surface.restype = L
+        r.ref_get_cells.argtypes = [v]
+        r.ref_num_particles.restype = L
+        r.ref_particles.argtypes = [v] * 7
+        _r = r
+    return _r
+
+
+# one-cell reference integration tests (tests/RunAutoTest*.sh): surface file,
+# music_input EoS id, parameter-file switches
+REF_CASES = {
+    "Ideal1": dict(surface="testIdealOneFluidCell1.dat", iEOS=91, params="ideal", rhob=0),
+    "Ideal2": dict(surface="testIdealOneFluidCell2.dat", iEOS=9, params="ideal", rhob=0),
+    "Ideal3": dict(surface="testIdealOneFluidCell3.dat", iEOS=12, params="ideal", rhob=1),
+    "Ideal4": dict(surface="testIdealOneFluidCell4.dat", iEOS=14, params="ideal", rhob=1),
+    "Viscous1": dict(surface="testViscousOneFluidCell1.dat", iEOS=91, params="CE", rhob=0),
+    "Viscous2": dict(surface="testViscousOneFluidCell2.dat", iEOS=9, params="CE", rhob=0),
+}
+
+
+def case_config(name):
+    """afterburner, delta-f switches and EoS file of a reference test case."""
+    c = REF_CASES[name]
+    ieos = c["iEOS"]
+    # ideal params: afterburner_type = 1, bulk delta f off, shear on (no pi in the cells)
+    # CE params: afterburner_type = 2, shear + bulk, kind 21; EOS 9 / 91 force UrQMD / SMASH
+    ab = 1 if c["params"] == "ideal" else 2
+    if ieos == 9:
+        ab = 1
+    if ieos == 91:
+        ab = 2
+    if c["params"] == "ideal":
+        df = dict(shear=1, bulk=0, bulk_kind=1)
+    else:
+        df = dict(shear=1, bulk=1, bulk_kind=21)
+    stem = {9: "HRGEOS_PST-", 91: "HRGEOS_PST-", 12: "HRGNEOS_B-", 14: "HRGNEOS_BQS-"}[ieos]
+    suffix = "SMASH.dat" if ab == 2 else "urqmd_v3.3+.dat"
+    eos = os.path.join(TABLE_DIR, "EOS_tables", stem + suffix)
+    return dict(afterburner=ab, eos_file=eos, iEOS=ieos, bulk_read=1, rhob=c["rhob"],
+                surface=os.path.join(FIXTURES, c["surface"]), **df)

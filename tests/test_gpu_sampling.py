@@ -1,0 +1,201 @@
+"""GPU sampler vs the reference's integration criterion and the CPU oracle.
+
+* The reference's one-cell tests (tests/RunAutoTest{Ideal1-4,Viscous1-2}.sh):
+  mean |T^{mu nu}_cell - T^{mu nu}_samples| over the 16 components and the
+  net-baryon row below 1e-3 (tests/TestOutputFiles.py:5-13).
+* Statistical agreement with the oracle on a small viscous surface:
+  per-species multiplicities within 3 sigma of E * N_s (chi^2 over species),
+  pT / rapidity / v2 spectra chi^2/ndf <= 1.1 against the oracle's sample,
+  KS p > 0.01 on pT.
+"""
+import numpy as np
+import pytest
+from scipy import stats as sst
+
+import iss_amd
+from iss_amd import synth, tables
+from tests import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_ctx(afterburner, shear, bulk, bulk_kind):
+    ctx = iss_amd.Context(0)
+    sp = tables.species_arrays(afterburner)
+    ctx.set_species(sp)
+    kw = dict(shear=shear, bulk=bulk, bulk_kind=bulk_kind)
+    if bulk_kind == 21:
+        kw["neos"] = tables.read_neos(21, afterburner)
+    if bulk and bulk_kind == 11:
+        kw["k11"] = tables.read_kind11(afterburner)
+    ctx.set_deltaf(**kw)
+    return ctx, sp
+
+
+def tmunu_from_hadrons(h, sp_all, nev, volume):
+    p = np.stack([h["E"], h["px"], h["py"], h["pz"]]).astype(np.float64)
+    w = 1.0 / p[0]
+    T = np.array([[np.dot(p[i] * w, p[j]) for j in range(4)] for i in range(4)]) / (nev * volume)
+    Bmap = dict(zip(sp_all["pid"].tolist(), sp_all["B"].tolist()))
+    B = np.array([Bmap[x] for x in h["pid"].tolist()], dtype=np.float64) if h.size else np.zeros(0)
+    return T.ravel(), B.sum() / (nev * volume)
+
+
+@pytest.mark.parametrize("case", list(ob.REF_CASES))
+def test_reference_tmunu_reconstruction(case):
+    cfg = ob.case_config(case)
+    ctx, sp = _gpu_ctx(cfg["afterburner"], cfg["shear"], cfg["bulk"], cfg["bulk_kind"])
+    lab = tables.read_music_surface(cfg["surface"], bulk=1, rhob=cfg["rhob"])
+    assert lab.shape[0] == 1
+    ctx.set_surface_lab(lab, eos=tables.read_eos(cfg["iEOS"], cfg["afterburner"]))
+    lrf = ctx.get_surface_lrf()
+    volume = lrf[0, 4] / lrf[0, 8]
+    nev = 1000
+    n = ctx.sample(nev, seed=20240 + len(case))
+    st = ctx.stats()
+    assert st.n_failed == 0 and st.n_range_errors == 0
+    h = ctx.hadrons()
+    assert h.size == n and np.all(h["pid"] != 0)
+    Ts, nBs = tmunu_from_hadrons(h, sp, nev, volume)
+    orc = ob.Oracle(afterburner=cfg["afterburner"], shear=cfg["shear"], bulk=cfg["bulk"],
+                    bulk_kind=cfg["bulk_kind"])
+    orc.read_surface(cfg["surface"], bulk=1, rhob=cfg["rhob"], eos_file=cfg["eos_file"],
+                     iEOS=cfg["iEOS"])
+    Tf, Qf = orc.surface_tmunu()
+    diff = np.concatenate([Tf - Ts, [Qf[0] - nBs]])
+    print(case, "hadrons/event", n / nev, "mean|diff|", np.mean(np.abs(diff)))
+    assert np.mean(np.abs(diff)) < 1e-3
+
+
+@pytest.fixture(scope="module")
+def viscous_pair():
+    """GPU and oracle samples of the same small viscous surface."""
+    df = dict(shear=1, bulk=1, bulk_kind=11)
+    ctx, sp = _gpu_ctx(1, **df)
+    cells, _ = synth.viscous_lrf(400, sp, seed=31, target_hadrons=800.0)
+    ctx.set_surface_lrf(cells)
+    tot = ctx.compute_yields(precision=64)
+    nev_g = 100000
+    ctx.sample(nev_g, seed=77, precision=64)
+    hg = ctx.hadrons()
+    cg = ctx.event_counts()
+    orc = ob.Oracle(afterburner=1, **df)
+    orc.set_cells_lrf(cells)
+    nev_o = 10000
+    co, ho = orc.sample(99, nev_o)
+    return dict(sp=sp, tot=tot, hg=hg, cg=cg, nev_g=nev_g, ho=ho, co=co, nev_o=nev_o)
+
+
+def test_species_multiplicities_within_3sigma(viscous_pair):
+    d = viscous_pair
+    sp, tot, hg, nev = d["sp"], d["tot"], d["hg"], d["nev_g"]
+    idx = {p: i for i, p in enumerate(sp["pid"].tolist())}
+    cnt = np.bincount([idx[p] for p in hg["pid"].tolist()], minlength=len(sp["pid"]))
+    mu = nev * tot
+    sel = mu > 0
+    z = (cnt[sel] - mu[sel]) / np.sqrt(mu[sel])
+    chi2 = np.sum(z ** 2)
+    pval = sst.chi2.sf(chi2, sel.sum())
+    print("species chi2/ndf", chi2 / sel.sum(), "p", pval, "max |z|", np.abs(z).max())
+    assert pval > 1e-3
+    assert np.mean(np.abs(z) > 3) < 0.02
+    assert np.abs(z).max() < 5
+
+
+def test_event_multiplicity_is_poisson(viscous_pair):
+    d = viscous_pair
+    c = d["cg"]
+    lam = d["tot"].sum()
+    assert abs(c.mean() - lam) < 4 * np.sqrt(lam / c.size)
+    assert abs(c.var() / lam - 1) < 0.1
+
+
+def _two_sample_chi2(a, wa, b, wb, bins):
+    """Two-sample chi^2 of histograms of samples a, b drawn from wa and wb
+    events (scaled for unequal sizes); returns (chi2, ndf)."""
+    ha, _ = np.histogram(a, bins=bins)
+    hb, _ = np.histogram(b, bins=bins)
+    k1, k2 = np.sqrt(wb / wa), np.sqrt(wa / wb)
+    m = (ha + hb) > 0
+    chi2 = np.sum((k1 * ha[m] - k2 * hb[m]) ** 2 / (ha[m] + hb[m]))
+    return chi2, m.sum() - 1
+
+
+def _quantile_bins(a, b, lo, hi, per_bin=25, max_bins=60):
+    """equal-probability bins of the pooled sample, >= per_bin oracle counts"""
+    nb = int(np.clip(min(a.size, b.size) // per_bin, 5, max_bins))
+    pool = np.concatenate([a, b])
+    pool = pool[(pool > lo) & (pool < hi)]
+    edges = np.unique(np.quantile(pool, np.linspace(0, 1, nb + 1)))
+    edges[0], edges[-1] = lo, hi
+    return edges
+
+
+def _obs(h):
+    pT = np.hypot(h["px"], h["py"]).astype(np.float64)
+    mT = np.sqrt(pT ** 2 + h["mass"].astype(np.float64) ** 2)
+    y = np.arcsinh(h["pz"] / mT)
+    phi = np.arctan2(h["py"], h["px"])
+    return pT, y, phi
+
+
+GROUPS = {"pion": [211, -211, 111], "kaon": [321, -321, 311, -311], "proton": [2212, -2212],
+          "other": None}
+
+
+def _group_chi2(d, pids):
+    hg, ho = d["hg"], d["ho"]
+    if pids is None:
+        allp = sum((v for v in GROUPS.values() if v), [])
+        hg = hg[~np.isin(hg["pid"], allp)]
+        ho = ho[~np.isin(ho["pid"], allp)]
+    else:
+        hg = hg[np.isin(hg["pid"], pids)]
+        ho = ho[np.isin(ho["pid"], pids)]
+    pg, yg, fg = _obs(hg)
+    po, yo, fo = _obs(ho)
+    chi, ndf = 0.0, 0
+    for a, b, lo, hi in ((pg, po, 0.0, 10.0), (yg, yo, -12.0, 12.0), (fg, fo, -np.pi, np.pi)):
+        c, n = _two_sample_chi2(a, d["nev_g"], b, d["nev_o"], _quantile_bins(a, b, lo, hi))
+        chi += c
+        ndf += n
+    # v2(pT): <cos 2 phi> in pT bins, difference over the combined error
+    edges = _quantile_bins(pg, po, 0.0, 10.0, per_bin=400, max_bins=12)
+    for k in range(len(edges) - 1):
+        cg = np.cos(2 * fg[(pg >= edges[k]) & (pg < edges[k + 1])])
+        co = np.cos(2 * fo[(po >= edges[k]) & (po < edges[k + 1])])
+        if cg.size > 1 and co.size > 1:
+            chi += (cg.mean() - co.mean()) ** 2 / (cg.var() / cg.size + co.var() / co.size)
+            ndf += 1
+    ks = sst.ks_2samp(pg, po).pvalue
+    return chi, ndf, ks, pg.size, po.size
+
+
+def test_spectra_chi2_and_ks_vs_oracle(viscous_pair):
+    """dN/dy, pT, phi and v2(pT) of pions, kaons, protons and the rest:
+    combined chi^2/ndf <= 1.1; each group within 4 sigma of chi^2/ndf = 1;
+    KS p > 0.01 on every group's pT."""
+    tot_chi, tot_ndf = 0.0, 0
+    for name, pids in GROUPS.items():
+        chi, ndf, ks, ng, no = _group_chi2(viscous_pair, pids)
+        print(name, "chi2/ndf %.3f ndf %d KS p %.3g N %d %d" % (chi / ndf, ndf, ks, ng, no))
+        assert chi / ndf <= 1 + 4 * np.sqrt(2.0 / ndf)
+        assert ks > 0.01
+        tot_chi += chi
+        tot_ndf += ndf
+    print("combined chi2/ndf %.4f ndf %d" % (tot_chi / tot_ndf, tot_ndf))
+    assert tot_chi / tot_ndf <= 1.1
+
+
+def test_same_seed_same_output():
+    ctx, sp = _gpu_ctx(1, 1, 1, 11)
+    cells, _ = synth.viscous_lrf(300, sp, seed=41, target_hadrons=300.0)
+    ctx.set_surface_lrf(cells)
+    ctx.sample(50, seed=5)
+    a = ctx.hadrons().copy()
+    ca = ctx.event_counts().copy()
+    ctx.sample(50, seed=5)
+    b = ctx.hadrons()
+    assert np.array_equal(a, b) and np.array_equal(ca, ctx.event_counts())
+    ctx.sample(50, seed=6)
+    assert not np.array_equal(a, ctx.hadrons()[: a.size]) or a.size != ctx.stats().n_hadrons
