@@ -38,7 +38,7 @@ using namespace issdev;
 // K_n(x) = int_0^inf exp(-x cosh t) cosh(n t) dt by the trapezoid rule.  The
 // reference fills its table with GSL (src/FSSW.cpp:1626-1633) and calls GSL
 // outside [0.5, 399.95] (src/FSSW.cpp:1648-1656); GSL is not used here.
-__device__ __noinline__ double kn_exact(int n, double x) {
+__device__ double kn_exact(int n, double x) {
     const double h = fmin(0.1, 0.25 / sqrt(x));
     const double tpeak = asinh(n / x);
     double sum = 0.5;
@@ -352,45 +352,70 @@ struct KLds<float> {
     using T = float2;
 };
 
+// LDS slot of K_order: K_2 first (every mode), then K_1, then K_3
+__device__ __forceinline__ int kslot(int order) { return order == 2 ? 0 : (order == 1 ? 1 : 2); }
+
 // K_n lookup, FP64: the reference's interpolation verbatim
-// (src/FSSW.cpp:1662-1666); rows below kKLds from LDS when staged
+// (src/FSSW.cpp:1662-1666); rows below kKLds from LDS when staged.  Outside
+// [0.5, 399.95] the reference calls the exact function; the fast variant
+// (EXACT = false) raises *oob instead and the pass is rerun with EXACT = true.
+template <bool EXACT>
 struct KLookup64 {
     const double* lds;   // may be null
     const double* glob;
+    int* oob;
     __device__ __forceinline__ double operator()(int order, double arg) const {
-        if (arg < kKx0 || arg > kKxmax - kKdx) return kn_exact(order, arg);
+        if (arg < kKx0 || arg > kKxmax - kKdx) {
+            if constexpr (EXACT) return kn_exact(order, arg);
+            *oob = 1;
+            return 0.;
+        }
         const int idx = static_cast<int>((arg - kKx0) / kKdx);
         const double fr = (arg - kKx0 - idx * kKdx) / kKdx;
-        const double* t = (lds && idx + 1 < kKLds) ? lds + (order - 1) * kKLds
+        const double* t = (lds && idx + 1 < kKLds) ? lds + kslot(order) * kKLds
                                                    : glob + (order - 1) * kKLen;
         return (1. - fr) * t[idx] + fr * t[idx + 1];
     }
 };
 
 // K_n lookup, FP32: same grid, K_i + f (K_{i+1} - K_i)
+template <bool EXACT>
 struct KLookup32 {
     const float2* lds;
     const float2* glob;
+    int* oob;
     __device__ __forceinline__ float operator()(int order, float arg) const {
-        if (arg < 0.5f || arg > 399.95f) return static_cast<float>(kn_exact(order, arg));
+        if (arg < 0.5f || arg > 399.95f) {
+            if constexpr (EXACT) return static_cast<float>(kn_exact(order, arg));
+            *oob = 1;
+            return 0.f;
+        }
         const float t = (arg - 0.5f) * 20.f;
         const int idx = static_cast<int>(t);
         const float fr = t - static_cast<float>(idx);
-        const float2 v = (idx < kKLds) ? lds[(order - 1) * kKLds + idx] : glob[(order - 1) * kKLen + idx];
+        const float2 v = (idx < kKLds) ? lds[kslot(order) * kKLds + idx] : glob[(order - 1) * kKLen + idx];
         return fmaf(fr, v.y, v.x);
     }
 };
 
 // calculate_dN_analytic + the per-cell combination and clamp
-// (src/FSSW.cpp:645-706, 719-848) for one (cell, species).  `cond` returns
-// (|N_eq| + |dN_bulk|) / |N|, the amplification of rounding errors when the
-// bulk correction nearly cancels the equilibrium yield.
+// (src/FSSW.cpp:645-706, 719-848) for one (cell, species).  `ill` reports
+// (|N_eq| + |dN_bulk|) > 4 |N|: rounding errors are amplified when the bulk
+// correction nearly cancels the equilibrium yield.
+
+// exp(x) and 1/n: correctly rounded in FP64 (parity with the reference);
+// the hardware exp2 / reciprocal in FP32 (within the 1e-5 FP32 budget).
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+__device__ __forceinline__ double frcp(double x) { return 1. / x; }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename R, int MODE, typename KL>
 __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, R T, R beta,
                                        R muB, R muS, R muQ, R dsu, R Pi, const R* c,
-                                       const KL& kl, R* cond) {
+                                       const KL& kl, bool* ill) {
     const R mu = Bq * muB + Sq * muS + Qq * muQ;
-    const R lambda = exp(beta * mu);
+    const R lambda = fexp(beta * mu);
     const int nt = (mass < R(0.7) && T > R(0.05)) ? 10 : 1;
     const R mb = mass * beta;
     R r0 = 0, r1 = 0, r2 = 0, r3 = 0;
@@ -398,19 +423,24 @@ __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, 
 #pragma unroll 1
     for (int n = 1; n <= nt; n++) {
         const R rn = static_cast<R>(n);
+        const R inv = (sizeof(R) == sizeof(double)) ? R(1) : frcp(rn);
         const R arg = rn * mass * beta;
         const R K2 = kl(2, arg);
-        r0 += th / rn * fug * K2;
+        // FP64: theta / n as the reference divides; FP32: times 1/n
+        const R thn = (sizeof(R) == sizeof(double)) ? th / rn : th * inv;
+        const R k2n = (sizeof(R) == sizeof(double)) ? 3 * K2 / rn : 3 * K2 * inv;
+        r0 += thn * fug * K2;
         if (MODE == BULK_RTA || MODE == BULK_CE) {
             const R K1 = kl(1, arg);
-            r1 += th * fug * (mb * K1 + 3 * K2 / rn);
+            r1 += th * fug * (mb * K1 + k2n);
             r2 += th * fug * K1;
         } else if (MODE == BULK_14MOM || MODE == BULK_22MOM) {
             const R K1 = kl(1, arg);
             const R K3 = kl(3, arg);
+            const R k3n = (sizeof(R) == sizeof(double)) ? 3 * K3 / rn : 3 * K3 * inv;
             r1 += th * fug * K2;
-            r2 += th * fug * (mb * K1 + 3 * K2 / rn);
-            r3 += th * fug * (mb * K2 + 3 * K3 / rn);
+            r2 += th * fug * (mb * K1 + k2n);
+            r3 += th * fug * (mb * K2 + k3n);
         }
         th *= -sgn;
         fug *= lambda;
@@ -432,118 +462,206 @@ __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, 
         dN = pd * Pi * (b1 * m2 * c[2] + b2 * (Bq * c[3] + Sq * c[4] + Qq * c[5]) + b3 * (c[1] - c[2]));
     }
     const R N = Neq + dN;
-    if (cond) *cond = (fabs(Neq) + fabs(dN)) / fmax(fabs(N), R(1e-30) * (fabs(Neq) + R(1e-300)));
+    // ill-conditioned: the bulk correction nearly cancels N_eq
+    if (ill) *ill = (fabs(Neq) + fabs(dN)) > R(4) * fabs(N);
     return N > R(0) ? N : R(0);
 }
 
-// FP32 yields whose bulk correction nearly cancels N_eq (cond > 4) are
-// recomputed in FP64 so that every FP32 yield stays within 1e-5 of FP64.
-constexpr float kCondFp64 = 4.f;
+// FP32 yields whose bulk correction nearly cancels N_eq ((|N_eq| + |dN|) >
+// 4 |N|) are recomputed in FP64 so that every FP32 yield stays within 1e-5
+// of FP64.
 
-// One wavefront per cell (persistent over cells): lanes walk the species in
-// chunks of 64, the FP64 inclusive species prefix is a wave scan carried
-// across chunks and stored as the cell's species CDF row; species and the
-// first kKLds rows of the K tables are staged in LDS once per workgroup;
-// per-species FP64 partial sums stay in registers until one LDS reduction
-// per workgroup.
-template <typename R, int MODE, bool RAW>
-__global__ __launch_bounds__(256) void k_yield(YieldArgs a) {
+// FP64 yield of one (cell, species) from global memory only: the fallback
+// for ill-conditioned FP32 entries (kept out of line: it is rare).
+template <int MODE, bool EXACT>
+__device__ double yield_fp64_global(const YieldArgs& a, int64_t c, int s) {
+    const int64_t Nc = a.Nc;
+    const KLookup64<EXACT> k64{nullptr, a.ktab, a.oob};
+    const double T = a.cells[F_T * Nc + c];
+    double cd[6];
+    for (int k = 0; k < 6; k++) cd[k] = a.coef[k * Nc + c];
+    double Pi = a.cells[F_PI * Nc + c];
+    if (MODE == BULK_RTA) Pi = Pi / kHbarC;
+    const double pref = static_cast<double>(a.sp.g[s]) * (1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi));
+    return yield_one<double, MODE>(a.sp.mass[s], pref, a.sp.B[s], a.sp.S[s], a.sp.Q[s], a.sp.sign[s], T,
+                                   1. / T, (double)a.cells[F_MUB * Nc + c], (double)a.cells[F_MUS * Nc + c],
+                                   (double)a.cells[F_MUQ * Nc + c], (double)a.cells[F_DA0 * Nc + c], Pi, cd,
+                                   k64, (bool*)nullptr);
+}
+
+constexpr int kYieldThreads = 256;
+// ill-conditioned FP32 items recomputed in FP64 after each tile; a tile with
+// more than this many falls back to recomputing inline
+constexpr int kFixupCap = 1024;
+
+__host__ __device__ constexpr int ktables_for(int mode) {
+    return mode == BULK_NONE ? 1 : ((mode == BULK_RTA || mode == BULK_CE) ? 2 : 3);
+}
+
+// LDS bytes of one k_yield workgroup for nsp (padded) species
+__host__ __device__ constexpr size_t yield_lds_bytes(int esz, int ntab, int tc, int nsp) {
+    return static_cast<size_t>(nsp) * (2 * esz + 4)                 // mass, pref, charges
+           + static_cast<size_t>(ntab) * kKLds * 8                  // K rows
+           + static_cast<size_t>(tc) * nsp * esz                    // yield tile
+           + static_cast<size_t>(kYieldThreads / kWave) * nsp * 8   // FP64 row buffers
+           + static_cast<size_t>(tc) * 16 * esz                     // cell parameters
+           + static_cast<size_t>(kFixupCap + 1) * 4;                // deferred FP64 items
+}
+
+__host__ __device__ constexpr int pad_species(int ns) { return (ns + 3) & ~3; }
+
+// Workgroup-tiled yield pass.  A workgroup owns a tile of TC cells at a time
+// (persistent over tiles):
+//   1. items (species s, cell c) in species-major order: 16 consecutive lanes
+//      share a species (LDS broadcast) and the 10-term light species fill
+//      whole waves; raw yields go to an LDS tile;
+//   2. one wave per cell row: lanes take contiguous species blocks, an FP64
+//      local prefix plus one wave scan gives the species CDF, which is staged
+//      in an LDS row buffer and stored coalesced;
+//   3. FP64 column sums of the tile feed per-thread species accumulators,
+//      written once per workgroup as partials.
+template <typename R, int MODE, bool RAW, int TC, bool EXACT>
+__global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NT = ktables_for(MODE);
     using KT = typename KLds<R>::T;
-    __shared__ R s_mass[kMaxSpecies], s_pref[kMaxSpecies];
-    __shared__ char4 s_q[kMaxSpecies];  // B, S, Q, sign
-    __shared__ KT s_k[3 * kKLds];
-    __shared__ double s_red[4][kMaxSpecies];
     const int Ns = a.Ns;
+    const int nsp = pad_species(Ns);
+    R* s_mass = reinterpret_cast<R*>(smem);
+    R* s_pref = s_mass + nsp;
+    char4* s_q = reinterpret_cast<char4*>(s_pref + nsp);
+    KT* s_k = reinterpret_cast<KT*>(s_q + nsp);
+    R* s_tile = reinterpret_cast<R*>(s_k + NT * kKLds);
+    double* s_row = reinterpret_cast<double*>(s_tile + TC * nsp);
+    R* s_cell = reinterpret_cast<R*>(s_row + (kYieldThreads / kWave) * nsp);
+    int* s_fix = reinterpret_cast<int*>(s_cell + TC * 16);  // [0] count, then (cl << 16 | s)
+
+    const int64_t Nc = a.Nc;
     const R uf = static_cast<R>(1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi));
     for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
         s_mass[s] = static_cast<R>(a.sp.mass[s]);
         s_pref[s] = static_cast<R>(a.sp.g[s]) * uf;
         s_q[s] = make_char4(a.sp.B[s], a.sp.S[s], a.sp.Q[s], a.sp.sign[s]);
     }
-    for (int i = threadIdx.x; i < 3 * kKLds; i += blockDim.x) {
-        const int o = i / kKLds, r = i % kKLds;
-        if constexpr (sizeof(KT) == sizeof(double) && sizeof(R) == sizeof(double))
-            s_k[i] = a.ktab[o * kKLen + r];
+    // K_2 first, then K_1, K_3 as the mode needs
+    for (int i = threadIdx.x; i < NT * kKLds; i += blockDim.x) {
+        const int t = i / kKLds, r = i % kKLds;
+        const int o = (t == 0) ? 2 : (t == 1 ? 1 : 3);
+        if constexpr (sizeof(R) == sizeof(double))
+            s_k[i] = a.ktab[(o - 1) * kKLen + r];
         else
-            s_k[i] = a.ktabf[o * kKLen + r];
+            s_k[i] = a.ktabf[(o - 1) * kKLen + r];
     }
-    __syncthreads();
-
+    // this thread's two species accumulators (Ns <= 2 * blockDim)
+    double acc0 = 0., acc1 = 0.;
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = threadIdx.x / kWave;
-    const int nch = (Ns + kWave - 1) / kWave;
-    const int64_t Nc = a.Nc;
-    // per-species FP64 partial sums: one LDS row per wave, lane-owned entries
-    double* acc = s_red[wib];
-    for (int s = lane; s < Ns; s += kWave) acc[s] = 0.;
+    const int64_t ntiles = (Nc + TC - 1) / TC;
+    const int items = TC * Ns;
+    // lane-contiguous species block for the row scan
+    const int per = (Ns + kWave - 1) / kWave;
+    const int s_lo = min(lane * per, Ns), s_hi = min(s_lo + per, Ns);
 
-    const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    int64_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + wib);
-    for (; c < Nc; c += wstride) {
-        const R T = a.cells[F_T * Nc + c];
-        const R muB = a.cells[F_MUB * Nc + c], muS = a.cells[F_MUS * Nc + c],
-                muQ = a.cells[F_MUQ * Nc + c];
-        const R dsu = a.cells[F_DA0 * Nc + c];
-        R Pi = 0;
-        R cc[6] = {0, 0, 0, 0, 0, 0};
-        if (MODE != BULK_NONE) {
-            Pi = a.cells[F_PI * Nc + c];
-            if (MODE == BULK_RTA) Pi = Pi / static_cast<R>(kHbarC);
-#pragma unroll
-            for (int k = 0; k < 6; k++) cc[k] = static_cast<R>(a.coef[k * Nc + c]);
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t c0 = tile * TC;
+        const int nc = static_cast<int>(min<int64_t>(TC, Nc - c0));
+        __syncthreads();  // previous tile's LDS reads are done
+        if (threadIdx.x == 0) s_fix[0] = 0;
+        if (threadIdx.x < nc) {
+            // parameter-major [k][cell]: the 16 lanes of a species read 16
+            // consecutive banks
+            const int64_t c = c0 + threadIdx.x;
+            R* p = s_cell + threadIdx.x;
+            const R T = a.cells[F_T * Nc + c];
+            p[0 * TC] = T;
+            p[1 * TC] = R(1) / T;
+            p[2 * TC] = a.cells[F_MUB * Nc + c];
+            p[3 * TC] = a.cells[F_MUS * Nc + c];
+            p[4 * TC] = a.cells[F_MUQ * Nc + c];
+            p[5 * TC] = a.cells[F_DA0 * Nc + c];
+            R Pi = 0;
+            if (MODE != BULK_NONE) {
+                Pi = a.cells[F_PI * Nc + c];
+                if (MODE == BULK_RTA) Pi = Pi / static_cast<R>(kHbarC);
+                for (int k = 0; k < 6; k++) p[(7 + k) * TC] = static_cast<R>(a.coef[k * Nc + c]);
+            }
+            p[6 * TC] = Pi;
         }
-        const R beta = R(1) / T;
-        double carry = 0.;
-        double* row = a.cdf + c * Ns;
-        for (int j = 0; j < nch; j++) {
-            const int s = j * kWave + lane;
+        __syncthreads();
+        // 1. yields into the LDS tile
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
+            const int s = it / TC, cl = it % TC;
             R y = 0;
-            if (s < Ns) {
+            if (cl < nc) {
+                const R* q0 = s_cell + cl;
+                const R p[13] = {q0[0], q0[TC], q0[2 * TC], q0[3 * TC], q0[4 * TC], q0[5 * TC], q0[6 * TC],
+                                 q0[7 * TC], q0[8 * TC], q0[9 * TC], q0[10 * TC], q0[11 * TC], q0[12 * TC]};
                 const char4 q = s_q[s];
                 if constexpr (sizeof(R) == sizeof(double)) {
-                    const KLookup64 kl{reinterpret_cast<const double*>(s_k), a.ktab};
-                    y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, T, beta,
-                                           muB, muS, muQ, dsu, Pi, cc, kl, (R*)nullptr);
+                    const KLookup64<EXACT> kl{reinterpret_cast<const double*>(s_k), a.ktab, a.oob};
+                    y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, p[0], p[1], p[2],
+                                           p[3], p[4], p[5], p[6], p + 7, kl, (bool*)nullptr);
                 } else {
-                    const KLookup32 kl{reinterpret_cast<const float2*>(s_k), a.ktabf};
-                    R cond = 0;
-                    y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, T, beta,
-                                           muB, muS, muQ, dsu, Pi, cc, kl,
-                                           MODE == BULK_NONE ? (R*)nullptr : &cond);
-                    if (MODE != BULK_NONE && cond > kCondFp64) {
-                        const KLookup64 k64{nullptr, a.ktab};
-                        const double Td = a.cells[F_T * Nc + c];
-                        double cd[6];
-                        for (int k = 0; k < 6; k++) cd[k] = a.coef[k * Nc + c];
-                        double Pd = a.cells[F_PI * Nc + c];
-                        if (MODE == BULK_RTA) Pd = Pd / kHbarC;
-                        y = static_cast<R>(yield_one<double, MODE>(
-                            a.sp.mass[s], static_cast<double>(a.sp.g[s]) *
-                                              (1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi)),
-                            q.x, q.y, q.z, q.w, Td, 1. / Td, (double)a.cells[F_MUB * Nc + c],
-                            (double)a.cells[F_MUS * Nc + c], (double)a.cells[F_MUQ * Nc + c],
-                            (double)a.cells[F_DA0 * Nc + c], Pd, cd, k64, (double*)nullptr));
+                    const KLookup32<EXACT> kl{reinterpret_cast<const float2*>(s_k), a.ktabf, a.oob};
+                    bool ill = false;
+                    y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, p[0], p[1], p[2],
+                                           p[3], p[4], p[5], p[6], p + 7, kl,
+                                           MODE == BULK_NONE ? (bool*)nullptr : &ill);
+                    if (MODE != BULK_NONE && ill) {
+                        const int k = atomicAdd(&s_fix[0], 1);
+                        if (k < kFixupCap) s_fix[1 + k] = (cl << 16) | s;
+                        else y = static_cast<R>(yield_fp64_global<MODE, EXACT>(a, c0 + cl, s));
                     }
                 }
             }
-            const double yd = static_cast<double>(y);
-            const double incl = wave_inclusive_scan(yd) + carry;
-            if (s < Ns) {
-                row[s] = incl;
-                if (RAW) static_cast<R*>(a.raw)[c * Ns + s] = y;
-                acc[s] += yd;
-            }
-            carry = __shfl(incl, kWave - 1, kWave);
+            s_tile[cl * nsp + s] = y;
         }
-        if (lane == 0) a.ctot[c] = carry;
+        __syncthreads();
+        if constexpr (sizeof(R) == sizeof(float) && MODE != BULK_NONE) {
+            // 1b. deferred FP64 recomputation of the ill-conditioned items
+            const int nfix = min(s_fix[0], kFixupCap);
+            for (int k = threadIdx.x; k < nfix; k += blockDim.x) {
+                const int e = s_fix[1 + k];
+                const int cl = e >> 16, sp = e & 0xFFFF;
+                s_tile[cl * nsp + sp] = static_cast<R>(yield_fp64_global<MODE, EXACT>(a, c0 + cl, sp));
+            }
+            if (nfix > 0) __syncthreads();
+        }
+        // 2. species CDF rows: one wave per row
+        double* rowbuf = s_row + wib * nsp;
+        for (int cl = wib; cl < nc; cl += blockDim.x / kWave) {
+            const R* yrow = s_tile + cl * nsp;
+            double run = 0.;
+            for (int s = s_lo; s < s_hi; s++) run += static_cast<double>(yrow[s]);
+            const double excl = wave_inclusive_scan(run) - run;
+            double pre = excl;
+            for (int s = s_lo; s < s_hi; s++) {
+                pre += static_cast<double>(yrow[s]);
+                rowbuf[s] = pre;
+            }
+            const double total = __shfl(excl + run, kWave - 1, kWave);
+            // rowbuf is written and read by this wave only
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int64_t c = c0 + cl;
+            double* row = a.cdf + c * Ns;
+            for (int s = lane; s < Ns; s += kWave) {
+                row[s] = rowbuf[s];
+                if (RAW) static_cast<R*>(a.raw)[c * Ns + s] = yrow[s];
+            }
+            if (lane == 0) a.ctot[c] = total;
+            __builtin_amdgcn_wave_barrier();
+        }
+        // 3. FP64 column sums into the species accumulators
+        for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
+            double v = 0.;
+            for (int cl = 0; cl < nc; cl++) v += static_cast<double>(s_tile[cl * nsp + s]);
+            if (s == static_cast<int>(threadIdx.x)) acc0 += v; else acc1 += v;
+        }
     }
-    __syncthreads();
-    const int nw = blockDim.x / kWave;
-    for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
-        double v = 0.;
-        for (int w = 0; w < nw; w++) v += s_red[w][s];
-        a.partial[static_cast<int64_t>(blockIdx.x) * Ns + s] = v;
-    }
+    for (int s = threadIdx.x; s < Ns; s += blockDim.x)
+        a.partial[static_cast<int64_t>(blockIdx.x) * Ns + s] = (s == static_cast<int>(threadIdx.x)) ? acc0 : acc1;
 }
 
 // FP64 per-species totals: one workgroup per species, fixed summation order
@@ -677,223 +795,300 @@ __global__ void k_multinomial_leaves(const int64_t* __restrict__ leaf, int D, in
 
 // ============================================================ sampling pass
 
-// CDF_0/1/2(E~) of the |p| samplers (src/BosonMomentumSampler.cpp:36-87,
-// src/FermionMomentumSampler.cpp:35-93), including the reference's fermion
-// quirks: CDF_0 divides int sign by int n1, CDF_2 drops the sign (optional).
-__device__ double pcdf(int fermion, int quirk, int trunc, double m, int k, double E) {
-    if (k == 0 && trunc > 5) {
-        if (fermion) return -exp(m) * log((1. + exp(-E)) / (1. + exp(-m)));
-        return exp(m) * log((1. - exp(-E)) / (1. - exp(-m)));
-    }
-    double res = 0., sign = 1.;
-    for (int n = 0; n < trunc; n++) {
-        const int n1 = n + 1;
-        const double em = exp(-m * n), ee = exp((m - E) * n1);
-        if (k == 0) {
-            const double c0 = fermion ? static_cast<double>(static_cast<int>(sign) / n1) : 1. / n1;
-            res += c0 * em * (1. - ee);
-        } else if (k == 1) {
-            const double s = fermion ? sign : 1.;
-            res += s / (n1 * n1) * em * ((m * n1 + 1) - ee * (E * n1 + 1));
-        } else {
-            const double s = (fermion && !quirk) ? sign : 1.;
-            res += s / (static_cast<double>(n1) * n1 * n1) * em *
-                   ((m * n1 * (m * n1 + 2) + 2) - ee * (E * n1 * (E * n1 + 2) + 2));
-        }
-        sign = -sign;
-    }
-    return res;
-}
-
 struct Hadron {
     int pid;
     float mass, E, px, py, pz, t, x, y, z;
 };
 
-// One hadron slot per thread: slot i of the cell-ordered list belongs to cell
-// c (offsets), draws its species from the cell's FP64 species CDF, its event
-// slot through the permutation, and its rest-frame momentum by the
-// reference's nested rejection (src/FSSW.cpp:1852-1966,
-// src/MomentumSamplerBase.cpp:20-58) flattened into one loop: each iteration
-// is one |p| proposal plus, when the |p| test passes, one full outer trial.
-// One Philox block per iteration: word 0 is the 32-bit CDF draw, the other
-// 96 bits give four 24-bit uniforms.
+// CDF_0, CDF_1, CDF_2 of one |p| table at E~ together: the per-term
+// exponentials become powers of exp(m0 - E~) and exp(-m0), so one hadron's
+// normalisation costs two exponentials and a logarithm.
+__device__ void pcdf_all(const PTabDev& pt, int tb, double E, double out[3]) {
+    const int fer = pt.fermion[tb], trunc = pt.trunc[tb], quirk = pt.quirk;
+    const double m = pt.m0[tb];
+    const double r = exp(m - E), em1 = pt.em1[tb];
+    double c0 = 0., c1 = 0., c2 = 0., sign = 1., em = 1., ee = r;
+    for (int n = 0; n < trunc; n++) {
+        const int n1 = n + 1;
+        const double dn = static_cast<double>(n1);
+        if (trunc <= 5)
+            c0 += (fer ? static_cast<double>(static_cast<int>(sign) / n1) : 1. / dn) * em * (1. - ee);
+        const double s1 = fer ? sign : 1.;
+        c1 += s1 / (dn * dn) * em * ((m * dn + 1) - ee * (E * dn + 1));
+        const double s2 = (fer && !quirk) ? sign : 1.;
+        c2 += s2 / (dn * dn * dn) * em * ((m * dn * (m * dn + 2) + 2) - ee * (E * dn * (E * dn + 2) + 2));
+        sign = -sign;
+        em *= em1;
+        ee *= r;
+    }
+    if (trunc > 5) {
+        const double eE = exp(-E);
+        c0 = fer ? -pt.ep[tb] * log((1. + eE) / (1. + em1)) : pt.ep[tb] * log((1. - eE) / (1. - em1));
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+}
+
+// Persistent sampling kernel with per-lane refill.  Each wavefront owns a
+// contiguous range of the cell-ordered hadron slots; a lane that finishes its
+// hadron (accepted, or out of trials) takes the next slot of the range, so
+// the geometric tails of the rejection loops do not idle the other 63 lanes.
+// Slot i of cell c (the j-th of the cell) draws its species from the cell's
+// FP64 species CDF, its event slot through the permutation, and its
+// rest-frame momentum by the reference's nested rejection
+// (src/FSSW.cpp:1852-1966, src/MomentumSamplerBase.cpp:20-58) flattened into
+// one loop: an iteration is one |p| proposal plus, when the |p| test passes,
+// one outer trial.  Iteration `it` of slot (c, j) uses Philox block
+// (c, j, tag, it): word 0 is the 32-bit CDF draw, the other 96 bits give
+// four 24-bit uniforms.  The stream depends on (c, j) only, so the output
+// does not depend on the launch shape.
+// free lanes that trigger a refill: setup then runs at least this wide
+constexpr int kRefillMin = 16;
+
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
     __shared__ unsigned long long s_stats[4];
     if (threadIdx.x < 4) s_stats[threadIdx.x] = 0ull;
     __syncthreads();
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int64_t base = a.K / nwaves, rem = a.K % nwaves;
+    const int64_t beg = wid * base + min(wid, rem);
+    const int64_t end = beg + base + (wid < rem ? 1 : 0);
+    const int64_t Nc = a.Nc;
+    const PTabDev& pt = a.pt;
+    const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
+    const uint32_t tw = tag_word(TAG_TRIAL, a.chunk);
+    const int max_it = a.max_trials * 64;
+
+    int64_t next = beg;
+    int64_t hint = (beg < end) ? upper_bound(a.off + 1, int64_t(0), Nc, beg) : 0;
     unsigned long long n_trial = 0, n_inner = 0, n_fail = 0, n_range = 0;
-    if (i < a.K) {
-        const int64_t Nc = a.Nc;
-        const int64_t c = upper_bound(a.off + 1, int64_t(0), Nc, i);
-        const int64_t j = i - a.off[c];
-        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(j), tag_word(TAG_SETUP, a.chunk));
-        // species from the cell's inclusive prefix (FP64)
-        const double* row = a.cdf + c * a.Ns;
-        const double target = st.uniform() * a.ctot[c];
-        int s = upper_bound(row, 0, a.Ns, target);
-        if (s >= a.Ns) s = a.Ns - 1;
-        const uint64_t p = a.perm(static_cast<uint64_t>(i));
-        // cell record
-        const float T = a.cells[F_T * Nc + c];
-        const float muB = a.cells[F_MUB * Nc + c], muS = a.cells[F_MUS * Nc + c],
-                    muQ = a.cells[F_MUQ * Nc + c];
-        const float Ed = a.cells[F_E * Nc + c], Pd = a.cells[F_P * Nc + c];
-        const float d0 = a.cells[F_DA0 * Nc + c], d1 = a.cells[F_DA1 * Nc + c],
-                    d2 = a.cells[F_DA2 * Nc + c], d3 = a.cells[F_DA3 * Nc + c];
-        const double mass = a.sp.mass[s];
-        const int sign = a.sp.sign[s], Bq = a.sp.B[s], Sq = a.sp.S[s], Qq = a.sp.Q[s];
-        const double mu = fmin(mass, static_cast<double>(Bq * muB + Sq * muS + Qq * muQ));
-        const double Td = T;
-        const double mt = mass / Td, mut = mu / Td, x = mt - mut;
-        const int tb = ((sign == -1) ? 0 : 3) + ((x < 30.) ? 0 : (x < 50. ? 1 : 2));
-        const PTabDev& pt = a.pt;
-        const double aa = mut * mut - mt * mt / 2.;
-        const double mterm = pcdf(pt.fermion[tb], pt.quirk, pt.trunc[tb], pt.m0[tb], 2, x) +
-                             2. * mut * pcdf(pt.fermion[tb], pt.quirk, pt.trunc[tb], pt.m0[tb], 1, x) +
-                             aa * pcdf(pt.fermion[tb], pt.quirk, pt.trunc[tb], pt.m0[tb], 0, x);
+
+    // per-lane hadron state
+    bool active = false;
+    int64_t slot = 0, c = 0;
+    uint32_t jj = 0;
+    int s = 0, tb = 0, i0 = 0, imax = 0, it = 0, trials = 1;
+    double mut = 0., aa = 0., mterm = 0., cmax = 0., rmin0 = 0., x = 0., mu = 0., Td = 1.;
+    float T = 1.f, fm = 0.f, fmu = 0.f, fsg = 0.f, dsfac = 1.f, dfpref = 0.f;
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, Pi = 0.f, Bq = 0.f, Sq = 0.f, Qq = 0.f;
+    float pixx = 0.f, pixy = 0.f, pixz = 0.f, piyy = 0.f, piyz = 0.f;
+    float cf0 = 0.f, cf1 = 0.f, cf2 = 0.f, cf3 = 0.f, cf4 = 0.f, cf5 = 0.f;
+
+    while (true) {
+        // ---- refill lanes without a hadron from this wave's slot range
+        const unsigned long long need = __ballot(!active);
+        if (need && next < end && (__popcll(need) >= kRefillMin || need == ~0ull)) {
+            const int64_t navail = end - next;
+            const int rank = __popcll(need & ((1ull << lane) - 1));
+            const bool take = !active && rank < navail;
+            const unsigned long long took = __ballot(take);
+            if (take) {
+                slot = next + rank;
+                // cell of the slot: exponential then binary search from the hint
+                int64_t lo = hint, step = 1, hi = hint + 1;
+                while (hi < Nc && a.off[hi + 1] <= slot) {
+                    lo = hi;
+                    step <<= 1;
+                    hi = min<int64_t>(Nc - 1, hi + step);
+                }
+                hi = min<int64_t>(hi + 1, Nc);
+                c = upper_bound(a.off + 1, lo, hi, slot);
+                jj = static_cast<uint32_t>(slot - a.off[c]);
+                Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+                const double* row = a.cdf + c * a.Ns;
+                s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
+                if (s >= a.Ns) s = a.Ns - 1;
+                T = a.cells[F_T * Nc + c];
+                const float muB = a.cells[F_MUB * Nc + c], muS = a.cells[F_MUS * Nc + c],
+                            muQ = a.cells[F_MUQ * Nc + c];
+                const float Ed = a.cells[F_E * Nc + c], Pd = a.cells[F_P * Nc + c];
+                d0 = a.cells[F_DA0 * Nc + c]; d1 = a.cells[F_DA1 * Nc + c];
+                d2 = a.cells[F_DA2 * Nc + c]; d3 = a.cells[F_DA3 * Nc + c];
+                const double mass = a.sp.mass[s];
+                const int sign = a.sp.sign[s];
+                Bq = a.sp.B[s]; Sq = a.sp.S[s]; Qq = a.sp.Q[s];
+                mu = fmin(mass, static_cast<double>(Bq * muB + Sq * muS + Qq * muQ));
+                Td = T;
+                const double mt = mass / Td;
+                mut = mu / Td;
+                x = mt - mut;
+                tb = ((sign == -1) ? 0 : 3) + ((x < 30.) ? 0 : (x < 50. ? 1 : 2));
+                aa = mut * mut - mt * mt / 2.;
+                double cm[3];
+                pcdf_all(pt, tb, x, cm);
+                mterm = cm[2] + 2. * mut * cm[1] + aa * cm[0];
+                imax = pt.len[tb] - 1;
+                i0 = static_cast<int>((x - pt.Et0[tb]) / pt.dEt[tb]);
+                fm = static_cast<float>(mass);
+                fmu = static_cast<float>(mu);
+                fsg = static_cast<float>(sign);
+                dsfac = fabsf(d0) + sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
+                dfpref = 1.0f / (2.0f * T * T * (Ed + Pd));
+                if (a.shear) {
+                    pixx = a.cells[F_PIXX * Nc + c]; pixy = a.cells[F_PIXY * Nc + c];
+                    pixz = a.cells[F_PIXZ * Nc + c]; piyy = a.cells[F_PIYY * Nc + c];
+                    piyz = a.cells[F_PIYZ * Nc + c];
+                }
+                if (a.bulk != BULK_NONE) {
+                    Pi = a.cells[F_PI * Nc + c];
+                    if (a.bulk == BULK_RTA) Pi = Pi / static_cast<float>(kHbarC);
+                }
+                if (a.bulk != BULK_NONE || a.neos >= 0) {
+                    cf0 = a.coef[c]; cf1 = a.coef[Nc + c]; cf2 = a.coef[2 * Nc + c];
+                    cf3 = a.coef[3 * Nc + c]; cf4 = a.coef[4 * Nc + c]; cf5 = a.coef[5 * Nc + c];
+                }
+                it = 0;
+                trials = 1;
+                active = true;
+                if (i0 < 0 || i0 >= imax) {
+                    // m/T - mu/T outside the table: the reference exits
+                    // (src/MomentumSamplerBase.cpp:33-43); leave a hole
+                    n_range++;
+                    active = false;
+                    Hadron h{};
+                    static_cast<Hadron*>(a.out)[a.perm(static_cast<uint64_t>(slot))] = h;
+                } else {
+                    const double* e0 = pt.C + (static_cast<int64_t>(tb) * kPTabLen + i0) * 4;
+                    const double* em = pt.C + (static_cast<int64_t>(tb) * kPTabLen + imax) * 4;
+                    rmin0 = e0[2] + 2. * mut * e0[1] + aa * e0[0] - mterm;
+                    cmax = em[2] + 2. * mut * em[1] + aa * em[0] - mterm;
+                }
+            }
+            const int ntook = __popcll(took);
+            next += ntook;
+            if (ntook) hint = __shfl(c, 63 - __clzll(took), kWave);
+        }
+        if (!__any(active)) {
+            if (next >= end) break;
+            continue;
+        }
+        if (!active) continue;
+        // ---- one iteration for this lane's hadron
+        const u32x4 w = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw, static_cast<uint32_t>(it)}, key0, key1);
+        it++;
+        n_inner++;
         const double* C = pt.C + static_cast<int64_t>(tb) * kPTabLen * 4;
-        const int imax = pt.len[tb] - 1;
-        const double Et0 = pt.Et0[tb], dEt = pt.dEt[tb];
+        const double r = u32d(w.x) * cmax;
+        // the bracket comb(lo) <= r < comb(hi = lo + 1) is unique, so any
+        // search reproduces the reference's bisection (inverse_CDF); a 4-ary
+        // search issues its three loads together and halves the latency chain
+        int lo = i0, hi = imax;
+        double rlo = rmin0, rhi = cmax;
         auto comb = [&](int q) {
             const double* e = C + 4 * q;
             return e[2] + 2. * mut * e[1] + aa * e[0] - mterm;
         };
-        const int i0 = static_cast<int>((x - Et0) / dEt);
-        Hadron h;
-        h.pid = a.sp.pid[s];
-        h.mass = static_cast<float>(mass);
-        bool ok = false;
-        if (i0 < 0 || i0 >= imax) {
-            n_range = 1;
-        } else {
-            const double cmax = comb(imax);
-            const double rmin0 = comb(i0);
-            const float dsfac = fabsf(d0) + sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
-            const float dfpref = 1.0f / (2.0f * T * T * (Ed + Pd));
-            const float fm = static_cast<float>(mass), fmu = static_cast<float>(mu);
-            const float fsg = static_cast<float>(sign);
-            float pixx = 0, pixy = 0, pixz = 0, piyy = 0, piyz = 0, Pi = 0;
-            float cf[6] = {0, 0, 0, 0, 0, 0};
+        while (hi - lo > 3) {
+            const int q = (hi - lo) >> 2;
+            const int m1 = lo + q, m2 = lo + 2 * q, m3 = lo + 3 * q;
+            const double v1 = comb(m1), v2 = comb(m2), v3 = comb(m3);
+            if (r < v2) {
+                if (r < v1) { hi = m1; rhi = v1; } else { lo = m1; rlo = v1; hi = m2; rhi = v2; }
+            } else {
+                if (r < v3) { lo = m2; rlo = v2; hi = m3; rhi = v3; } else { lo = m3; rlo = v3; }
+            }
+        }
+        while (hi - lo > 1) {
+            const int mid = (hi + lo) >> 1;
+            const double rm = comb(mid);
+            if (r < rm) { hi = mid; rhi = rm; } else { lo = mid; rlo = rm; }
+        }
+        const double Et0 = pt.Et0[tb], dEt = pt.dEt[tb];
+        double E0 = Et0 + lo * dEt;
+        if (E0 < x) { E0 = x; rlo = 0.; }
+        const double Ehi = Et0 + hi * dEt;
+        const double Etl = E0 + (Ehi - E0) / fmax(1e-16, rhi - rlo) * (r - rlo);
+        const float Ep = static_cast<float>(Td * Etl + mu);
+        const float pm = sqrtf(fmaxf(Ep * Ep - fm * fm, 0.f));
+        const float ratio = (pm / Ep) / (1.f - fm * fm / (2.f * Ep * Ep));
+        bool done = false;
+        if (u24(w.y) <= ratio) {
+            n_trial++;
+            const float phi = 2.f * static_cast<float>(kPi) * u24(w.z);
+            const float cth = 2.f * u24(w.w) - 1.f;
+            const float sth = sqrtf(fmaxf(1.f - cth * cth, 0.f));
+            float sphi, cphi;
+            __sincosf(phi, &sphi, &cphi);
+            const float pTc = pm * sth;
+            const float px = pTc * cphi, py = pTc * sphi, pz = pm * cth;
+            const float p0 = sqrtf(fm * fm + pm * pm);
+            const float pds = p0 * d0 + px * d1 + py * d2 + pz * d3;
+            const float f0 = 1.f / (__expf((p0 - fmu) / T) + fsg);
+            const float omf = 1.f - fsg * f0;
+            float dfs = 0.f, dfb = 0.f;
             if (a.shear) {
-                pixx = a.cells[F_PIXX * Nc + c]; pixy = a.cells[F_PIXY * Nc + c];
-                pixz = a.cells[F_PIXZ * Nc + c]; piyy = a.cells[F_PIYY * Nc + c];
-                piyz = a.cells[F_PIYZ * Nc + c];
+                const float W = px * px * pixx + 2.f * px * py * pixy + 2.f * px * pz * pixz +
+                                py * py * piyy + 2.f * py * pz * piyz + pz * pz * (-pixx - piyy);
+                if (a.neos == 1) dfs = omf * W / (2.f * cf2) / (p0 * T);
+                else if (a.neos == 0) dfs = omf * W * cf0;
+                else dfs = omf * W * dfpref;
             }
-            if (a.bulk != BULK_NONE) {
-                Pi = a.cells[F_PI * Nc + c];
-                if (a.bulk == BULK_RTA) Pi = Pi / static_cast<float>(kHbarC);
+            if (a.bulk == BULK_RTA || a.bulk == BULK_CE) {
+                const float Eo = p0 / T, mo = fm / T;
+                dfb = -omf * cf0 * (mo * mo / (3.f * Eo) - cf1 * Eo) * Pi;
+            } else if (a.bulk == BULK_14MOM) {
+                dfb = omf * Pi * (cf0 * fm * fm + cf1 * Bq * p0 + cf2 * p0 * p0);
+            } else if (a.bulk == BULK_22MOM) {
+                dfb = omf * Pi * (fm * fm * cf2 + p0 * (Bq * cf3 + Sq * cf4 + Qq * cf5) + p0 * p0 * (cf1 - cf2));
             }
-            if (a.bulk != BULK_NONE || a.neos >= 0)
-                for (int k = 0; k < 6; k++) cf[k] = static_cast<float>(a.coef[k * Nc + c]);
-            const uint32_t tw = tag_word(TAG_TRIAL, a.chunk);
-            const uint32_t kc0 = static_cast<uint32_t>(c), kc1 = static_cast<uint32_t>(j);
-            const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
-            const int max_it = a.max_trials * 64;
-            int trials = 1;
-            for (int it = 0; it < max_it && trials < a.max_trials; it++) {
-                const u32x4 w = philox4x32(u32x4{kc0, kc1, tw, static_cast<uint32_t>(it)}, key0, key1);
-                n_inner++;
-                // |p| proposal from the tabulated CDF (inverse_CDF)
-                const double r = u32d(w.x) * cmax;
-                int lo = i0, hi = imax;
-                double rlo = rmin0, rhi = cmax;
-                while (hi - lo > 1) {
-                    const int mid = (hi + lo) >> 1;
-                    const double rm = comb(mid);
-                    if (r < rm) { hi = mid; rhi = rm; } else { lo = mid; rlo = rm; }
+            const float w1 = fminf(fmaxf(pds / p0 / dsfac, 0.f), 1.f);
+            const float w2 = fminf(fmaxf((1.f + dfs + dfb) * 0.5f, 0.f), 1.f);
+            const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
+            const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
+            if (ua < w1 * w2) {
+                // boost to the lab frame (src/FSSW.cpp:2001-2010)
+                const float u0 = a.cells[F_U0 * Nc + c], u1 = a.cells[F_U1 * Nc + c],
+                            u2 = a.cells[F_U2 * Nc + c], u3 = a.cells[F_U3 * Nc + c];
+                const float pdu = px * u1 + py * u2 + pz * u3;
+                const float k1 = pdu / (u0 + 1.f) + p0;
+                const float lx = px + k1 * u1, ly = py + k1 * u2, lz = pz + k1 * u3;
+                const float mT = sqrtf(lx * lx + ly * ly + fm * fm);
+                const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
+                float eta_s, E, pzl;
+                if (a.boost_invariant) {
+                    // y uniform in [y_LB, y_RB], eta_s = y - (y - eta_s)
+                    // (src/FSSW.cpp:1024-1029)
+                    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk) ^ 0x00800000u);
+                    const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
+                    eta_s = rap - (asinhf(lz / mT) - eta);
+                    E = mT * coshf(rap);
+                    pzl = mT * sinhf(rap);
+                } else {
+                    eta_s = eta;
+                    E = sqrtf(mT * mT + lz * lz);
+                    pzl = lz;
                 }
-                double E0 = Et0 + lo * dEt;
-                if (E0 < x) { E0 = x; rlo = 0.; }
-                const double Ehi = Et0 + hi * dEt;
-                const double Etl = E0 + (Ehi - E0) / fmax(1e-16, rhi - rlo) * (r - rlo);
-                const float Ep = static_cast<float>(Td * Etl + mu);
-                const float pm = sqrtf(fmaxf(Ep * Ep - fm * fm, 0.f));
-                const float ratio = (pm / Ep) / (1.f - fm * fm / (2.f * Ep * Ep));
-                if (u24(w.y) > ratio) continue;
-                // outer trial
-                n_trial++;
-                const float phi = 2.f * static_cast<float>(kPi) * u24(w.z);
-                const float cth = 2.f * u24(w.w) - 1.f;
-                const float sth = sqrtf(fmaxf(1.f - cth * cth, 0.f));
-                float sphi, cphi;
-                __sincosf(phi, &sphi, &cphi);
-                const float pTc = pm * sth;
-                const float px = pTc * cphi, py = pTc * sphi, pz = pm * cth;
-                const float p0 = sqrtf(fm * fm + pm * pm);
-                const float pds = p0 * d0 + px * d1 + py * d2 + pz * d3;
-                const float f0 = 1.f / (__expf((p0 - fmu) / T) + fsg);
-                const float omf = 1.f - fsg * f0;
-                float dfs = 0.f, dfb = 0.f;
-                if (a.shear) {
-                    const float W = px * px * pixx + 2.f * px * py * pixy + 2.f * px * pz * pixz +
-                                    py * py * piyy + 2.f * py * pz * piyz + pz * pz * (-pixx - piyy);
-                    if (a.neos == 1) dfs = omf * W / (2.f * cf[2]) / (p0 * T);
-                    else if (a.neos == 0) dfs = omf * W * cf[0];
-                    else dfs = omf * W * dfpref;
-                }
-                if (a.bulk == BULK_RTA || a.bulk == BULK_CE) {
-                    const float Eo = p0 / T, mo = fm / T;
-                    dfb = -omf * cf[0] * (mo * mo / (3.f * Eo) - cf[1] * Eo) * Pi;
-                } else if (a.bulk == BULK_14MOM) {
-                    dfb = omf * Pi * (cf[0] * fm * fm + cf[1] * Bq * p0 + cf[2] * p0 * p0);
-                } else if (a.bulk == BULK_22MOM) {
-                    dfb = omf * Pi * (fm * fm * cf[2] + p0 * (Bq * cf[3] + Sq * cf[4] + Qq * cf[5]) +
-                                      p0 * p0 * (cf[1] - cf[2]));
-                }
-                const float w1 = fminf(fmaxf(pds / p0 / dsfac, 0.f), 1.f);
-                const float w2 = fminf(fmaxf((1.f + dfs + dfb) * 0.5f, 0.f), 1.f);
-                const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
-                const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
-                if (ua < w1 * w2) {
-                    // boost to the lab frame (src/FSSW.cpp:2001-2010)
-                    const float u0 = a.cells[F_U0 * Nc + c], u1 = a.cells[F_U1 * Nc + c],
-                                u2 = a.cells[F_U2 * Nc + c], u3 = a.cells[F_U3 * Nc + c];
-                    const float pdu = px * u1 + py * u2 + pz * u3;
-                    const float k1 = pdu / (u0 + 1.f) + p0;
-                    const float lx = px + k1 * u1, ly = py + k1 * u2, lz = pz + k1 * u3;
-                    const float pT2 = lx * lx + ly * ly;
-                    const float mT = sqrtf(pT2 + fm * fm);
-                    const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
-                    float eta_s, E, pzl;
-                    if (a.boost_invariant) {
-                        // y uniform in [y_LB, y_RB], eta_s = y - (y - eta_s)
-                        // (src/FSSW.cpp:1024-1029)
-                        const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
-                        const float ymeta = asinhf(lz / mT) - eta;
-                        eta_s = rap - ymeta;
-                        E = mT * coshf(rap);
-                        pzl = mT * sinhf(rap);
-                    } else {
-                        eta_s = eta;
-                        E = sqrtf(mT * mT + lz * lz);
-                        pzl = lz;
-                    }
-                    h.E = E; h.px = lx; h.py = ly; h.pz = pzl;
-                    h.t = tau * coshf(eta_s);
-                    h.x = a.cells[F_X * Nc + c];
-                    h.y = a.cells[F_Y * Nc + c];
-                    h.z = tau * sinhf(eta_s);
-                    ok = true;
-                    break;
-                }
+                Hadron h;
+                h.pid = a.sp.pid[s];
+                h.mass = fm;
+                h.E = E; h.px = lx; h.py = ly; h.pz = pzl;
+                h.t = tau * coshf(eta_s);
+                h.x = a.cells[F_X * Nc + c];
+                h.y = a.cells[F_Y * Nc + c];
+                h.z = tau * sinhf(eta_s);
+                static_cast<Hadron*>(a.out)[a.perm(static_cast<uint64_t>(slot))] = h;
+                done = true;
+            } else {
                 trials++;
             }
-            if (!ok) n_fail = 1;
         }
-        if (!ok) {
-            h.pid = 0;
-            h.E = h.px = h.py = h.pz = h.t = h.x = h.y = h.z = 0.f;
+        if (!done && (trials >= a.max_trials || it >= max_it)) {
+            n_fail++;
+            Hadron h{};
+            static_cast<Hadron*>(a.out)[a.perm(static_cast<uint64_t>(slot))] = h;
+            done = true;
         }
-        static_cast<Hadron*>(a.out)[p] = h;
+        if (done) active = false;
     }
     // statistics: wave sums, then one atomic per workgroup
     n_trial = wave_sum(n_trial);
     n_inner = wave_sum(n_inner);
     n_fail = wave_sum(n_fail);
     n_range = wave_sum(n_range);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (lane == 0) {
         atomicAdd(&s_stats[0], n_trial);
         atomicAdd(&s_stats[1], n_inner);
         atomicAdd(&s_stats[2], n_fail);
@@ -971,6 +1166,7 @@ struct iss_ctx {
     bool yields_valid = false;
     int yield_precision = 0;
     DBuf<double> d_cdf, d_ctot, d_partial, d_stot;
+    DBuf<int> d_oob;
     DBuf<char> d_raw;
     // sampling
     DBuf<uint32_t> d_counts, d_evcount;
@@ -980,7 +1176,7 @@ struct iss_ctx {
     int64_t K = 0;
     int E = 0;
     iss_stats stats{};
-    int yield_blocks_per_cu = 3;
+    int yield_grid = 1;
 };
 
 namespace {
@@ -1046,6 +1242,8 @@ int build_ptables(iss_ctx* ctx, int quirk) {
         pt.len[t] = np;
         pt.trunc[t] = tr[t];
         pt.fermion[t] = fer;
+        pt.ep[t] = std::exp(m0s[t]);
+        pt.em1[t] = std::exp(-m0s[t]);
         for (int i = 0; i < np; i++) {
             const double e = Emin + i * dE;
             double* r = &h[(static_cast<size_t>(t) * kPTabLen + i) * 4];
@@ -1113,20 +1311,38 @@ SpeciesDev species_dev(iss_ctx* ctx) {
     return s;
 }
 
+template <typename R, int MODE, bool RAW, bool EXACT>
+int launch_yield_t(iss_ctx* ctx, const YieldArgs& a) {
+    constexpr int TC = 16;
+    const size_t lds = yield_lds_bytes(sizeof(R), ktables_for(MODE), TC, pad_species(a.Ns));
+    auto kern = k_yield<R, MODE, RAW, TC, EXACT>;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds)) != hipSuccess)
+        return ISS_ERR_HIP;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kYieldThreads, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int64_t ntiles = (a.Nc + TC - 1) / TC;
+    const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ntiles, int64_t(ctx->num_cu) * per_cu)));
+    ctx->yield_grid = nblk;
+    kern<<<nblk, kYieldThreads, lds, ctx->stream>>>(a);
+    return hipGetLastError() == hipSuccess ? ISS_OK : ISS_ERR_HIP;
+}
+
 template <typename R, int MODE>
-void launch_yield_t(const YieldArgs& a, int nblk, bool raw, hipStream_t st) {
-    if (raw) k_yield<R, MODE, true><<<nblk, 256, 0, st>>>(a);
-    else k_yield<R, MODE, false><<<nblk, 256, 0, st>>>(a);
+int launch_yield_m(iss_ctx* ctx, const YieldArgs& a, bool raw, bool exact) {
+    if (exact) return raw ? launch_yield_t<R, MODE, true, true>(ctx, a) : launch_yield_t<R, MODE, false, true>(ctx, a);
+    return raw ? launch_yield_t<R, MODE, true, false>(ctx, a) : launch_yield_t<R, MODE, false, false>(ctx, a);
 }
 
 template <typename R>
-void launch_yield(const YieldArgs& a, int nblk, bool raw, hipStream_t st) {
+int launch_yield(iss_ctx* ctx, const YieldArgs& a, bool raw, bool exact) {
     switch (a.bulk) {
-        case BULK_RTA: launch_yield_t<R, BULK_RTA>(a, nblk, raw, st); break;
-        case BULK_CE: launch_yield_t<R, BULK_CE>(a, nblk, raw, st); break;
-        case BULK_14MOM: launch_yield_t<R, BULK_14MOM>(a, nblk, raw, st); break;
-        case BULK_22MOM: launch_yield_t<R, BULK_22MOM>(a, nblk, raw, st); break;
-        default: launch_yield_t<R, BULK_NONE>(a, nblk, raw, st); break;
+        case BULK_RTA: return launch_yield_m<R, BULK_RTA>(ctx, a, raw, exact);
+        case BULK_CE: return launch_yield_m<R, BULK_CE>(ctx, a, raw, exact);
+        case BULK_14MOM: return launch_yield_m<R, BULK_14MOM>(ctx, a, raw, exact);
+        case BULK_22MOM: return launch_yield_m<R, BULK_22MOM>(ctx, a, raw, exact);
+        default: return launch_yield_m<R, BULK_NONE>(ctx, a, raw, exact);
     }
 }
 
@@ -1140,9 +1356,9 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     HIPCHK(ctx->d_cdf.ensure(std::max<int64_t>(1, Nc) * Ns));
     HIPCHK(ctx->d_ctot.ensure(std::max<int64_t>(1, Nc)));
     HIPCHK(ctx->d_stot.ensure(Ns));
-    int nblk = static_cast<int>(std::min<int64_t>((Nc + 3) / 4, int64_t(ctx->num_cu) * ctx->yield_blocks_per_cu));
-    nblk = std::max(nblk, 1);
-    HIPCHK(ctx->d_partial.ensure(static_cast<size_t>(nblk) * Ns));
+    // partials sized for the largest grid the occupancy can give
+    const int max_blk = ctx->num_cu * 8;
+    HIPCHK(ctx->d_partial.ensure(static_cast<size_t>(max_blk) * Ns));
     const size_t esz = precision == 64 ? sizeof(double) : sizeof(float);
     const bool raw = per_cell_host != nullptr;
     if (raw) HIPCHK(ctx->d_raw.ensure(std::max<int64_t>(1, Nc) * Ns * esz));
@@ -1159,15 +1375,29 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     a.ctot = ctx->d_ctot.p;
     a.raw = raw ? static_cast<void*>(ctx->d_raw.p) : nullptr;
     a.partial = ctx->d_partial.p;
+    HIPCHK(ctx->d_oob.ensure(1));
+    a.oob = ctx->d_oob.p;
+    HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    int nblk = 1;
     if (Nc > 0) {
-        if (precision == 64) launch_yield<double>(a, nblk, raw, ctx->stream);
-        else launch_yield<float>(a, nblk, raw, ctx->stream);
-        HIPCHK(hipGetLastError());
+        int rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, false) : launch_yield<float>(ctx, a, raw, false);
+        if (rc) return set_err(ctx, rc, "yield kernel launch failed");
+        HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+        int oob = 0;
+        HIPCHK(hipMemcpyAsync(&oob, ctx->d_oob.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (oob) {
+            // a Bessel argument left the table: rerun with the exact fallback
+            rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, true) : launch_yield<float>(ctx, a, raw, true);
+            if (rc) return set_err(ctx, rc, "yield kernel launch failed");
+            HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+        }
+        nblk = ctx->yield_grid;
     } else {
-        HIPCHK(hipMemsetAsync(ctx->d_partial.p, 0, sizeof(double) * nblk * Ns, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->d_partial.p, 0, sizeof(double) * Ns, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     }
-    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     k_species_reduce<<<Ns, 256, 0, ctx->stream>>>(ctx->d_partial.p, nblk, Ns, ctx->d_stot.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -1223,7 +1453,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_ktab.release(); ctx->d_ktabf.release(); ctx->d_ptab.release();
     ctx->d_cells.release(); ctx->d_tmp.release(); ctx->d_coef.release(); ctx->d_u32.release();
     ctx->d_i64.release(); ctx->d_scan.release(); ctx->d_eos.release(); ctx->d_cdf.release();
-    ctx->d_ctot.release(); ctx->d_partial.release(); ctx->d_stot.release(); ctx->d_raw.release();
+    ctx->d_ctot.release(); ctx->d_partial.release(); ctx->d_oob.release(); ctx->d_stot.release(); ctx->d_raw.release();
     ctx->d_counts.release(); ctx->d_evcount.release(); ctx->d_off.release(); ctx->d_evoff.release();
     ctx->d_tree0.release(); ctx->d_tree1.release(); ctx->d_out.release(); ctx->d_stats.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
@@ -1477,7 +1707,14 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-    if (K > 0) k_sample<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a);
+    if (K > 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sample, 256, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        const int64_t want = (K + 255) / 256;
+        const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
+        k_sample<<<nblk, 256, 0, ctx->stream>>>(a);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     unsigned long long st[4];

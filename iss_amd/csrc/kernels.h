@@ -51,6 +51,7 @@ struct SpeciesDev {
 struct PTabDev {
     // per table: first E~, step, length, m0, truncation, fermion flag
     double Et0[6], dEt[6], m0[6];
+    double ep[6], em1[6];  // exp(m0), exp(-m0)
     int len[6], trunc[6], fermion[6];
     int quirk;
     const double* C;  // [6][3][kPTabLen]
@@ -69,6 +70,7 @@ struct YieldArgs {
     double* ctot;
     void* raw;       // optional per-(cell, species) yields in working precision
     double* partial; // [gridDim.x][Ns]
+    int* oob;        // set when a Bessel argument leaves the table
 };
 
 struct SampleArgs {
