@@ -1,0 +1,101 @@
+"""The drop-in iSS.e run as the reference's integration tests run it
+(tests/RunAutoTest*.sh): copy music_input_<EoS> to music_input, run
+`iSS.e <parameter file> <work dir> <surface file>`, then apply
+tests/TestOutputFiles.py's criterion to checkReconstructedTmunu.dat:
+mean |diff| over the 16 T^{mu nu} components and the net-baryon row < 1e-3.
+
+The ideal tests' parameter file names bulk_deltaf_kind = 20, whose 22-moment
+table is not part of this snapshot; ideal cells carry no Pi and no pi, so the
+kind is overridden to 1 there (it cannot change those samples)."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "iss_amd", "iSS.e")
+FIX = os.path.join(ROOT, "tests", "golden", "ref_fixtures")
+
+RUNS = {  # name: (music_input, parameter file, surface)
+    "Ideal1": ("music_input_91", "iSS_parameters_ideal.dat", "testIdealOneFluidCell1.dat"),
+    "Ideal2": ("music_input_9", "iSS_parameters_ideal.dat", "testIdealOneFluidCell2.dat"),
+    "Ideal3": ("music_input_12", "iSS_parameters_ideal.dat", "testIdealOneFluidCell3.dat"),
+    "Ideal4": ("music_input_14", "iSS_parameters_ideal.dat", "testIdealOneFluidCell4.dat"),
+    "Viscous1": ("music_input_91", "iSS_parameters_CEdeltaf.dat", "testViscousOneFluidCell1.dat"),
+    "Viscous2": ("music_input_9", "iSS_parameters_CEdeltaf.dat", "testViscousOneFluidCell2.dat"),
+}
+
+
+def _run(tmp_path, name, extra=()):
+    mi, params, surf = RUNS[name]
+    work = tmp_path / "tests"
+    work.mkdir()
+    shutil.copy(os.path.join(FIX, mi), work / "music_input")
+    shutil.copy(os.path.join(FIX, surf), work / surf)
+    shutil.copy(os.path.join(FIX, params), tmp_path / params)
+    os.symlink(os.path.join(ROOT, "iss_amd", "tables"), tmp_path / "iSS_tables")
+    args = [EXE, params, "tests", surf, "randomSeed=7", "number_of_repeated_sampling=1000"]
+    if "ideal" in params:
+        args.append("bulk_deltaf_kind=1")
+    args += list(extra)
+    r = subprocess.run(args, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
+    return r
+
+
+@pytest.mark.parametrize("name", list(RUNS))
+def test_dropin_cli_passes_reference_criterion(tmp_path, name):
+    _run(tmp_path, name)
+    data = np.loadtxt(tmp_path / "checkReconstructedTmunu.dat")
+    assert data.shape == (17, 3)
+    res = np.mean(np.abs(data[:, 2]))
+    print(name, "mean |diff|", res)
+    assert res < 1e-3
+
+
+def test_dropin_binary_output_format(tmp_path):
+    """particle_samples.bin: per event int32 N, then per hadron int32 pid and
+    9 float32 (mass, t, x, y, z, E, px, py, pz) (src/FSSW.cpp:529-561)."""
+    _run(tmp_path, "Ideal1", ["use_binary_format=1", "number_of_repeated_sampling=20",
+                              "perform_checks=0"])
+    raw = (tmp_path / "particle_samples.bin").read_bytes()
+    off, nev, ntot = 0, 0, 0
+    while off < len(raw):
+        (n,) = struct.unpack_from("<i", raw, off)
+        off += 4
+        for _ in range(n):
+            pid = struct.unpack_from("<i", raw, off)[0]
+            m, t, x, y, z, E, px, py, pz = struct.unpack_from("<9f", raw, off + 4)
+            assert pid != 0 and E > 0 and abs(E * E - px * px - py * py - pz * pz - m * m) < 1e-3 * E * E
+            off += 40
+        nev += 1
+        ntot += n
+    assert nev == 20 and off == len(raw) and ntot > 20 * 10000
+
+
+def test_dropin_oscar_and_gzip(tmp_path):
+    import gzip
+    _run(tmp_path, "Ideal2", ["use_OSCAR_format=1", "number_of_repeated_sampling=3",
+                              "perform_checks=0"])
+    lines = (tmp_path / "OSCAR.DAT").read_text().splitlines()
+    assert lines[0].startswith("OSC1997A")
+    hdr = [l for l in lines if len(l.split()) == 4]
+    assert len(hdr) >= 3
+    (tmp_path / "tests").rename(tmp_path / "t_old")
+    for f in ("iSS_tables", "iSS_parameters_ideal.dat"):
+        os.remove(tmp_path / f)
+    _run(tmp_path, "Ideal2", ["use_gzip_format=1", "number_of_repeated_sampling=3",
+                              "perform_checks=0"])
+    with gzip.open(tmp_path / "particle_samples.gz", "rt") as g:
+        first = g.readline().split()
+        n = int(first[0])
+        rec = g.readline().split()
+    assert n > 10000 and len(rec) == 10
