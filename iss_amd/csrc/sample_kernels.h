@@ -1,0 +1,462 @@
+// Sampling pass: per-cell Poisson counts, scans, the event multinomial and the momentum sampler.
+// (one translation unit: included by iss_hip.hip)
+#pragma once
+
+#include "kernels.h"
+
+namespace issdev {
+
+// ============================================================ counting pass
+
+enum StreamTag : uint32_t { TAG_COUNT = 1, TAG_MULTI = 2, TAG_SETUP = 3, TAG_TRIAL = 4 };
+
+__device__ __forceinline__ uint32_t tag_word(uint32_t tag, uint32_t chunk) {
+    return (tag << 24) ^ (chunk & 0x00FFFFFFu);
+}
+
+// hadrons of cell c over all E events of this pass: Poisson(E * scale * N_c)
+__global__ void k_counts(const double* __restrict__ ctot, int64_t Nc, double mult, uint64_t seed,
+                         uint32_t chunk, uint32_t* __restrict__ counts) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    Stream st(seed, static_cast<uint32_t>(c), static_cast<uint32_t>(c >> 32), tag_word(TAG_COUNT, chunk));
+    const int64_t k = poisson(mult * ctot[c], st);
+    counts[c] = static_cast<uint32_t>(min<int64_t>(k, 0xFFFFFFFFll));
+}
+
+// ----- exclusive scan of uint32 -> int64 (out[n] = total), 3 phases
+constexpr int kScanBlock = 256, kScanItems = 8, kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+    __shared__ int64_t wsum[kScanBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int64_t incl = wave_inclusive_scan(v);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int64_t pre = 0, tot = 0;
+    for (int i = 0; i < kScanBlock / kWave; i++) {
+        if (i < w) pre += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + incl - v;
+}
+
+__global__ void k_scan_tiles(const uint32_t* __restrict__ in, int64_t n, int64_t* __restrict__ tile_sums) {
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    int64_t v = 0;
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) v += in[base + k];
+    int64_t tot;
+    block_exclusive_scan(v, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_top(int64_t* __restrict__ tile_sums, int64_t ntiles) {
+    // one workgroup: exclusive scan in place, loops over the tile sums
+    __shared__ int64_t carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t b = 0; b < ntiles; b += kScanBlock) {
+        const int64_t i = b + threadIdx.x;
+        const int64_t v = (i < ntiles) ? tile_sums[i] : 0;
+        int64_t tot;
+        const int64_t ex = block_exclusive_scan(v, &tot);
+        const int64_t carry = carry_s;
+        if (i < ntiles) tile_sums[i] = ex + carry;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_sums[ntiles] = carry_s;
+}
+
+__global__ void k_scan_apply(const uint32_t* __restrict__ in, int64_t n,
+                             const int64_t* __restrict__ tile_sums, int64_t* __restrict__ out) {
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    int64_t loc[kScanItems];
+    int64_t v = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        loc[k] = (base + k < n) ? in[base + k] : 0;
+        v += loc[k];
+    }
+    int64_t tot;
+    int64_t ex = block_exclusive_scan(v, &tot) + tile_sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += loc[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_sums[gridDim.x];
+}
+
+// ----- hadrons per event: Multinomial(K, 1/E) as a binomial tree.  Level d
+// has 2^d nodes; node k covers events [floor(kE/2^d), floor((k+1)E/2^d)).
+__global__ void k_multinomial_level(const int64_t* __restrict__ parent, int64_t* __restrict__ child,
+                                    int d, int E, uint64_t seed, uint32_t chunk) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nodes = 1ll << d;
+    if (k >= nodes) return;
+    const int64_t n = parent[k];
+    const int64_t lo = (k * E) >> d, hi = ((k + 1) * E) >> d;
+    const int64_t mid = ((2 * k + 1) * E) >> (d + 1);
+    int64_t nl = 0;
+    if (hi > lo && n > 0) {
+        const double p = static_cast<double>(mid - lo) / static_cast<double>(hi - lo);
+        Stream st(seed, static_cast<uint32_t>(k), static_cast<uint32_t>(d), tag_word(TAG_MULTI, chunk));
+        nl = binomial(n, p, st);
+    }
+    child[2 * k] = nl;
+    child[2 * k + 1] = n - nl;
+}
+
+// leaves (level D, 2^D >= E): node k covers at most one event
+__global__ void k_multinomial_leaves(const int64_t* __restrict__ leaf, int D, int E,
+                                     uint32_t* __restrict__ evcount) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= (1ll << D)) return;
+    const int64_t lo = (k * E) >> D, hi = ((k + 1) * E) >> D;
+    if (hi - lo == 1) evcount[lo] = static_cast<uint32_t>(leaf[k]);
+}
+
+// ============================================================ sampling pass
+
+struct Hadron {
+    int pid;
+    float mass, E, px, py, pz, t, x, y, z;
+};
+
+// CDF_0, CDF_1, CDF_2 of one |p| table at E~ together: the per-term
+// exponentials become powers of exp(m0 - E~) and exp(-m0), so one hadron's
+// normalisation costs two exponentials and a logarithm.
+__device__ void pcdf_all(const PTabDev& pt, int tb, double E, double out[3]) {
+    const int fer = pt.fermion[tb], trunc = pt.trunc[tb], quirk = pt.quirk;
+    const double m = pt.m0[tb];
+    const double r = exp(m - E), em1 = pt.em1[tb];
+    double c0 = 0., c1 = 0., c2 = 0., sign = 1., em = 1., ee = r;
+    for (int n = 0; n < trunc; n++) {
+        const int n1 = n + 1;
+        const double dn = static_cast<double>(n1);
+        if (trunc <= 5)
+            c0 += (fer ? static_cast<double>(static_cast<int>(sign) / n1) : 1. / dn) * em * (1. - ee);
+        const double s1 = fer ? sign : 1.;
+        c1 += s1 / (dn * dn) * em * ((m * dn + 1) - ee * (E * dn + 1));
+        const double s2 = (fer && !quirk) ? sign : 1.;
+        c2 += s2 / (dn * dn * dn) * em * ((m * dn * (m * dn + 2) + 2) - ee * (E * dn * (E * dn + 2) + 2));
+        sign = -sign;
+        em *= em1;
+        ee *= r;
+    }
+    if (trunc > 5) {
+        const double eE = exp(-E);
+        c0 = fer ? -pt.ep[tb] * log((1. + eE) / (1. + em1)) : pt.ep[tb] * log((1. - eE) / (1. - em1));
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+}
+
+// Per-hadron set-up record, written by k_setup, read once by k_trials.
+struct alignas(16) HSetup {
+    uint32_t cell;
+    uint32_t j;       // index of the hadron within its cell (RNG counter)
+    uint16_t species;
+    uint8_t tb;       // |p| table
+    uint8_t flags;    // 1: m/T - mu/T outside the table
+    int32_t i0;       // first table row at or below m~ - mu~
+    double mterm;     // proposal CDF at E~ = m~ - mu~
+    uint64_t out;     // output slot (event-ordered, through the permutation)
+};
+static_assert(sizeof(HSetup) == 32, "HSetup layout");
+
+// hadron-independent quantities of one |p| proposal, rebuilt from the record
+struct PState {
+    double mut, aa, x, mu, Td;
+    float T, fm, fmu, fsg;
+};
+
+__device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s, PState& q) {
+    const int64_t Nc = a.Nc;
+    q.T = a.cells[F_T * Nc + c];
+    const float muB = a.cells[F_MUB * Nc + c], muS = a.cells[F_MUS * Nc + c], muQ = a.cells[F_MUQ * Nc + c];
+    const double mass = a.sp.mass[s];
+    const float Bq = a.sp.B[s], Sq = a.sp.S[s], Qq = a.sp.Q[s];
+    q.mu = fmin(mass, static_cast<double>(Bq * muB + Sq * muS + Qq * muQ));
+    q.Td = q.T;
+    const double mt = mass / q.Td;
+    q.mut = q.mu / q.Td;
+    q.x = mt - q.mut;
+    q.aa = q.mut * q.mut - mt * mt / 2.;
+    q.fm = static_cast<float>(mass);
+    q.fmu = static_cast<float>(q.mu);
+    q.fsg = static_cast<float>(a.sp.sign[s]);
+}
+
+// Set-up of every hadron slot, one thread per slot (no divergence): the cell
+// (offsets), the species (the cell's FP64 species CDF), the |p| table and its
+// normalisation at E~ = m~ - mu~, and the output slot.
+__global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict__ rec) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.K) return;
+    const int64_t Nc = a.Nc;
+    const int64_t c = upper_bound(a.off + 1, int64_t(0), Nc, i);
+    const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
+    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+    const double* row = a.cdf + c * a.Ns;
+    int s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
+    if (s >= a.Ns) s = a.Ns - 1;
+    PState q;
+    pstate_of(a, c, s, q);
+    const int sign = a.sp.sign[s];
+    const int tb = ((sign == -1) ? 0 : 3) + ((q.x < 30.) ? 0 : (q.x < 50. ? 1 : 2));
+    double cm[3];
+    pcdf_all(a.pt, tb, q.x, cm);
+    HSetup r;
+    r.cell = static_cast<uint32_t>(c);
+    r.j = jj;
+    r.species = static_cast<uint16_t>(s);
+    r.tb = static_cast<uint8_t>(tb);
+    r.i0 = static_cast<int32_t>((q.x - a.pt.Et0[tb]) / a.pt.dEt[tb]);
+    r.flags = (r.i0 < 0 || r.i0 >= a.pt.len[tb] - 1) ? 1 : 0;
+    r.mterm = cm[2] + 2. * q.mut * cm[1] + q.aa * cm[0];
+    r.out = a.perm(static_cast<uint64_t>(i));
+    rec[i] = r;
+}
+
+// Persistent trial kernel with per-lane refill.  Each wavefront owns a
+// contiguous range of hadron slots; a lane that finishes its hadron
+// (accepted, or out of trials) loads the next set-up record of the range, so
+// the geometric tails of the rejection loops do not idle the other lanes.
+// The momentum follows the reference's nested rejection
+// (src/FSSW.cpp:1852-1966, src/MomentumSamplerBase.cpp:20-58) flattened into
+// one loop: an iteration is one |p| proposal plus, when the |p| test passes,
+// one outer trial.  Iteration `it` of hadron (c, j) uses Philox block
+// (c, j, tag, it): word 0 is the 32-bit CDF draw, the other 96 bits give four
+// 24-bit uniforms.  Streams depend on (c, j) only, so the output does not
+// depend on the launch shape.
+// free lanes that trigger a refill
+constexpr int kRefillMin = 16;
+
+__global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) {
+    __shared__ unsigned long long s_stats[4];
+    if (threadIdx.x < 4) s_stats[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int64_t base = a.K / nwaves, rem = a.K % nwaves;
+    const int64_t beg = wid * base + min(wid, rem);
+    const int64_t end = beg + base + (wid < rem ? 1 : 0);
+    const int64_t Nc = a.Nc;
+    const PTabDev& pt = a.pt;
+    const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
+    const uint32_t tw = tag_word(TAG_TRIAL, a.chunk);
+    const int max_it = a.max_trials * 64;
+
+    int64_t next = beg;
+    unsigned long long n_trial = 0, n_inner = 0, n_fail = 0, n_range = 0;
+
+    bool active = false;
+    int64_t c = 0;
+    uint64_t outp = 0;
+    uint32_t jj = 0;
+    int s = 0, tb = 0, i0 = 0, imax = 0, it = 0, trials = 1;
+    double mterm = 0., cmax = 0., rmin0 = 0.;
+    PState q{};
+    float dsfac = 1.f, dfpref = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, Pi = 0.f;
+    float Bq = 0.f, Sq = 0.f, Qq = 0.f;
+    float pixx = 0.f, pixy = 0.f, pixz = 0.f, piyy = 0.f, piyz = 0.f;
+    float cf0 = 0.f, cf1 = 0.f, cf2 = 0.f, cf3 = 0.f, cf4 = 0.f, cf5 = 0.f;
+
+    while (true) {
+        const unsigned long long need = __ballot(!active);
+        if (need && next < end && (__popcll(need) >= kRefillMin || need == ~0ull)) {
+            const int64_t navail = end - next;
+            const int rank = __popcll(need & ((1ull << lane) - 1));
+            if (!active && rank < navail) {
+                const HSetup r = rec[next + rank];
+                c = r.cell;
+                jj = r.j;
+                s = r.species;
+                tb = r.tb;
+                i0 = r.i0;
+                mterm = r.mterm;
+                outp = r.out;
+                pstate_of(a, c, s, q);
+                d0 = a.cells[F_DA0 * Nc + c]; d1 = a.cells[F_DA1 * Nc + c];
+                d2 = a.cells[F_DA2 * Nc + c]; d3 = a.cells[F_DA3 * Nc + c];
+                const float Ed = a.cells[F_E * Nc + c], Pd = a.cells[F_P * Nc + c];
+                Bq = a.sp.B[s]; Sq = a.sp.S[s]; Qq = a.sp.Q[s];
+                dsfac = fabsf(d0) + sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
+                dfpref = 1.0f / (2.0f * q.T * q.T * (Ed + Pd));
+                if (a.shear) {
+                    pixx = a.cells[F_PIXX * Nc + c]; pixy = a.cells[F_PIXY * Nc + c];
+                    pixz = a.cells[F_PIXZ * Nc + c]; piyy = a.cells[F_PIYY * Nc + c];
+                    piyz = a.cells[F_PIYZ * Nc + c];
+                }
+                if (a.bulk != BULK_NONE) {
+                    Pi = a.cells[F_PI * Nc + c];
+                    if (a.bulk == BULK_RTA) Pi = Pi / static_cast<float>(kHbarC);
+                }
+                if (a.bulk != BULK_NONE || a.neos >= 0) {
+                    cf0 = a.coef[c]; cf1 = a.coef[Nc + c]; cf2 = a.coef[2 * Nc + c];
+                    cf3 = a.coef[3 * Nc + c]; cf4 = a.coef[4 * Nc + c]; cf5 = a.coef[5 * Nc + c];
+                }
+                it = 0;
+                trials = 1;
+                imax = pt.len[tb] - 1;
+                if (r.flags & 1) {
+                    // m/T - mu/T outside the table: the reference exits
+                    // (src/MomentumSamplerBase.cpp:33-43); leave a hole
+                    n_range++;
+                    Hadron h{};
+                    static_cast<Hadron*>(a.out)[outp] = h;
+                } else {
+                    const double* e0 = pt.C + (static_cast<int64_t>(tb) * kPTabLen + i0) * 4;
+                    const double* em = pt.C + (static_cast<int64_t>(tb) * kPTabLen + imax) * 4;
+                    rmin0 = e0[2] + 2. * q.mut * e0[1] + q.aa * e0[0] - mterm;
+                    cmax = em[2] + 2. * q.mut * em[1] + q.aa * em[0] - mterm;
+                    active = true;
+                }
+            }
+            next += min<int64_t>(__popcll(need), navail);
+        }
+        if (!__any(active)) {
+            if (next >= end) break;
+            continue;
+        }
+        if (!active) continue;
+        // ---- one iteration for this lane's hadron
+        const u32x4 w = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw, static_cast<uint32_t>(it)}, key0, key1);
+        it++;
+        n_inner++;
+        const double* C = pt.C + static_cast<int64_t>(tb) * kPTabLen * 4;
+        const double r = u32d(w.x) * cmax;
+        // the bracket comb(lo) <= r < comb(hi = lo + 1) is unique, so any
+        // search reproduces the reference's bisection (inverse_CDF); a 4-ary
+        // search issues its three loads together
+        int lo = i0, hi = imax;
+        double rlo = rmin0, rhi = cmax;
+        auto comb = [&](int qq) {
+            const double* e = C + 4 * qq;
+            return e[2] + 2. * q.mut * e[1] + q.aa * e[0] - mterm;
+        };
+        while (hi - lo > 3) {
+            const int qd = (hi - lo) >> 2;
+            const int m1 = lo + qd, m2 = lo + 2 * qd, m3 = lo + 3 * qd;
+            const double v1 = comb(m1), v2 = comb(m2), v3 = comb(m3);
+            if (r < v2) {
+                if (r < v1) { hi = m1; rhi = v1; } else { lo = m1; rlo = v1; hi = m2; rhi = v2; }
+            } else {
+                if (r < v3) { lo = m2; rlo = v2; hi = m3; rhi = v3; } else { lo = m3; rlo = v3; }
+            }
+        }
+        while (hi - lo > 1) {
+            const int mid = (hi + lo) >> 1;
+            const double rm = comb(mid);
+            if (r < rm) { hi = mid; rhi = rm; } else { lo = mid; rlo = rm; }
+        }
+        const double Et0 = pt.Et0[tb], dEt = pt.dEt[tb];
+        double E0 = Et0 + lo * dEt;
+        if (E0 < q.x) { E0 = q.x; rlo = 0.; }
+        const double Ehi = Et0 + hi * dEt;
+        const double Etl = E0 + (Ehi - E0) / fmax(1e-16, rhi - rlo) * (r - rlo);
+        const float fm = q.fm;
+        const float Ep = static_cast<float>(q.Td * Etl + q.mu);
+        const float pm = sqrtf(fmaxf(Ep * Ep - fm * fm, 0.f));
+        const float ratio = (pm / Ep) / (1.f - fm * fm / (2.f * Ep * Ep));
+        bool done = false;
+        if (u24(w.y) <= ratio) {
+            n_trial++;
+            const float T = q.T;
+            const float phi = 2.f * static_cast<float>(kPi) * u24(w.z);
+            const float cth = 2.f * u24(w.w) - 1.f;
+            const float sth = sqrtf(fmaxf(1.f - cth * cth, 0.f));
+            float sphi, cphi;
+            __sincosf(phi, &sphi, &cphi);
+            const float pTc = pm * sth;
+            const float px = pTc * cphi, py = pTc * sphi, pz = pm * cth;
+            const float p0 = sqrtf(fm * fm + pm * pm);
+            const float pds = p0 * d0 + px * d1 + py * d2 + pz * d3;
+            const float f0 = 1.f / (__expf((p0 - q.fmu) / T) + q.fsg);
+            const float omf = 1.f - q.fsg * f0;
+            float dfs = 0.f, dfb = 0.f;
+            if (a.shear) {
+                const float W = px * px * pixx + 2.f * px * py * pixy + 2.f * px * pz * pixz +
+                                py * py * piyy + 2.f * py * pz * piyz + pz * pz * (-pixx - piyy);
+                if (a.neos == 1) dfs = omf * W / (2.f * cf2) / (p0 * T);
+                else if (a.neos == 0) dfs = omf * W * cf0;
+                else dfs = omf * W * dfpref;
+            }
+            if (a.bulk == BULK_RTA || a.bulk == BULK_CE) {
+                const float Eo = p0 / T, mo = fm / T;
+                dfb = -omf * cf0 * (mo * mo / (3.f * Eo) - cf1 * Eo) * Pi;
+            } else if (a.bulk == BULK_14MOM) {
+                dfb = omf * Pi * (cf0 * fm * fm + cf1 * Bq * p0 + cf2 * p0 * p0);
+            } else if (a.bulk == BULK_22MOM) {
+                dfb = omf * Pi * (fm * fm * cf2 + p0 * (Bq * cf3 + Sq * cf4 + Qq * cf5) + p0 * p0 * (cf1 - cf2));
+            }
+            const float w1 = fminf(fmaxf(pds / p0 / dsfac, 0.f), 1.f);
+            const float w2 = fminf(fmaxf((1.f + dfs + dfb) * 0.5f, 0.f), 1.f);
+            const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
+            const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
+            if (ua < w1 * w2) {
+                // boost to the lab frame (src/FSSW.cpp:2001-2010)
+                const float u0 = a.cells[F_U0 * Nc + c], u1 = a.cells[F_U1 * Nc + c],
+                            u2 = a.cells[F_U2 * Nc + c], u3 = a.cells[F_U3 * Nc + c];
+                const float pdu = px * u1 + py * u2 + pz * u3;
+                const float k1 = pdu / (u0 + 1.f) + p0;
+                const float lx = px + k1 * u1, ly = py + k1 * u2, lz = pz + k1 * u3;
+                const float mT = sqrtf(lx * lx + ly * ly + fm * fm);
+                const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
+                float eta_s, E, pzl;
+                if (a.boost_invariant) {
+                    // y uniform in [y_LB, y_RB], eta_s = y - (y - eta_s)
+                    // (src/FSSW.cpp:1024-1029)
+                    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk) ^ 0x00800000u);
+                    const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
+                    eta_s = rap - (asinhf(lz / mT) - eta);
+                    E = mT * coshf(rap);
+                    pzl = mT * sinhf(rap);
+                } else {
+                    eta_s = eta;
+                    E = sqrtf(mT * mT + lz * lz);
+                    pzl = lz;
+                }
+                Hadron h;
+                h.pid = a.sp.pid[s];
+                h.mass = fm;
+                h.E = E; h.px = lx; h.py = ly; h.pz = pzl;
+                h.t = tau * coshf(eta_s);
+                h.x = a.cells[F_X * Nc + c];
+                h.y = a.cells[F_Y * Nc + c];
+                h.z = tau * sinhf(eta_s);
+                static_cast<Hadron*>(a.out)[outp] = h;
+                done = true;
+            } else {
+                trials++;
+            }
+        }
+        if (!done && (trials >= a.max_trials || it >= max_it)) {
+            n_fail++;
+            Hadron h{};
+            static_cast<Hadron*>(a.out)[outp] = h;
+            done = true;
+        }
+        if (done) active = false;
+    }
+    n_trial = wave_sum(n_trial);
+    n_inner = wave_sum(n_inner);
+    n_fail = wave_sum(n_fail);
+    n_range = wave_sum(n_range);
+    if (lane == 0) {
+        atomicAdd(&s_stats[0], n_trial);
+        atomicAdd(&s_stats[1], n_inner);
+        atomicAdd(&s_stats[2], n_fail);
+        atomicAdd(&s_stats[3], n_range);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+}  // namespace issdev
