@@ -115,6 +115,8 @@ typedef struct {
     double ms_sample;        /* device time of the last sampling pass */
     double ms_sample_kernel; /* device time of the last sampling kernel alone */
     double ms_yield_kernel;  /* device time of the last yield kernel alone */
+    int64_t n_fallback;      /* learned proposal: hadrons handed to the plain sampler */
+    int64_t n_violations;    /* learned proposal: draws above the envelope (0 when exact) */
 } iss_stats;
 
 /* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */
@@ -154,6 +156,52 @@ int iss_get_event_counts(iss_ctx* ctx, int64_t* counts);
 /* all hadrons, grouped by event in event order (FSSW::Hadron_list) */
 int iss_get_hadrons(iss_ctx* ctx, iss_hadron* dst, int64_t capacity);
 int iss_get_stats(const iss_ctx* ctx, iss_stats* stats);
+
+/* ---- learned proposal (no counterpart in the reference; the reference's
+ * rejection sampler, src/FSSW.cpp:1852-1966, is the plain proposal).
+ * A 14 -> 64 -> 64 -> 8 network (BF16, FP32 accumulation) shapes the
+ * angular part of a banded envelope of the exact rest-frame target; accepted
+ * momenta follow the same distribution as the plain sampler's.  Parameters
+ * are FP32, row-major [out][in]: W1 64x16, b1 64, W2 64x64, b2 64, W3 8x64,
+ * b3 8 (iss_mlp_num_params() values).  iss_sample with proposal = 1 needs
+ * parameters (init, set or trained). ---- */
+int iss_mlp_num_params(void);
+/* Xavier-uniform weights from seed, zero biases; seed 0 gives all zeros */
+int iss_mlp_init(iss_ctx* ctx, uint64_t seed);
+int iss_mlp_set_params(iss_ctx* ctx, const float* params, int n);
+int iss_mlp_get_params(iss_ctx* ctx, float* params, int n);
+
+typedef struct {
+    int64_t batch;           /* samples per step, a multiple of 64 */
+    int32_t steps;
+    float lr;                /* Adam step size */
+    uint64_t seed;           /* minibatch draws */
+} iss_train_opts;
+
+typedef struct {
+    int64_t n_samples;       /* training-set size */
+    int32_t steps;
+    double loss_first;       /* mean log envelope mass, first and last step */
+    double loss_last;
+    double ms;               /* device time of all steps */
+} iss_train_report;
+
+/* Keep the (cell, species) of the hadrons accepted by the last plain
+ * iss_sample (at most max_samples) as the training set. */
+int iss_mlp_collect(iss_ctx* ctx, int64_t max_samples, int64_t* n_samples);
+/* Adam on the mean log envelope mass over minibatches of the training set. */
+int iss_mlp_train(iss_ctx* ctx, const iss_train_opts* opts, iss_train_report* report);
+/* One forward and backward pass on explicit samples: per-sample log
+ * envelope mass, its gradient at the 8 outputs, the 16 network inputs
+ * (after BF16 rounding) and the gradient of the summed loss with respect to
+ * every parameter.  Any output pointer may be NULL. */
+int iss_mlp_eval(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, float* loss,
+                 float* dz, float* inputs, float* grad);
+/* Envelope tables for explicit (cell, species, rho[8]): 47 floats per
+ * sample (ok, log M, c[9], s[9], kappa[9], beta[9], Q[8] / 2^32, last band)
+ * and the 24 hadron parameters they were built from. */
+int iss_band_diag(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, const float* rho,
+                  float* tables, float* params);
 
 #ifdef __cplusplus
 }

@@ -73,10 +73,29 @@ class Stats(C.Structure):
     _fields_ = [("n_hadrons", C.c_int64), ("n_trials", C.c_int64), ("n_inner", C.c_int64),
                 ("n_failed", C.c_int64), ("n_range_errors", C.c_int64),
                 ("ms_yield", C.c_double), ("ms_sample", C.c_double),
-                ("ms_sample_kernel", C.c_double), ("ms_yield_kernel", C.c_double)]
+                ("ms_sample_kernel", C.c_double), ("ms_yield_kernel", C.c_double),
+                ("n_fallback", C.c_int64), ("n_violations", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class _TrainOpts(C.Structure):
+    _fields_ = [("batch", C.c_int64), ("steps", C.c_int32), ("lr", C.c_float), ("seed", C.c_uint64)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("n_samples", C.c_int64), ("steps", C.c_int32), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# network layout (parameters, FP32, row-major [out][in])
+N_IN, N_HID, N_OUT = 16, 64, 8
+N_BANDS = 9
+BAND_DIAG = 47
 
 
 # every symbol include/iss_hip.h declares, with its ctypes signature
@@ -95,6 +114,16 @@ _SIGNATURES = {
     "iss_get_event_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "iss_get_hadrons": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "iss_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "iss_mlp_num_params": (C.c_int, []),
+    "iss_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "iss_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "iss_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "iss_mlp_collect": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "iss_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(_TrainOpts), C.POINTER(TrainReport)]),
+    "iss_mlp_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p]),
+    "iss_band_diag": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -239,3 +268,52 @@ class Context:
         s = Stats()
         self._check(self.lib.iss_get_stats(self.h, C.byref(s)), "iss_get_stats")
         return s
+
+    # ------------------------------------------------------ learned proposal
+    def num_params(self) -> int:
+        return int(self.lib.iss_mlp_num_params())
+
+    def mlp_init(self, seed: int = 1):
+        self._check(self.lib.iss_mlp_init(self.h, seed), "iss_mlp_init")
+
+    def mlp_set_params(self, params: np.ndarray):
+        a = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._check(self.lib.iss_mlp_set_params(self.h, _ptr(a), a.size), "iss_mlp_set_params")
+
+    def mlp_get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        self._check(self.lib.iss_mlp_get_params(self.h, _ptr(out), out.size), "iss_mlp_get_params")
+        return out
+
+    def mlp_collect(self, max_samples: int = 10_000_000) -> int:
+        n = C.c_int64()
+        self._check(self.lib.iss_mlp_collect(self.h, max_samples, C.byref(n)), "iss_mlp_collect")
+        return int(n.value)
+
+    def mlp_train(self, batch=8192, steps=500, lr=2e-3, seed=7) -> dict:
+        o = _TrainOpts(batch, steps, lr, seed)
+        r = TrainReport()
+        self._check(self.lib.iss_mlp_train(self.h, C.byref(o), C.byref(r)), "iss_mlp_train")
+        return r.as_dict()
+
+    def mlp_eval(self, cell, species):
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        species = np.ascontiguousarray(species, dtype=np.int32)
+        n = cell.size
+        loss = np.zeros(n, dtype=np.float32)
+        dz = np.zeros((n, N_OUT), dtype=np.float32)
+        x = np.zeros((n, N_IN), dtype=np.float32)
+        grad = np.zeros(self.num_params(), dtype=np.float32)
+        self._check(self.lib.iss_mlp_eval(self.h, n, _ptr(cell), _ptr(species), _ptr(loss), _ptr(dz),
+                                          _ptr(x), _ptr(grad)), "iss_mlp_eval")
+        return {"loss": loss, "dz": dz, "x": x, "grad": grad}
+
+    def band_diag(self, cell, species, rho):
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        species = np.ascontiguousarray(species, dtype=np.int32)
+        rho = np.ascontiguousarray(rho, dtype=np.float32).reshape(cell.size, N_OUT)
+        tab = np.zeros((cell.size, BAND_DIAG), dtype=np.float32)
+        par = np.zeros((cell.size, 24), dtype=np.float32)
+        self._check(self.lib.iss_band_diag(self.h, cell.size, _ptr(cell), _ptr(species), _ptr(rho),
+                                           _ptr(tab), _ptr(par)), "iss_band_diag")
+        return tab, par

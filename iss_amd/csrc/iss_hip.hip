@@ -36,6 +36,7 @@ using namespace issdev;
 #include "surface_kernels.h"
 #include "yield_kernels.h"
 #include "sample_kernels.h"
+#include "learned_kernels.h"
 
 // =================================================================== host
 
@@ -117,6 +118,22 @@ struct iss_ctx {
     int E = 0;
     iss_stats stats{};
     int yield_grid = 1;
+    // learned proposal
+    DBuf<float> d_aux;
+    DBuf<float> d_mlp_par, d_adam_m, d_adam_v, d_grad;
+    DBuf<uint4> d_mlp_frag;
+    bool mlp_ready = false;
+    int adam_t = 0;
+    DBuf<int64_t> d_fb_list;
+    DBuf<LRec> d_lrec;
+    DBuf<unsigned long long> d_fb_count;
+    DBuf<float> d_tpart, d_tloss, d_tout;
+    DBuf<uint32_t> d_tr_cell, d_ev_cell;
+    DBuf<uint16_t> d_tr_spec, d_ev_spec;
+    DBuf<int32_t> d_i32a, d_i32b;
+    DBuf<float> d_f32a, d_f32b, d_f32c;
+    int64_t n_train = 0;
+    bool last_plain = false;  // d_setup / d_out hold the last plain pass
 };
 
 namespace {
@@ -234,6 +251,8 @@ int cell_prep(iss_ctx* ctx) {
     HIPCHK(ctx->d_coef.ensure(6 * ctx->Nc));
     const int64_t nb = (ctx->Nc + 255) / 256;
     k_cell_prep<<<nb, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->Nc, deltaf_dev(ctx), ctx->d_coef.p);
+    HIPCHK(ctx->d_aux.ensure(static_cast<size_t>(kAuxFields) * ctx->Nc));
+    k_cell_aux<<<nb, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->Nc, ctx->d_aux.p);
     HIPCHK(hipGetLastError());
     return ISS_OK;
 }
@@ -356,6 +375,92 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     return ISS_OK;
 }
 
+
+// surface, species and delta-f fields of SampleArgs (no sampling pass)
+SampleArgs base_args(iss_ctx* ctx) {
+    SampleArgs a{};
+    a.cells = ctx->d_cells.p;
+    a.coef = ctx->d_coef.p;
+    a.Nc = ctx->Nc;
+    a.Ns = ctx->Ns;
+    a.sp = species_dev(ctx);
+    a.pt = ctx->pt;
+    a.shear = ctx->df.include_shear;
+    a.bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
+    a.neos = ctx->neos;
+    a.max_trials = 5000;
+    return a;
+}
+
+int mlp_pack(iss_ctx* ctx) {
+    k_mlp_pack<<<(kNumFrags + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_mlp_par.p, ctx->d_mlp_frag.p);
+    HIPCHK(hipGetLastError());
+    return ISS_OK;
+}
+
+int mlp_alloc(iss_ctx* ctx) {
+    HIPCHK(ctx->d_mlp_par.ensure(kNumParams));
+    HIPCHK(ctx->d_adam_m.ensure(kNumParams));
+    HIPCHK(ctx->d_adam_v.ensure(kNumParams));
+    HIPCHK(ctx->d_grad.ensure(kNumParams));
+    HIPCHK(ctx->d_mlp_frag.ensure(kNumFrags));
+    return ISS_OK;
+}
+
+int mlp_upload(iss_ctx* ctx, const float* params) {
+    int rc = mlp_alloc(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_mlp_par.p, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_adam_m.p, 0, sizeof(float) * kNumParams, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_adam_v.p, 0, sizeof(float) * kNumParams, ctx->stream));
+    rc = mlp_pack(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->adam_t = 0;
+    ctx->mlp_ready = true;
+    return ISS_OK;
+}
+
+int check_samples(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species) {
+    if (n < 0 || (n > 0 && (!cell || !species))) return set_err(ctx, ISS_ERR_ARG, "bad sample arrays");
+    if (ctx->Nc <= 0 || ctx->Ns <= 0 || !ctx->tables_ready)
+        return set_err(ctx, ISS_ERR_STATE, "surface, species and delta f must be set");
+    for (int64_t i = 0; i < n; i++)
+        if (cell[i] < 0 || cell[i] >= ctx->Nc || species[i] < 0 || species[i] >= ctx->Ns)
+            return set_err(ctx, ISS_ERR_ARG, "cell or species index out of range");
+    return ISS_OK;
+}
+
+// one training step (or a gradient evaluation) over nb * 64 samples
+int mlp_step(iss_ctx* ctx, TrainArgs& t, int nb, float lr, float* grad_out) {
+    HIPCHK(ctx->d_tpart.ensure(static_cast<size_t>(nb) * kNumParams));
+    HIPCHK(ctx->d_tloss.ensure(2 * static_cast<size_t>(nb)));
+    t.partial = ctx->d_tpart.p;
+    t.loss_part = ctx->d_tloss.p;
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_train),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, kTrainLds));
+    k_mlp_train<<<nb, 64, kTrainLds, ctx->stream>>>(t);
+    HIPCHK(hipGetLastError());
+    if (lr > 0.f) ctx->adam_t++;
+    k_mlp_reduce_adam<<<(kNumParams + 255) / 256, 256, 0, ctx->stream>>>(
+        ctx->d_tpart.p, nb, ctx->d_tloss.p, ctx->d_mlp_par.p, ctx->d_adam_m.p, ctx->d_adam_v.p, grad_out, lr,
+        ctx->adam_t);
+    HIPCHK(hipGetLastError());
+    if (lr > 0.f) return mlp_pack(ctx);
+    return ISS_OK;
+}
+
+double mean_loss(iss_ctx* ctx, int nb) {
+    std::vector<float> h(2 * static_cast<size_t>(nb));
+    if (hipMemcpyAsync(h.data(), ctx->d_tloss.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return 0.;
+    double ls = 0., nv = 0.;
+    for (int b = 0; b < nb; b++) { ls += h[b]; nv += h[nb + b]; }
+    return nv > 0. ? ls / nv : 0.;
+}
+
 }  // namespace
 
 // ================================================================ C-ABI
@@ -377,7 +482,7 @@ int iss_ctx_create(int device, iss_ctx** out) {
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
-    if (ctx->d_stats.ensure(4) != hipSuccess) {
+    if (ctx->d_stats.ensure(6) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
         delete ctx;
         return ISS_ERR_NOMEM;
     }
@@ -396,6 +501,11 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_ctot.release(); ctx->d_partial.release(); ctx->d_oob.release(); ctx->d_stot.release(); ctx->d_raw.release();
     ctx->d_counts.release(); ctx->d_evcount.release(); ctx->d_off.release(); ctx->d_evoff.release();
     ctx->d_tree0.release(); ctx->d_tree1.release(); ctx->d_out.release(); ctx->d_setup.release(); ctx->d_stats.release();
+    ctx->d_aux.release(); ctx->d_mlp_par.release(); ctx->d_adam_m.release(); ctx->d_adam_v.release();
+    ctx->d_grad.release(); ctx->d_mlp_frag.release(); ctx->d_fb_list.release(); ctx->d_fb_count.release(); ctx->d_lrec.release();
+    ctx->d_tpart.release(); ctx->d_tloss.release(); ctx->d_tout.release(); ctx->d_tr_cell.release();
+    ctx->d_ev_cell.release(); ctx->d_tr_spec.release(); ctx->d_ev_spec.release(); ctx->d_i32a.release();
+    ctx->d_i32b.release(); ctx->d_f32a.release(); ctx->d_f32b.release(); ctx->d_f32c.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -577,7 +687,9 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     if (o->n_events <= 0) return set_err(ctx, ISS_ERR_ARG, "n_events must be positive");
     if (o->sampling_model != 30)
         return set_err(ctx, ISS_ERR_UNSUPPORTED, "only dN_dy_sampling_model = 30 (Poisson) is supported");
-    if (o->proposal != 0) return set_err(ctx, ISS_ERR_UNSUPPORTED, "learned proposal not available");
+    if (o->proposal != 0 && o->proposal != 1) return set_err(ctx, ISS_ERR_ARG, "proposal must be 0 or 1");
+    if (o->proposal == 1 && !ctx->mlp_ready)
+        return set_err(ctx, ISS_ERR_STATE, "learned proposal: no network parameters (init, set or train)");
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
     if (!ctx->yields_valid) {
         int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr);
@@ -596,7 +708,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     while ((1ll << D) < E) D++;
     HIPCHK(ctx->d_tree0.ensure(1ll << D));
     HIPCHK(ctx->d_tree1.ensure(1ll << D));
-    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 6 * sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if (Nc > 0)
         k_counts<<<(Nc + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_ctot.p, Nc, mult, o->seed, chunk,
@@ -646,20 +758,58 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.max_trials = o->max_trials;
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
-    HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
+    a.Kdev = nullptr;
+    int per_cu_trials = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, 256, 0) != hipSuccess ||
+        per_cu_trials < 1)
+        per_cu_trials = 1;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-    if (K > 0) {
-        k_setup<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, 256, 0) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        const int64_t want = (K + 255) / 256;
-        const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
-        k_trials<<<nblk, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+    if (o->proposal == 0) {
+        HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
+        if (K > 0) {
+            k_setup<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
+            const int64_t want = (K + 255) / 256;
+            const int nblk = static_cast<int>(
+                std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu_trials)));
+            k_trials<<<nblk, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+        }
+        ctx->last_plain = true;
+    } else {
+        ctx->last_plain = false;
+        HIPCHK(ctx->d_fb_list.ensure(std::max<int64_t>(K, 1)));
+        HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
+        HIPCHK(hipMemsetAsync(ctx->d_fb_count.p, 0, sizeof(unsigned long long), ctx->stream));
+        if (K > 0) {
+            LearnArgs L;
+            L.a = a;
+            L.aux = ctx->d_aux.p;
+            L.W = MlpDev{ctx->d_mlp_frag.p, ctx->d_mlp_par.p};
+            L.fb_list = ctx->d_fb_list.p;
+            L.fb_count = ctx->d_fb_count.p;
+            L.stats2 = ctx->d_stats.p + 4;
+            HIPCHK(ctx->d_lrec.ensure(std::max<int64_t>(K, 1)));
+            const int lds = kMlpBytes * (kSetupThreads / kWave);
+            const int64_t nwave = (K + 63) / 64;
+            k_learn_setup<<<(nwave + kSetupThreads / 64 - 1) / (kSetupThreads / 64), kSetupThreads, lds, ctx->stream>>>(
+                L, ctx->d_lrec.p);
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_learn_trials, 256, 0) != hipSuccess ||
+                per_cu < 1)
+                per_cu = 1;
+            const int64_t want = (K + 255) / 256;
+            const int nblk = static_cast<int>(
+                std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
+            k_learn_trials<<<nblk, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
+            // hadrons without an envelope: the plain sampler, sized on the device
+            k_setup<<<ctx->num_cu * 4, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p, ctx->d_fb_count.p);
+            SampleArgs af = a;
+            af.Kdev = ctx->d_fb_count.p;
+            k_trials<<<ctx->num_cu * per_cu_trials, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+        }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
-    unsigned long long st[4];
+    unsigned long long st[6];
     HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms_all = 0.f, ms_k = 0.f;
@@ -674,6 +824,8 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
     ctx->stats.ms_sample = ms_all;
     ctx->stats.ms_sample_kernel = ms_k;
+    ctx->stats.n_fallback = static_cast<int64_t>(st[4]);
+    ctx->stats.n_violations = static_cast<int64_t>(st[5]);
     if (n_hadrons) *n_hadrons = K;
     if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
     return ISS_OK;
@@ -701,6 +853,185 @@ int iss_get_hadrons(iss_ctx* ctx, iss_hadron* dst, int64_t capacity) {
 int iss_get_stats(const iss_ctx* ctx, iss_stats* s) {
     if (!ctx || !s) return ISS_ERR_ARG;
     *s = ctx->stats;
+    return ISS_OK;
+}
+
+int iss_mlp_num_params(void) { return kNumParams; }
+
+int iss_mlp_init(iss_ctx* ctx, uint64_t seed) {
+    if (!ctx) return ISS_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    std::vector<float> par(kNumParams, 0.f);
+    if (seed != 0) {
+        // Xavier-uniform from a 64-bit LCG stream
+        uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1ull;
+        auto uni = [&st]() {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            return static_cast<double>(st >> 11) * (1.0 / 9007199254740992.0);
+        };
+        auto fill = [&](int off, int fan_out, int fan_in, int used_in) {
+            const double lim = std::sqrt(6.0 / (fan_in + fan_out));
+            for (int o = 0; o < fan_out; o++)
+                for (int i = 0; i < fan_in; i++)
+                    par[off + o * fan_in + i] = i < used_in ? static_cast<float>((2. * uni() - 1.) * lim) : 0.f;
+        };
+        fill(P_W1, kHid, kIn, 14);
+        fill(P_W2, kHid, kHid, kHid);
+        fill(P_W3, kOut, kHid, kHid);
+    }
+    return mlp_upload(ctx, par.data());
+}
+
+int iss_mlp_set_params(iss_ctx* ctx, const float* params, int n) {
+    if (!ctx || !params || n != kNumParams) return ISS_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    return mlp_upload(ctx, params);
+}
+
+int iss_mlp_get_params(iss_ctx* ctx, float* params, int n) {
+    if (!ctx || !params || n != kNumParams) return ISS_ERR_ARG;
+    if (!ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters");
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipMemcpyAsync(params, ctx->d_mlp_par.p, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ISS_OK;
+}
+
+int iss_mlp_collect(iss_ctx* ctx, int64_t max_samples, int64_t* n_samples) {
+    if (!ctx || max_samples <= 0) return ISS_ERR_ARG;
+    if (!ctx->last_plain) return set_err(ctx, ISS_ERR_STATE, "the last iss_sample was not a plain pass");
+    (void)hipSetDevice(ctx->device);
+    const int64_t K = ctx->K;
+    HIPCHK(ctx->d_u32.ensure(std::max<int64_t>(K, 1)));
+    HIPCHK(ctx->d_i64.ensure(K + 1));
+    if (K > 0) {
+        k_collect_flag<<<(K + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_setup.p, K, ctx->d_out.p, ctx->d_u32.p);
+        HIPCHK(hipGetLastError());
+    }
+    int rc = launch_scan(ctx, ctx->d_u32.p, K, ctx->d_i64.p);
+    if (rc) return rc;
+    int64_t nacc = 0;
+    HIPCHK(hipMemcpyAsync(&nacc, ctx->d_i64.p + K, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int64_t n = std::min(nacc, max_samples);
+    HIPCHK(ctx->d_tr_cell.ensure(std::max<int64_t>(n, 1)));
+    HIPCHK(ctx->d_tr_spec.ensure(std::max<int64_t>(n, 1)));
+    if (K > 0) {
+        k_collect_write<<<(K + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_setup.p, K, ctx->d_u32.p, ctx->d_i64.p,
+                                                                  max_samples, ctx->d_tr_cell.p, ctx->d_tr_spec.p);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->n_train = n;
+    if (n_samples) *n_samples = n;
+    return ISS_OK;
+}
+
+int iss_mlp_train(iss_ctx* ctx, const iss_train_opts* o, iss_train_report* rep) {
+    if (!ctx || !o || o->batch <= 0 || o->batch % 64 != 0 || o->steps < 0 || !(o->lr > 0.f)) return ISS_ERR_ARG;
+    if (!ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters (iss_mlp_init)");
+    if (ctx->n_train <= 0) return set_err(ctx, ISS_ERR_STATE, "empty training set (iss_mlp_collect)");
+    if (ctx->Nc <= 0 || !ctx->tables_ready) return set_err(ctx, ISS_ERR_STATE, "surface and tables must be set");
+    (void)hipSetDevice(ctx->device);
+    const int nb = static_cast<int>(o->batch / 64);
+    TrainArgs t{};
+    t.a = base_args(ctx);
+    t.aux = ctx->d_aux.p;
+    t.W = MlpDev{ctx->d_mlp_frag.p, ctx->d_mlp_par.p};
+    t.cell = ctx->d_tr_cell.p;
+    t.species = ctx->d_tr_spec.p;
+    t.n = ctx->n_train;
+    t.pick = nullptr;
+    t.direct = 0;
+    t.npick = o->batch;
+    t.seed = o->seed;
+    double first = 0., last = 0.;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int step = 0; step < o->steps; step++) {
+        t.step = static_cast<uint32_t>(step);
+        int rc = mlp_step(ctx, t, nb, o->lr, nullptr);
+        if (rc) return rc;
+        if (step == 0) first = mean_loss(ctx, nb);
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    if (o->steps > 0) last = mean_loss(ctx, nb);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (rep) {
+        rep->n_samples = ctx->n_train;
+        rep->steps = o->steps;
+        rep->loss_first = first;
+        rep->loss_last = last;
+        rep->ms = ms;
+    }
+    return ISS_OK;
+}
+
+int iss_mlp_eval(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, float* loss, float* dz,
+                 float* inputs, float* grad) {
+    if (!ctx) return ISS_ERR_ARG;
+    if (!ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters");
+    int rc = check_samples(ctx, n, cell, species);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    if (n == 0) return ISS_OK;
+    std::vector<uint32_t> hc(n);
+    std::vector<uint16_t> hs(n);
+    for (int64_t i = 0; i < n; i++) { hc[i] = static_cast<uint32_t>(cell[i]); hs[i] = static_cast<uint16_t>(species[i]); }
+    HIPCHK(ctx->d_ev_cell.ensure(n));
+    HIPCHK(ctx->d_ev_spec.ensure(n));
+    HIPCHK(ctx->d_f32a.ensure(n));
+    HIPCHK(ctx->d_f32b.ensure(n * kOut));
+    HIPCHK(ctx->d_f32c.ensure(n * kIn));
+    HIPCHK(hipMemcpyAsync(ctx->d_ev_cell.p, hc.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_ev_spec.p, hs.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    TrainArgs t{};
+    t.a = base_args(ctx);
+    t.aux = ctx->d_aux.p;
+    t.W = MlpDev{ctx->d_mlp_frag.p, ctx->d_mlp_par.p};
+    t.cell = ctx->d_ev_cell.p;
+    t.species = ctx->d_ev_spec.p;
+    t.n = n;
+    t.pick = nullptr;
+    t.direct = 1;
+    t.npick = n;
+    t.out_loss = ctx->d_f32a.p;
+    t.out_dz = ctx->d_f32b.p;
+    t.out_x = ctx->d_f32c.p;
+    const int nb = static_cast<int>((n + 63) / 64);
+    rc = mlp_step(ctx, t, nb, 0.f, ctx->d_grad.p);
+    if (rc) return rc;
+    if (loss) HIPCHK(hipMemcpyAsync(loss, ctx->d_f32a.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (dz) HIPCHK(hipMemcpyAsync(dz, ctx->d_f32b.p, sizeof(float) * n * kOut, hipMemcpyDeviceToHost, ctx->stream));
+    if (inputs)
+        HIPCHK(hipMemcpyAsync(inputs, ctx->d_f32c.p, sizeof(float) * n * kIn, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, ctx->d_grad.p, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ISS_OK;
+}
+
+int iss_band_diag(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, const float* rho,
+                  float* tables, float* params) {
+    if (!ctx || (n > 0 && (!rho || !tables))) return ISS_ERR_ARG;
+    int rc = check_samples(ctx, n, cell, species);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    if (n == 0) return ISS_OK;
+    HIPCHK(ctx->d_i32a.ensure(n));
+    HIPCHK(ctx->d_i32b.ensure(n));
+    HIPCHK(ctx->d_f32a.ensure(n * kOut));
+    HIPCHK(ctx->d_f32b.ensure(n * kBandDiag));
+    HIPCHK(ctx->d_f32c.ensure(n * 24));
+    HIPCHK(hipMemcpyAsync(ctx->d_i32a.p, cell, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_i32b.p, species, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_f32a.p, rho, sizeof(float) * n * kOut, hipMemcpyHostToDevice, ctx->stream));
+    k_band_diag<<<(n + 255) / 256, 256, 0, ctx->stream>>>(base_args(ctx), ctx->d_aux.p, ctx->d_i32a.p, ctx->d_i32b.p,
+                                                         ctx->d_f32a.p, n, ctx->d_f32b.p, ctx->d_f32c.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(tables, ctx->d_f32b.p, sizeof(float) * n * kBandDiag, hipMemcpyDeviceToHost, ctx->stream));
+    if (params) HIPCHK(hipMemcpyAsync(params, ctx->d_f32c.p, sizeof(float) * n * 24, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return ISS_OK;
 }
 

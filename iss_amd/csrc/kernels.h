@@ -86,6 +86,7 @@ struct SampleArgs {
     const int64_t* evoff;    // [E + 1]
     int E;
     int64_t K;               // hadrons in this pass
+    const unsigned long long* Kdev;  // if set, the trial kernel's record count (device)
     uint64_t seed;
     uint32_t chunk;
     Permutation perm;

@@ -1,0 +1,1182 @@
+// Learned banded proposal for the rest-frame momentum, and the network that
+// shapes it.  (one translation unit: included by iss_hip.hip)
+//
+// Target.  For one hadron of species s emitted by cell c, the reference's
+// rejection scheme (src/FSSW.cpp:1852-1966 with src/MomentumSamplerBase.cpp:
+// 20-58) produces rest-frame momenta with density, in t = (E - mu)/T and the
+// direction n,
+//     pi(t, n) ~ p E f0(E) w1(p) w2(p),      t in [x, t_max], x = (m - mu)/T,
+// w1 = clamp(p.dsigma / (E (|ds0| + |ds|)), 0, 1) and
+// w2 = clamp((1 + df_shear + df_bulk) / 2, 0, 1); t_max is the top of the
+// reference's |p| table, so the support is the reference's.
+//
+// Proposal.  t - x is cut into 9 bands: 8 bands with edges
+// 2 (-log(1 - k/8)) and a far tail.  In band k the |p| part uses an
+// exponential envelope exp(c_k + s_k (t - x)) of G(t) = log(p E f0 e^x): the
+// tangent of G at the band midpoint when G is concave on the band (checked
+// per band through an upper bound of G''), else the bound
+// sup(p E) sup(e^t f0) e^-t.  Directions follow
+//     q(n) = (1 + kappa_k n.pi.n) (1 + beta_k n.d) / (4 pi),
+// sampled exactly as a mixture of the three n_i^2 laws in the eigenframe of
+// the shear tensor (weights (1 + kappa lambda_i)/3, cos = cbrt(2u - 1))
+// followed by an antipodal flip that keeps n with probability
+// (1 + beta n.d)/2.  Per band, S1 >= w1 / (1 + beta y) and
+// S2 >= w2 / (1 + kappa W) are verified suprema over the band's momenta and
+// all directions (interval bounds of the delta-f terms, endpoint and
+// clamp-crossing candidates of piecewise Moebius functions).  Band k is
+// chosen with probability proportional to its envelope mass and a draw is
+// accepted with probability pi / envelope <= 1, so the accepted momenta
+// follow pi exactly.  Hadrons whose envelope cannot be formed (a
+// Bose-Einstein species at its condensation point, non-finite bounds, or the
+// reference's Fermi-Dirac table quirk in force) go to the plain sampler.
+//
+// Network.  A 14 -> 64 -> 64 -> 8 perceptron with tanh-GELU, BF16 weights and
+// activations, FP32 accumulation on v_mfma_f32_32x32x16_bf16, maps cell state
+// and species to kappa_k = sigmoid(z_k) kappa0_k (band k), where kappa0_k is
+// the unclamped optimum at the band midpoint.  It is trained on the
+// (cell, species) of hadrons accepted by the plain sampler, minimising the
+// mean log envelope mass, i.e. maximising the acceptance of the exact
+// sampler.  With zero weights every kappa is kappa0 / 2.
+#pragma once
+
+#include "kernels.h"
+#include "sample_kernels.h"
+
+namespace issdev {
+
+// ============================================================== cell aux
+// eigen-decomposition of the rest-frame shear tensor, the dsigma direction
+// in its eigenframe and |dsigma|; [kAuxFields][Nc] floats
+enum AuxField {
+    A_L1 = 0, A_L2, A_L3,                       // eigenvalues, descending
+    A_R00, A_R01, A_R02, A_R10, A_R11, A_R12,   // R[r][c]: component r of eigenvector c
+    A_R20, A_R21, A_R22,
+    A_DH0, A_DH1, A_DH2,                        // unit dsigma in the eigenframe
+    A_DN,                                       // |dsigma| (spatial)
+    kAuxFields
+};
+
+// cyclic Jacobi rotations of a symmetric 3x3 matrix; A becomes diagonal, V
+// holds the eigenvectors as columns
+__device__ inline void jacobi3(double A[3][3], double V[3][3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) V[i][j] = (i == j) ? 1. : 0.;
+    for (int sweep = 0; sweep < 16; sweep++) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-30 * dia || off == 0.) break;
+        for (int pq = 0; pq < 3; pq++) {
+            const int p = (pq == 2) ? 1 : 0, q = (pq == 0) ? 1 : 2;
+            const double apq = A[p][q];
+            if (apq == 0.) continue;
+            const double theta = (A[q][q] - A[p][p]) / (2. * apq);
+            const double t = (theta >= 0. ? 1. : -1.) / (fabs(theta) + sqrt(theta * theta + 1.));
+            const double cs = 1. / sqrt(t * t + 1.), sn = t * cs;
+            A[p][p] -= t * apq;
+            A[q][q] += t * apq;
+            A[p][q] = A[q][p] = 0.;
+            const int r = 3 - p - q;
+            const double arp = A[r][p], arq = A[r][q];
+            A[r][p] = A[p][r] = cs * arp - sn * arq;
+            A[r][q] = A[q][r] = sn * arp + cs * arq;
+            for (int k = 0; k < 3; k++) {
+                const double vkp = V[k][p], vkq = V[k][q];
+                V[k][p] = cs * vkp - sn * vkq;
+                V[k][q] = sn * vkp + cs * vkq;
+            }
+        }
+    }
+}
+
+__global__ void k_cell_aux(const float* __restrict__ cells, int64_t Nc, float* __restrict__ aux) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const double xx = cells[F_PIXX * Nc + c], xy = cells[F_PIXY * Nc + c], xz = cells[F_PIXZ * Nc + c];
+    const double yy = cells[F_PIYY * Nc + c], yz = cells[F_PIYZ * Nc + c];
+    double A[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, -xx - yy}};
+    double V[3][3];
+    jacobi3(A, V);
+    double lam[3] = {A[0][0], A[1][1], A[2][2]};
+    int o[3] = {0, 1, 2};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2 - i; j++)
+            if (lam[o[j]] < lam[o[j + 1]]) { const int t = o[j]; o[j] = o[j + 1]; o[j + 1] = t; }
+    const double d1 = cells[F_DA1 * Nc + c], d2 = cells[F_DA2 * Nc + c], d3 = cells[F_DA3 * Nc + c];
+    const double dn = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+    for (int i = 0; i < 3; i++) {
+        aux[(A_L1 + i) * Nc + c] = static_cast<float>(lam[o[i]]);
+        for (int r = 0; r < 3; r++) aux[(A_R00 + 3 * r + i) * Nc + c] = static_cast<float>(V[r][o[i]]);
+        const double proj = d1 * V[0][o[i]] + d2 * V[1][o[i]] + d3 * V[2][o[i]];
+        aux[(A_DH0 + i) * Nc + c] = static_cast<float>(dn > 0. ? proj / dn : 0.);
+    }
+    aux[A_DN * Nc + c] = static_cast<float>(dn);
+}
+
+// ======================================================= per-hadron params
+struct LParams {
+    float T, mu, m, sgn;      // sgn: -1 Bose-Einstein, +1 Fermi-Dirac, 0 Boltzmann
+    float x, tmax;            // t = (E - mu)/T range of the reference's |p| table
+    float d0, dn, D;          // rest-frame dsigma_0, |dsigma|, |dsigma_0| + |dsigma|
+    float l1, l2, l3;         // shear eigenvalues [GeV/fm^3]
+    float dh0, dh1, dh2;      // unit dsigma in the eigenframe
+    float ps0;                // shear: df_s = omf p^2 W Ps(E), Ps = ps0 (smode 1) or ps0/E (smode 2)
+    float qi, q0, q1, q2;     // bulk: df_b = omf (qi/E + q0 + q1 E + q2 E^2)
+    int smode;
+    int flags;                // 1: m/T - mu/T outside the table (hole); 2: plain-sampler fallback
+};
+
+__device__ __forceinline__ float ps_of(const LParams& P, float E) {
+    return P.smode == 2 ? P.ps0 / E : (P.smode == 1 ? P.ps0 : 0.f);
+}
+__device__ __forceinline__ float qb_of(const LParams& P, float E) {
+    return P.qi / E + P.q0 + E * (P.q1 + P.q2 * E);
+}
+
+// The delta-f forms of the plain sampler's trial loop, rewritten as
+// polynomials in E: shear (src/FSSW.cpp:1897-1913), bulk (src/FSSW.cpp:
+// 1916-1928, 1795-1849).
+__device__ inline void lparams_of(const SampleArgs& a, const float* __restrict__ aux, int64_t c, int s,
+                                  LParams& P) {
+    const int64_t Nc = a.Nc;
+    PState q;
+    pstate_of(a, c, s, q);
+    const int sign = a.sp.sign[s];
+    const int tb = ((sign == -1) ? 0 : 3) + ((q.x < 30.) ? 0 : (q.x < 50. ? 1 : 2));
+    const int i0 = static_cast<int>((q.x - a.pt.Et0[tb]) / a.pt.dEt[tb]);
+    P.flags = (i0 < 0 || i0 >= a.pt.len[tb] - 1) ? 1 : 0;
+    P.T = q.T;
+    P.mu = static_cast<float>(q.mu);
+    P.m = q.fm;
+    P.sgn = static_cast<float>(sign);
+    P.x = static_cast<float>(q.x);
+    P.tmax = static_cast<float>(a.pt.Et0[tb] + (a.pt.len[tb] - 1) * a.pt.dEt[tb]);
+    P.d0 = a.cells[F_DA0 * Nc + c];
+    P.dn = aux[A_DN * Nc + c];
+    P.D = fabsf(P.d0) + P.dn;
+    P.l1 = aux[A_L1 * Nc + c]; P.l2 = aux[A_L2 * Nc + c]; P.l3 = aux[A_L3 * Nc + c];
+    P.dh0 = aux[A_DH0 * Nc + c]; P.dh1 = aux[A_DH1 * Nc + c]; P.dh2 = aux[A_DH2 * Nc + c];
+    const float T = q.T;
+    const float Ed = a.cells[F_E * Nc + c], Pd = a.cells[F_P * Nc + c];
+    float cf0 = 0.f, cf1 = 0.f, cf2 = 0.f, cf3 = 0.f, cf4 = 0.f, cf5 = 0.f;
+    if (a.bulk != BULK_NONE || a.neos >= 0) {
+        cf0 = a.coef[c]; cf1 = a.coef[Nc + c]; cf2 = a.coef[2 * Nc + c];
+        cf3 = a.coef[3 * Nc + c]; cf4 = a.coef[4 * Nc + c]; cf5 = a.coef[5 * Nc + c];
+    }
+    P.smode = 0;
+    P.ps0 = 0.f;
+    if (a.shear) {
+        if (a.neos == 1) { P.smode = 2; P.ps0 = 1.f / (2.f * cf2) / T; }
+        else if (a.neos == 0) { P.smode = 1; P.ps0 = cf0; }
+        else { P.smode = 1; P.ps0 = 1.0f / (2.0f * T * T * (Ed + Pd)); }
+    }
+    P.qi = P.q0 = P.q1 = P.q2 = 0.f;
+    if (a.bulk != BULK_NONE) {
+        float Pi = a.cells[F_PI * Nc + c];
+        if (a.bulk == BULK_RTA) Pi = Pi / static_cast<float>(kHbarC);
+        const float m = P.m;
+        const float Bq = a.sp.B[s], Sq = a.sp.S[s], Qq = a.sp.Q[s];
+        if (a.bulk == BULK_RTA || a.bulk == BULK_CE) {
+            P.qi = -cf0 * Pi * m * m / (3.f * T);
+            P.q1 = cf0 * cf1 * Pi / T;
+        } else if (a.bulk == BULK_14MOM) {
+            P.q0 = Pi * cf0 * m * m;
+            P.q1 = Pi * cf1 * Bq;
+            P.q2 = Pi * cf2;
+        } else if (a.bulk == BULK_22MOM) {
+            P.q0 = Pi * m * m * cf2;
+            P.q1 = Pi * (Bq * cf3 + Sq * cf4 + Qq * cf5);
+            P.q2 = Pi * (cf1 - cf2);
+        }
+    }
+    if ((sign < 0 && q.x < 1e-3) || (a.pt.quirk && tb >= 3)) P.flags |= 2;
+}
+
+// 14 network inputs (two zero pads): T, muB, the five rest-frame shear
+// components and the bulk pressure relative to e + P, mass, degeneracy,
+// baryon number, statistics, and the shear and bulk delta-f at the thermal
+// energy E = m + 2T
+constexpr int kIn = 16, kHid = 64, kOut = 8;
+__device__ inline void features_of(const SampleArgs& a, int64_t c, int s, const LParams& P, float f[kIn]) {
+    const int64_t Nc = a.Nc;
+    const float epp = a.cells[F_E * Nc + c] + a.cells[F_P * Nc + c];
+    const float ie = epp > 0.f ? 5.f / epp : 0.f;
+    f[0] = (P.T / 0.15f - 1.f) * 10.f;
+    f[1] = a.cells[F_MUB * Nc + c] / 0.3f;
+    f[2] = a.cells[F_PIXX * Nc + c] * ie;
+    f[3] = a.cells[F_PIXY * Nc + c] * ie;
+    f[4] = a.cells[F_PIXZ * Nc + c] * ie;
+    f[5] = a.cells[F_PIYY * Nc + c] * ie;
+    f[6] = a.cells[F_PIYZ * Nc + c] * ie;
+    f[7] = a.cells[F_PI * Nc + c] * ie;
+    f[8] = P.m;
+    f[9] = static_cast<float>(a.sp.g[s]) * 0.25f;
+    f[10] = static_cast<float>(a.sp.B[s]);
+    f[11] = P.sgn;
+    const float Eth = P.m + 2.f * P.T;
+    f[12] = (Eth * Eth - P.m * P.m) * ps_of(P, Eth) * (P.l1 - P.l3);
+    f[13] = qb_of(P, Eth);
+    f[14] = 0.f;
+    f[15] = 0.f;
+}
+
+// ============================================================ BF16 MFMA
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ f32x16 mfma32(uint4 a, uint4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                     c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 zero16() {
+    f32x16 z;
+#pragma unroll
+    for (int i = 0; i < 16; i++) z[i] = 0.f;
+    return z;
+}
+// LDS traffic between lanes of one wave: wait for this wave's LDS operations
+// and keep the compiler from moving memory accesses across
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// Tiles are 32 x 32 in the layout of v_mfma_f32_32x32x16_bf16: operand A
+// (rows m0.., 16 k per step) comes from lane l as row l%32, k = 8 (l/32)..+7;
+// operand B^T likewise with rows n0..; the accumulator register i of lane l
+// is C[m0 + 8 (i/4) + 4 (l/32) + i%4][n0 + l%32].
+//
+// A from pre-packed fragments in global memory (16 bytes per lane and step),
+// B^T from LDS rows (bf16, row stride ldb elements)
+__device__ __forceinline__ f32x16 tile_gl(const uint4* __restrict__ afrag, const uint16_t* Bt, int ldb, int n0,
+                                          int nk, f32x16 acc) {
+    const int l = threadIdx.x & 63, r = l & 31, g = l >> 5;
+    for (int kt = 0; kt < nk; kt++) {
+        const uint4 av = afrag[kt * 64 + l];
+        const uint4 bv = *reinterpret_cast<const uint4*>(Bt + (n0 + r) * ldb + kt * 16 + 8 * g);
+        acc = mfma32(av, bv, acc);
+    }
+    return acc;
+}
+// both operands from LDS; rows past mvalid / nvalid read as zero
+__device__ __forceinline__ f32x16 tile_ll(const uint16_t* A, int lda, int m0, int mvalid, const uint16_t* Bt,
+                                          int ldb, int n0, int nvalid, int nk, f32x16 acc) {
+    const int l = threadIdx.x & 63, r = l & 31, g = l >> 5;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (int kt = 0; kt < nk; kt++) {
+        const uint4 av = (m0 + r < mvalid) ? *reinterpret_cast<const uint4*>(A + (m0 + r) * lda + kt * 16 + 8 * g) : z;
+        const uint4 bv = (n0 + r < nvalid) ? *reinterpret_cast<const uint4*>(Bt + (n0 + r) * ldb + kt * 16 + 8 * g) : z;
+        acc = mfma32(av, bv, acc);
+    }
+    return acc;
+}
+
+// tanh-form GELU and its derivative
+__device__ __forceinline__ float gelu_t(float x) {
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    return 1.f - 2.f / (expf(2.f * u) + 1.f);
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + gelu_t(x)); }
+__device__ __forceinline__ float gelu_d(float x) {
+    const float th = gelu_t(x);
+    return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * x * x);
+}
+
+// parameters, FP32 master layout (row-major [out][in])
+constexpr int P_W1 = 0;                       // [64][16]
+constexpr int P_B1 = P_W1 + kHid * kIn;       // [64]
+constexpr int P_W2 = P_B1 + kHid;             // [64][64]
+constexpr int P_B2 = P_W2 + kHid * kHid;      // [64]
+constexpr int P_W3 = P_B2 + kHid;             // [8][64]
+constexpr int P_B3 = P_W3 + kOut * kHid;      // [8]
+constexpr int kNumParams = P_B3 + kOut;       // 5768
+
+// packed A fragments: [mt][kt][64 lanes] x 16 bytes
+constexpr int F_W1 = 0;                       // W1   (64 x 16): 2 x 1
+constexpr int F_W2 = F_W1 + 2 * 1 * 64;       // W2   (64 x 64): 2 x 4
+constexpr int F_W3 = F_W2 + 2 * 4 * 64;       // W3   (8->32 x 64): 1 x 4
+constexpr int F_W3T = F_W3 + 1 * 4 * 64;      // W3^T (64 x 8->16): 2 x 1
+constexpr int F_W2T = F_W3T + 2 * 1 * 64;     // W2^T (64 x 64): 2 x 4
+constexpr int kNumFrags = F_W2T + 2 * 4 * 64;
+
+struct MlpDev {
+    const uint4* frag;   // [kNumFrags]
+    const float* par;    // FP32 master parameters (biases read from here)
+};
+
+// one thread per 16-byte fragment
+__global__ void k_mlp_pack(const float* __restrict__ par, uint4* __restrict__ frag) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= kNumFrags) return;
+    int base, M, K, nkt, src, trans;
+    if (f < F_W2) { base = F_W1; M = kHid; K = kIn; nkt = 1; src = P_W1; trans = 0; }
+    else if (f < F_W3) { base = F_W2; M = kHid; K = kHid; nkt = 4; src = P_W2; trans = 0; }
+    else if (f < F_W3T) { base = F_W3; M = kOut; K = kHid; nkt = 4; src = P_W3; trans = 0; }
+    else if (f < F_W2T) { base = F_W3T; M = kHid; K = kOut; nkt = 1; src = P_W3; trans = 1; }
+    else { base = F_W2T; M = kHid; K = kHid; nkt = 4; src = P_W2; trans = 1; }
+    const int idx = f - base;
+    const int lane = idx & 63, kt = (idx >> 6) % nkt, mt = (idx >> 6) / nkt;
+    const int row = mt * 32 + (lane & 31);
+    uint16_t v[8];
+    for (int j = 0; j < 8; j++) {
+        const int k = kt * 16 + 8 * (lane >> 5) + j;
+        float w = 0.f;
+        if (row < M && k < K) {
+            // A = W: A[row][k] = W[row][k]; A = W^T: A[row][k] = W[k][row]
+            w = trans ? par[src + k * M + row] : par[src + row * K + k];
+        }
+        v[j] = f2bf(w);
+    }
+    frag[f] = make_uint4(v[0] | (uint32_t(v[1]) << 16), v[2] | (uint32_t(v[3]) << 16),
+                         v[4] | (uint32_t(v[5]) << 16), v[6] | (uint32_t(v[7]) << 16));
+}
+
+// LDS strides (elements)
+constexpr int kXld = 24, kHld = 72, kZld = 68;
+
+// store a 32x32 accumulator tile, after bias and GELU, into a sample-major
+// bf16 array H[sample][hid] (rows n0.., columns m0..)
+__device__ __forceinline__ void store_act_s(const f32x16& acc, const float* __restrict__ bias, uint16_t* H,
+                                            int m0, int n0) {
+    const int l = threadIdx.x & 63, r = l & 31, g = l >> 5;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int h0 = m0 + 8 * q + 4 * g;
+        uint16_t v[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) v[e] = f2bf(gelu(acc[4 * q + e] + bias[h0 + e]));
+        *reinterpret_cast<uint2*>(H + (n0 + r) * kHld + h0) =
+            make_uint2(v[0] | (uint32_t(v[1]) << 16), v[2] | (uint32_t(v[3]) << 16));
+    }
+}
+
+// Forward pass of one wave's 64 samples: X [64][kXld] (bf16) -> O [64][8]
+// (FP32, aliasing X).  H [64][kHld] holds the hidden activations.
+__device__ void mlp_forward_wave(const MlpDev& W, uint16_t* Xs, uint16_t* Hs, float* Os) {
+    const int l = threadIdx.x & 63, r = l & 31, g = l >> 5;
+    const float* b1 = W.par + P_B1;
+    const float* b2 = W.par + P_B2;
+    const float* b3 = W.par + P_B3;
+    wave_lds_sync();
+    for (int nt = 0; nt < 2; nt++)
+        for (int mt = 0; mt < 2; mt++) {
+            const f32x16 acc = tile_gl(W.frag + F_W1 + mt * 64, Xs, kXld, nt * 32, 1, zero16());
+            store_act_s(acc, b1, Hs, mt * 32, nt * 32);
+        }
+    wave_lds_sync();
+    for (int nt = 0; nt < 2; nt++) {
+        const f32x16 a0 = tile_gl(W.frag + F_W2 + 0 * 4 * 64, Hs, kHld, nt * 32, 4, zero16());
+        const f32x16 a1 = tile_gl(W.frag + F_W2 + 1 * 4 * 64, Hs, kHld, nt * 32, 4, zero16());
+        wave_lds_sync();
+        store_act_s(a0, b2, Hs, 0, nt * 32);
+        store_act_s(a1, b2, Hs, 32, nt * 32);
+        wave_lds_sync();
+    }
+    for (int nt = 0; nt < 2; nt++) {
+        const f32x16 acc = tile_gl(W.frag + F_W3, Hs, kHld, nt * 32, 4, zero16());
+#pragma unroll
+        for (int e = 0; e < 4; e++) Os[(nt * 32 + r) * kOut + 4 * g + e] = acc[e] + b3[4 * g + e];
+    }
+    wave_lds_sync();
+}
+
+// ========================================================== band envelope
+constexpr int kBands = 9;
+constexpr float kLogMargin = 1e-4f;
+// band edges in t - x: 2 (-log(1 - k/8)), k = 0..7, then +4 for the far tail
+__device__ __forceinline__ float band_edge(int k) {
+    switch (k) {
+        case 0: return 0.f;
+        case 1: return 0.26706278524904526f;
+        case 2: return 0.57536414490356180f;
+        case 3: return 0.94000725849147116f;
+        case 4: return 1.38629436111989062f;
+        case 5: return 1.96165850602344528f;
+        case 6: return 2.77258872223978124f;
+        case 7: return 4.15888308335967186f;
+        default: return 8.15888308335967186f;
+    }
+}
+
+// One band of the envelope, 8 bytes: log envelope c + s (t - x) (fp16,
+// rounded so that the stored line lies above the exact one on the band),
+// kappa (fp16) and beta (u8, beta = b / 255), both quantised before the
+// bounds are computed.
+struct BandTable {
+    uint32_t Q[8];        // band k is chosen when Q[k-1] <= word < Q[k] (Q[-1] = 0)
+    uint2 band[kBands];   // {c | s << 16, kappa | beta << 16}
+    int klast;            // last band of positive mass (chosen for word >= Q[klast - 1])
+};
+
+__device__ __forceinline__ uint16_t h_bits(_Float16 h) { return __builtin_bit_cast(uint16_t, h); }
+__device__ __forceinline__ float h2f(uint16_t b) { return static_cast<float>(__builtin_bit_cast(_Float16, b)); }
+__device__ __forceinline__ uint16_t f2h(float x) { return h_bits(static_cast<_Float16>(x)); }
+// smallest fp16 >= x (x finite and within the fp16 range)
+__device__ __forceinline__ uint16_t f2h_up(float x) {
+    uint16_t b = f2h(x);
+    if (h2f(b) < x) {
+        if (b == 0x8000u) b = 0x0001u;
+        else if (b & 0x8000u) b = static_cast<uint16_t>(b - 1u);
+        else b = static_cast<uint16_t>(b + 1u);
+    }
+    return b;
+}
+__device__ __forceinline__ void band_unpack(uint2 v, float& c, float& s, float& kap, float& beta) {
+    c = h2f(static_cast<uint16_t>(v.x & 0xFFFFu));
+    s = h2f(static_cast<uint16_t>(v.x >> 16));
+    kap = h2f(static_cast<uint16_t>(v.y & 0xFFFFu));
+    beta = static_cast<float>((v.y >> 16) & 0xFFu) * (1.f / 255.f);
+}
+
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.f), 1.f); }
+
+// Envelope of one hadron.  rho[k] in (0, 1) scales the band-midpoint
+// optimum kappa0_k.  Returns false when no envelope can be formed.  With
+// dz != nullptr also returns d log(M) / d z_k for rho = sigmoid(z)
+// (M = sum of the band masses, the expected number of proposals per
+// accepted hadron times the target mass).
+__device__ bool band_setup(const LParams& P, const float rho[kOut], BandTable& Bt, float* logM, float* dz) {
+    const float T = P.T, mu = P.mu, m = P.m, sg = P.sgn, x = P.x;
+    const float tau_max = P.tmax - x;
+    double mk[kBands];
+    float dS2[kOut], Mt_S1[kOut];
+    bool ok = tau_max > 0.f;
+    for (int k = 0; k < kBands; k++) {
+        mk[k] = 0.;
+        Bt.band[k] = make_uint2(0u, 0u);
+        if (k < kOut) { dS2[k] = 0.f; Mt_S1[k] = 0.f; }
+        const float a = fminf(band_edge(k), tau_max);
+        const float b = (k < kBands - 1) ? fminf(band_edge(k + 1), tau_max) : tau_max;
+        const float w = b - a;
+        if (!(w > 0.f)) continue;
+        const float ta = x + a, tb = x + b;
+        const float Elo = T * ta + mu, Ehi = T * tb + mu;
+        const float plo = sqrtf(fmaxf(Elo * Elo - m * m, 0.f)), phi = sqrtf(fmaxf(Ehi * Ehi - m * m, 0.f));
+        if (!(phi > 0.f) || !(Ehi > 0.f)) continue;
+        const float vlo = Elo > 0.f ? plo / Elo : 0.f, vhi = phi / Ehi;
+        // w1 / (1 + beta y) <= S1
+        float beta = 0.f;
+        if (P.dn > 0.f) beta = P.d0 > 0.f ? fminf(0.98f, vlo * P.dn / P.d0) : 0.98f;
+        const uint32_t beta_b = static_cast<uint32_t>(floorf(beta * 255.f));
+        beta = static_cast<float>(beta_b) * (1.f / 255.f);
+        float S1 = 0.f;
+        if (P.D > 0.f)
+            S1 = fmaxf(clamp01((P.d0 + vhi * P.dn) / P.D) / (1.f + beta),
+                       clamp01((P.d0 - vlo * P.dn) / P.D) / (1.f - beta));
+        // 1 - sign f0 = e^t f0 = 1 / (1 + sign e^-t): monotone in t
+        const float ea = expf(-ta), eb = expf(-tb);
+        const float oma = 1.f / (1.f + sg * ea), omb = 1.f / (1.f + sg * eb);
+        const float om_lo = fminf(oma, omb), om_hi = fmaxf(oma, omb);
+        // bulk: sup of omf Q(E) over the band
+        float Qmax = fmaxf(qb_of(P, Elo), qb_of(P, Ehi));
+        if (P.qi == 0.f && P.q2 != 0.f) {
+            const float Es = -P.q1 / (2.f * P.q2);
+            if (Es > Elo && Es < Ehi) Qmax = fmaxf(Qmax, qb_of(P, Es));
+        } else if (P.qi != 0.f && P.q2 == 0.f && P.q1 != 0.f && P.qi / P.q1 > 0.f) {
+            const float Es = sqrtf(P.qi / P.q1);
+            if (Es > Elo && Es < Ehi) Qmax = fmaxf(Qmax, qb_of(P, Es));
+        }
+        const float Amax = 1.f + (Qmax >= 0.f ? om_hi * Qmax : om_lo * Qmax);
+        // shear: c(E) = omf p^2 Ps(E) in [cmin, cmax]
+        float cmin = 0.f, cmax = 0.f;
+        const bool shear = P.smode != 0 && P.l1 > 0.f;
+        if (shear) {
+            const float pa = ps_of(P, fmaxf(Elo, 1e-30f)), pb = ps_of(P, Ehi);
+            const float psmin = fminf(pa, pb), psmax = fmaxf(pa, pb);
+            if (psmin >= 0.f) { cmin = om_lo * plo * plo * psmin; cmax = om_hi * phi * phi * psmax; }
+            else if (psmax <= 0.f) { cmin = om_hi * phi * phi * psmin; cmax = om_lo * plo * plo * psmax; }
+            else { cmin = om_hi * phi * phi * psmin; cmax = om_hi * phi * phi * psmax; }
+        }
+        // kappa: the network's share of the band-midpoint optimum
+        float kap = 0.f;
+        bool clipped = true;
+        float kap0 = 0.f;
+        if (k < kOut && shear && P.l3 < 0.f) {
+            const float tm = x + 0.5f * (a + b);
+            const float Em = T * tm + mu;
+            const float pm2 = fmaxf(Em * Em - m * m, 0.f);
+            const float omm = 1.f / (1.f + sg * expf(-tm));
+            const float cm = omm * pm2 * ps_of(P, Em);
+            const float am = 1.f + omm * qb_of(P, Em);
+            kap0 = am > 0.f ? cm / am : 0.f;
+            const float raw = rho[k] * kap0;
+            const float lo = -0.98f / P.l1, hi = 0.98f / (-P.l3);
+            kap = h2f(f2h(fminf(fmaxf(raw, lo), hi)));
+            if (1.f + kap * P.l1 < 0.01f || 1.f + kap * P.l3 < 0.01f) kap = 0.f;
+            clipped = !(raw > lo && raw < hi);
+        }
+        // w2 / (1 + kappa W) <= S2: W in [l3, l1]; the numerator's bound is
+        // linear on W <= 0 (slope cmin) and W >= 0 (slope cmax), then clamped
+        float S2 = 0.f, Wst = 0.f, Nst = 0.f;
+        {
+            float cand[5];
+            int nc = 0;
+            cand[nc++] = shear ? P.l3 : 0.f;
+            cand[nc++] = 0.f;
+            cand[nc++] = shear ? P.l1 : 0.f;
+            if (shear && cmax != 0.f) {
+                const float wc = (2.f - Amax) / cmax;
+                if (wc > 0.f && wc < P.l1) cand[nc++] = wc;
+            }
+            if (shear && cmin != 0.f) {
+                const float wc = (2.f - Amax) / cmin;
+                if (wc < 0.f && wc > P.l3) cand[nc++] = wc;
+            }
+            for (int i = 0; i < nc; i++) {
+                const float W = cand[i];
+                const float num = clamp01(0.5f * (Amax + (W >= 0.f ? cmax * W : cmin * W)));
+                const float v = num / (1.f + kap * W);
+                if (v > S2) { S2 = v; Wst = W; Nst = num; }
+            }
+        }
+        // |p| envelope exp(c + s tau)
+        float cc, ss;
+        const float U = -T * T * (Ehi * Ehi + m * m) / (phi * phi * phi * phi) - T * T / (Ehi * Ehi) +
+                        (sg < 0.f ? ea / ((1.f - ea) * (1.f - ea)) : 0.f);
+        const float tau_s = (k < kBands - 1) ? 0.5f * (a + b) : fminf(a + 1.f, b);
+        const float ts = x + tau_s;
+        const float Es = T * ts + mu;
+        const float ps2 = Es * Es - m * m;
+        if (U <= 0.f && ps2 > 0.f) {
+            const float es = expf(-ts);
+            const float Gs = logf(sqrtf(ps2) * Es) - tau_s - log1pf(sg * es);
+            ss = T * Es / ps2 + T / Es - 1.f + sg * es / (1.f + sg * es);
+            cc = Gs - ss * tau_s;
+        } else {
+            ss = -1.f;
+            cc = logf(phi * Ehi * om_hi);
+        }
+        const float sw = ss * w;
+        const float Mt = expf(cc + ss * a) * (fabsf(sw) > 1e-6f ? expm1f(sw) / ss : w);
+        if (!isfinite(cc) || !isfinite(ss) || !isfinite(Mt) || !(Mt >= 0.f)) { ok = false; continue; }
+        if (!(S1 * S2 > 0.f)) continue;
+        // stored line: fp16 slope, intercept raised to stay above the exact
+        // line at both band ends, then rounded up
+        const float c0 = cc + logf(S1 * S2) + kLogMargin;
+        const uint16_t sb = f2h(ss);
+        const float s16 = h2f(sb);
+        const float c1 = c0 + fmaxf(0.f, fmaxf((ss - s16) * a, (ss - s16) * b));
+        if (!(fabsf(c1) < 6.0e4f) || !(fabsf(ss) < 6.0e4f)) { ok = false; continue; }
+        const uint16_t cb = f2h_up(c1);
+        const float c16 = h2f(cb);
+        const float sw16 = s16 * w;
+        mk[k] = exp(static_cast<double>(c16) + static_cast<double>(s16) * a) *
+                (fabs(sw16) > 1e-6 ? expm1(static_cast<double>(sw16)) / static_cast<double>(s16) : static_cast<double>(w));
+        Bt.band[k] = make_uint2(static_cast<uint32_t>(cb) | (static_cast<uint32_t>(sb) << 16),
+                                static_cast<uint32_t>(f2h(kap)) | (beta_b << 16));
+        if (k < kOut) {
+            Mt_S1[k] = Mt * S1 * expf(kLogMargin);
+            dS2[k] = (clipped || !shear) ? 0.f : -Nst * Wst / ((1.f + kap * Wst) * (1.f + kap * Wst)) * kap0;
+        }
+    }
+    double M = 0.;
+    for (int k = 0; k < kBands; k++) M += mk[k];
+    if (!ok || !(M > 0.) || !isfinite(M)) {
+        if (logM) *logM = 0.f;
+        if (dz) for (int k = 0; k < kOut; k++) dz[k] = 0.f;
+        return false;
+    }
+    if (logM) *logM = static_cast<float>(log(M));
+    if (dz)
+        for (int k = 0; k < kOut; k++) dz[k] = static_cast<float>(Mt_S1[k] * dS2[k] / M) * rho[k] * (1.f - rho[k]);
+    // selection thresholds; every band of positive mass keeps a non-empty
+    // range and the last one reaches 2^32
+    const double two32 = 4294967296.;
+    int klast = 0;
+    for (int k = 0; k < kBands; k++)
+        if (mk[k] > 0.) klast = k;
+    uint64_t Qp = 0, Qk[kBands];
+    double cum = 0.;
+    for (int k = 0; k < kBands; k++) {
+        cum += mk[k];
+        uint64_t q;
+        if (k >= klast) {
+            q = static_cast<uint64_t>(two32);
+        } else {
+            q = static_cast<uint64_t>(floor(cum / M * two32));
+            if (mk[k] > 0.) q = q > Qp + 1 ? q : Qp + 1;
+            else q = Qp;
+            int after = 0;
+            for (int j = k + 1; j <= klast; j++) after += (mk[j] > 0.) ? 1 : 0;
+            const uint64_t cap = static_cast<uint64_t>(two32) - static_cast<uint64_t>(after);
+            if (q > cap) q = cap;
+        }
+        Qk[k] = q;
+        Qp = q;
+    }
+    Bt.klast = klast;
+    // the chosen probabilities are (Q_k - Q_{k-1}) / 2^32; rescale each
+    // band's envelope by g_k = m_k / (c_sel_k R), R = max_j m_j / c_sel_j
+    double R = 0.;
+    Qp = 0;
+    for (int k = 0; k < kBands; k++) {
+        const double cs = static_cast<double>(Qk[k] - Qp) / two32;
+        if (mk[k] > 0. && cs > 0.) R = fmax(R, mk[k] / cs);
+        Qp = Qk[k];
+    }
+    Qp = 0;
+    for (int k = 0; k < kBands; k++) {
+        const double cs = static_cast<double>(Qk[k] - Qp) / two32;
+        if (mk[k] > 0. && cs > 0.) {
+            const float c16 = h2f(static_cast<uint16_t>(Bt.band[k].x & 0xFFFFu));
+            const float cg = c16 - static_cast<float>(log(mk[k] / (cs * R)));
+            Bt.band[k].x = (Bt.band[k].x & 0xFFFF0000u) | f2h_up(fmaxf(cg, c16));
+        }
+        Qp = Qk[k];
+        if (k < kBands - 1) Bt.Q[k] = static_cast<uint32_t>(Qk[k] < 0xFFFFFFFFull ? Qk[k] : 0xFFFFFFFFull);
+    }
+    return true;
+}
+
+// ======================================================== learned sampler
+struct LearnArgs {
+    SampleArgs a;
+    const float* aux;
+    MlpDev W;
+    int64_t* fb_list;                 // slots handed to the plain sampler
+    unsigned long long* fb_count;
+    unsigned long long* stats2;       // [0] fallback hadrons, [1] envelope violations
+};
+
+// Per-hadron record of the learned sampler (128 bytes).
+struct alignas(16) LRec {
+    uint32_t cell;
+    uint32_t j;
+    uint16_t species;
+    uint8_t flags;        // 1: hole (table range), 2: plain-sampler fallback
+    uint8_t klast;
+    uint32_t pad;
+    uint64_t out;
+    uint32_t Q[8];
+    uint2 band[kBands];
+};
+static_assert(sizeof(LRec) == 128, "LRec layout");
+
+constexpr int kSetupThreads = 256;
+constexpr int kMlpBytes = (64 * kXld + 64 * kHld) * 2;
+
+// Set-up of the learned sampler, one wave per 64 consecutive slots: cell,
+// species and output slot exactly as k_setup, then the network inputs, the
+// network on MFMA and the band envelope.
+__global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec* __restrict__ rec) {
+    extern __shared__ __align__(16) char lds[];
+    const SampleArgs& a = L.a;
+    const int lane = threadIdx.x & 63;
+    char* wlds = lds + (threadIdx.x >> 6) * kMlpBytes;
+    uint16_t* Xs = reinterpret_cast<uint16_t*>(wlds);
+    uint16_t* Hs = Xs + 64 * kXld;
+    float* Os = reinterpret_cast<float*>(wlds);
+    const int64_t s0 = (static_cast<int64_t>(blockIdx.x) * (kSetupThreads / 64) + (threadIdx.x >> 6)) * 64;
+    if (s0 >= a.K) return;  // whole waves only: the MFMA below needs every lane
+    const int64_t i = s0 + lane;
+    const bool live = i < a.K;
+    int64_t c = 0;
+    int s = 0;
+    uint32_t jj = 0;
+    uint64_t outp = 0;
+    LParams P{};
+    float f[kIn];
+    for (int k = 0; k < kIn; k++) f[k] = 0.f;
+    if (live) {
+        c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
+        jj = static_cast<uint32_t>(i - a.off[c]);
+        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+        const double* row = a.cdf + c * a.Ns;
+        s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
+        if (s >= a.Ns) s = a.Ns - 1;
+        outp = a.perm(static_cast<uint64_t>(i));
+        lparams_of(a, L.aux, c, s, P);
+        features_of(a, c, s, P, f);
+    }
+    for (int k = 0; k < kIn; k += 4)
+        *reinterpret_cast<uint2*>(Xs + lane * kXld + k) =
+            make_uint2(f2bf(f[k]) | (uint32_t(f2bf(f[k + 1])) << 16), f2bf(f[k + 2]) | (uint32_t(f2bf(f[k + 3])) << 16));
+    mlp_forward_wave(L.W, Xs, Hs, Os);
+    if (!live) return;
+    float rho[kOut];
+    for (int k = 0; k < kOut; k++) rho[k] = 1.f / (1.f + expf(-Os[lane * kOut + k]));
+    BandTable B;
+    int flags = P.flags & 3;
+    if (!flags && !band_setup(P, rho, B, nullptr, nullptr)) flags = 2;
+    LRec r;
+    r.cell = static_cast<uint32_t>(c);
+    r.j = jj;
+    r.species = static_cast<uint16_t>(s);
+    r.flags = static_cast<uint8_t>(flags);
+    r.klast = static_cast<uint8_t>(flags ? 0 : B.klast);
+    r.pad = 0u;
+    r.out = outp;
+    for (int k = 0; k < 8; k++) r.Q[k] = flags ? 0u : B.Q[k];
+    for (int k = 0; k < kBands; k++) r.band[k] = flags ? make_uint2(0u, 0u) : B.band[k];
+    rec[i] = r;
+}
+
+// Persistent trial kernel of the learned sampler with per-lane refill (as
+// k_trials).  A lane keeps the hadron's parameters and band thresholds in
+// registers and reads the chosen band's 8 bytes per proposal.  Proposal
+// `it` of hadron (c, j) uses Philox blocks (c, j, TAG_LEARN, it) and
+// (c, j, TAG_LEARN2, it).
+__global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* __restrict__ rec) {
+    __shared__ unsigned long long s_stats[6];
+    if (threadIdx.x < 6) s_stats[threadIdx.x] = 0ull;
+    __syncthreads();
+    const SampleArgs& a = L.a;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int64_t base = a.K / nwaves, rem = a.K % nwaves;
+    const int64_t beg = wid * base + min(wid, rem);
+    const int64_t end = beg + base + (wid < rem ? 1 : 0);
+    const int64_t Nc = a.Nc;
+    const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
+    const uint32_t tw1 = tag_word(TAG_LEARN, a.chunk), tw2 = tag_word(TAG_LEARN2, a.chunk);
+    // one proposal is one outer trial of the reference (src/FSSW.cpp:1872)
+    const int max_it = a.max_trials;
+
+    int64_t next = beg;
+    unsigned long long n_it = 0, n_fail = 0, n_range = 0, n_fb = 0, n_viol = 0;
+
+    bool active = false;
+    int64_t c = 0, cur = 0;
+    uint32_t jj = 0;
+    int s = 0, it = 0, klast = 0;
+    uint64_t outp = 0;
+    uint32_t Q[8];
+    for (int k = 0; k < 8; k++) Q[k] = 0u;
+    LParams P{};
+
+    while (true) {
+        const unsigned long long need = __ballot(!active);
+        if (need && next < end && (__popcll(need) >= kRefillMin || need == ~0ull)) {
+            const int64_t navail = end - next;
+            const int rank = __popcll(need & ((1ull << lane) - 1));
+            if (!active && rank < navail) {
+                cur = next + rank;
+                const LRec* r = rec + cur;
+                const uint4 h0 = *reinterpret_cast<const uint4*>(r);
+                c = h0.x;
+                jj = h0.y;
+                s = static_cast<int>(h0.z & 0xFFFFu);
+                const int flags = static_cast<int>((h0.z >> 16) & 0xFFu);
+                klast = static_cast<int>(h0.z >> 24);
+                const uint4 h1 = *(reinterpret_cast<const uint4*>(r) + 1);
+                outp = static_cast<uint64_t>(h1.x) | (static_cast<uint64_t>(h1.y) << 32);
+                if (flags & 1) {
+                    // m/T - mu/T outside the table: a hole, as in the plain sampler
+                    n_range++;
+                    Hadron h{};
+                    static_cast<Hadron*>(a.out)[outp] = h;
+                } else if (flags & 2) {
+                    n_fb++;
+                    const unsigned long long pos = atomicAdd(L.fb_count, 1ull);
+                    L.fb_list[pos] = cur;
+                } else {
+                    Q[0] = h1.z; Q[1] = h1.w;
+                    const uint4 h2 = *(reinterpret_cast<const uint4*>(r) + 2);
+                    const uint2 h3 = *(reinterpret_cast<const uint2*>(r) + 6);
+                    Q[2] = h2.x; Q[3] = h2.y; Q[4] = h2.z; Q[5] = h2.w; Q[6] = h3.x; Q[7] = h3.y;
+                    lparams_of(a, L.aux, c, s, P);
+                    it = 0;
+                    active = true;
+                }
+            }
+            next += min<int64_t>(__popcll(need), navail);
+        }
+        if (!__any(active)) {
+            if (next >= end) break;
+            continue;
+        }
+        if (!active) continue;
+        // ---- one proposal
+        const u32x4 w1r = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw1, static_cast<uint32_t>(it)}, key0, key1);
+        const u32x4 w2r = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw2, static_cast<uint32_t>(it)}, key0, key1);
+        it++;
+        n_it++;
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) k += (w1r.x >= Q[j]) ? 1 : 0;
+        k = min(k, klast);
+        float ck, sk, kk, bk;
+        band_unpack(rec[cur].band[k], ck, sk, kk, bk);
+        const float tau_max = P.tmax - P.x;
+        const float ba = fminf(band_edge(k), tau_max);
+        const float bb = (k < kBands - 1) ? fminf(band_edge(k + 1), tau_max) : tau_max;
+        const float bw = bb - ba;
+        // tau from exp(s tau) on [ba, bb]
+        const float ut = u24(w1r.y);
+        const float sw = sk * bw;
+        float tau = fabsf(sw) > 1e-5f ? ba + log1pf(ut * expm1f(sw)) / sk : ba + ut * bw;
+        tau = fminf(fmaxf(tau, ba), bb);
+        const float t = P.x + tau;
+        const float E = P.T * t + P.mu;
+        const float p2 = fmaxf(E * E - P.m * P.m, 0.f);
+        const float p = sqrtf(p2);
+        const float et = expf(-t);
+        const float fe = 1.f / (1.f + P.sgn * et);     // e^t f0 = 1 - sign f0
+        const float G = logf(p * E) - tau + logf(fe);
+        // direction in the shear eigenframe
+        const float P0 = (1.f + kk * P.l1) * (1.f / 3.f), P1 = P0 + (1.f + kk * P.l2) * (1.f / 3.f);
+        const float uy = u24(w2r.x);
+        int ax;
+        float rr;
+        if (uy < P0) { ax = 0; rr = uy / P0; }
+        else if (uy < P1) { ax = 1; rr = (uy - P0) / (P1 - P0); }
+        else { ax = 2; rr = (uy - P1) / (1.f - P1); }
+        rr = fminf(fmaxf(rr, 0.f), 1.f);
+        const float cz = cbrtf(2.f * rr - 1.f);
+        const float sz = sqrtf(fmaxf(1.f - cz * cz, 0.f));
+        float sph, cph;
+        __sincosf(2.f * static_cast<float>(kPi) * u24(w2r.y), &sph, &cph);
+        float n0, n1, n2;
+        if (ax == 0) { n0 = cz; n1 = sz * cph; n2 = sz * sph; }
+        else if (ax == 1) { n1 = cz; n2 = sz * cph; n0 = sz * sph; }
+        else { n2 = cz; n0 = sz * cph; n1 = sz * sph; }
+        float y = n0 * P.dh0 + n1 * P.dh1 + n2 * P.dh2;
+        if (u24(w1r.w) >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }
+        const float W = P.l1 * n0 * n0 + P.l2 * n1 * n1 + P.l3 * n2 * n2;
+        // the exact Cooper-Frye weights of the plain sampler
+        const float w1 = (P.D > 0.f) ? clamp01((E * P.d0 + p * P.dn * y) / (E * P.D)) : 0.f;
+        const float dfs = fe * p2 * W * ps_of(P, E);
+        const float dfb = fe * qb_of(P, E);
+        const float w2 = clamp01(0.5f * (1.f + dfs + dfb));
+        const float racc = expf(G - (ck + sk * tau)) * w1 * w2 / ((1.f + kk * W) * (1.f + bk * y));
+        if (racc > 1.f + 1e-5f) n_viol++;
+        bool done = false;
+        if (u24(w1r.z) < racc) {
+            const float R00 = L.aux[A_R00 * Nc + c], R01 = L.aux[A_R01 * Nc + c], R02 = L.aux[A_R02 * Nc + c];
+            const float R10 = L.aux[A_R10 * Nc + c], R11 = L.aux[A_R11 * Nc + c], R12 = L.aux[A_R12 * Nc + c];
+            const float R20 = L.aux[A_R20 * Nc + c], R21 = L.aux[A_R21 * Nc + c], R22 = L.aux[A_R22 * Nc + c];
+            const float px = p * (R00 * n0 + R01 * n1 + R02 * n2);
+            const float py = p * (R10 * n0 + R11 * n1 + R12 * n2);
+            const float pz = p * (R20 * n0 + R21 * n1 + R22 * n2);
+            emit_lab(a, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
+            done = true;
+        } else if (it >= max_it) {
+            n_fail++;
+            Hadron h{};
+            static_cast<Hadron*>(a.out)[outp] = h;
+            done = true;
+        }
+        if (done) active = false;
+    }
+    n_it = wave_sum(n_it);
+    n_fail = wave_sum(n_fail);
+    n_range = wave_sum(n_range);
+    n_fb = wave_sum(n_fb);
+    n_viol = wave_sum(n_viol);
+    if (lane == 0) {
+        atomicAdd(&s_stats[0], n_it);
+        atomicAdd(&s_stats[1], n_it);
+        atomicAdd(&s_stats[2], n_fail);
+        atomicAdd(&s_stats[3], n_range);
+        atomicAdd(&s_stats[4], n_fb);
+        atomicAdd(&s_stats[5], n_viol);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+    if (threadIdx.x >= 4 && threadIdx.x < 6 && s_stats[threadIdx.x])
+        atomicAdd(&L.stats2[threadIdx.x - 4], s_stats[threadIdx.x]);
+}
+
+// =============================================================== training
+struct TrainArgs {
+    SampleArgs a;                  // surface, species, tables, delta-f switches
+    const float* aux;
+    MlpDev W;
+    const uint32_t* cell;          // training set: (cell, species) of accepted hadrons
+    const uint16_t* species;
+    int64_t n;                     // training-set size
+    const int64_t* pick;           // explicit sample indices (or nullptr)
+    int direct;                    // sample i of the step is training sample i (else Philox draws)
+    int64_t npick;                 // samples in this step
+    uint64_t seed;
+    uint32_t step;
+    float* partial;                // [gridDim.x][kNumParams]
+    float* loss_part;              // [gridDim.x]: sum of log M, [gridDim.x + b]: valid samples
+    float* out_loss;               // optional per-sample log M
+    float* out_dz;                 // optional per-sample dlogM/dz [8]
+    float* out_x;                  // optional per-sample inputs [16] (after bf16 rounding)
+};
+
+constexpr int kTrainLds =
+    (64 * kXld + kIn * kHld + 4 * 64 * kHld + 64 * kXld + kOut * kHld + 3 * 64 * kHld) * 2 +
+    (2 * 64 * kZld + 64 * kOut) * 4;
+
+// bias gradient: sum over the 32 columns (samples) of one lane half
+__device__ __forceinline__ float half_sum(float v) {
+#pragma unroll
+    for (int d = 16; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Forward and backward pass for 64 samples per block (one wave); writes the
+// block's gradient of sum_s log M_s and its loss.
+__global__ __launch_bounds__(64) void k_mlp_train(TrainArgs t) {
+    extern __shared__ __align__(16) char lds[];
+    uint16_t* Xs = reinterpret_cast<uint16_t*>(lds);            // [64][kXld]
+    uint16_t* Xt = Xs + 64 * kXld;                              // [16][kHld]
+    uint16_t* H1s = Xt + kIn * kHld;                            // [64][kHld]
+    uint16_t* H1t = H1s + 64 * kHld;
+    uint16_t* H2s = H1t + 64 * kHld;
+    uint16_t* H2t = H2s + 64 * kHld;
+    uint16_t* dOs = H2t + 64 * kHld;                            // [64][kXld]
+    uint16_t* dOt = dOs + 64 * kXld;                            // [8][kHld]
+    uint16_t* dZ2s = dOt + kOut * kHld;                         // [64][kHld]
+    uint16_t* dZ2t = dZ2s + 64 * kHld;
+    uint16_t* dZ1t = dZ2t + 64 * kHld;
+    float* Z1 = reinterpret_cast<float*>(dZ1t + 64 * kHld);     // [64 hid][kZld]
+    float* Z2 = Z1 + 64 * kZld;
+    float* Os = Z2 + 64 * kZld;                                 // [64][8]
+    const int l = threadIdx.x & 63, r = l & 31, g = l >> 5;
+    const SampleArgs& a = t.a;
+    const float* par = t.W.par;
+    float* gp = t.partial + static_cast<int64_t>(blockIdx.x) * kNumParams;
+
+    // ---- inputs
+    const int64_t si = static_cast<int64_t>(blockIdx.x) * 64 + l;
+    const bool live = si < t.npick && t.n > 0;
+    float f[kIn];
+    for (int i = 0; i < kIn; i++) f[i] = 0.f;
+    LParams P{};
+    bool valid = false;
+    if (live) {
+        int64_t idx;
+        if (t.pick) idx = t.pick[si];
+        else if (t.direct) idx = si;
+        else {
+            const u32x4 w = philox4x32(u32x4{static_cast<uint32_t>(si), t.step, tag_word(TAG_TRAIN, 0), 0u},
+                                       static_cast<uint32_t>(t.seed), static_cast<uint32_t>(t.seed >> 32));
+            idx = static_cast<int64_t>((static_cast<uint64_t>(w.x) << 32 | w.y) % static_cast<uint64_t>(t.n));
+        }
+        const int64_t c = t.cell[idx];
+        const int s = t.species[idx];
+        lparams_of(a, t.aux, c, s, P);
+        valid = !(P.flags & 3);
+        if (valid) features_of(a, c, s, P, f);
+    }
+    for (int i = 0; i < kIn; i++) {
+        const uint16_t h = f2bf(f[i]);
+        Xs[l * kXld + i] = h;
+        Xt[i * kHld + l] = h;
+        if (t.out_x && live) t.out_x[si * kIn + i] = bf2f(h);
+    }
+    wave_lds_sync();
+    // ---- forward
+    for (int nt = 0; nt < 2; nt++)
+        for (int mt = 0; mt < 2; mt++) {
+            const f32x16 acc = tile_gl(t.W.frag + F_W1 + mt * 64, Xs, kXld, nt * 32, 1, zero16());
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int h = mt * 32 + 8 * (i >> 2) + 4 * g + (i & 3), s = nt * 32 + r;
+                const float z = acc[i] + par[P_B1 + h];
+                Z1[h * kZld + s] = z;
+                const uint16_t hb = f2bf(gelu(z));
+                H1s[s * kHld + h] = hb;
+                H1t[h * kHld + s] = hb;
+            }
+        }
+    wave_lds_sync();
+    for (int nt = 0; nt < 2; nt++)
+        for (int mt = 0; mt < 2; mt++) {
+            const f32x16 acc = tile_gl(t.W.frag + F_W2 + mt * 4 * 64, H1s, kHld, nt * 32, 4, zero16());
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int h = mt * 32 + 8 * (i >> 2) + 4 * g + (i & 3), s = nt * 32 + r;
+                const float z = acc[i] + par[P_B2 + h];
+                Z2[h * kZld + s] = z;
+                const uint16_t hb = f2bf(gelu(z));
+                H2s[s * kHld + h] = hb;
+                H2t[h * kHld + s] = hb;
+            }
+        }
+    wave_lds_sync();
+    for (int nt = 0; nt < 2; nt++) {
+        const f32x16 acc = tile_gl(t.W.frag + F_W3, H2s, kHld, nt * 32, 4, zero16());
+#pragma unroll
+        for (int e = 0; e < 4; e++) Os[(nt * 32 + r) * kOut + 4 * g + e] = acc[e] + par[P_B3 + 4 * g + e];
+    }
+    wave_lds_sync();
+    // ---- loss log M and its gradient at the outputs
+    float dz[kOut], lm = 0.f;
+    for (int k = 0; k < kOut; k++) dz[k] = 0.f;
+    if (valid) {
+        float rho[kOut];
+        for (int k = 0; k < kOut; k++) rho[k] = 1.f / (1.f + expf(-Os[l * kOut + k]));
+        BandTable B;
+        if (!band_setup(P, rho, B, &lm, dz)) { valid = false; lm = 0.f; }
+    }
+    if (t.out_loss && live) t.out_loss[si] = valid ? lm : 0.f;
+    if (t.out_dz && live)
+        for (int k = 0; k < kOut; k++) t.out_dz[si * kOut + k] = dz[k];
+    for (int k = 0; k < kIn; k++) dOs[l * kXld + k] = k < kOut ? f2bf(dz[k]) : 0;
+    for (int k = 0; k < kOut; k++) dOt[k * kHld + l] = f2bf(dz[k]);
+    {
+        const float lsum = wave_sum(valid ? lm : 0.f);
+        const float nval = wave_sum(valid ? 1.f : 0.f);
+        for (int k = 0; k < kOut; k++) {
+            const float gb = wave_sum(dz[k]);
+            if (l == 0) gp[P_B3 + k] = gb;
+        }
+        if (l == 0) {
+            t.loss_part[blockIdx.x] = lsum;
+            t.loss_part[gridDim.x + blockIdx.x] = nval;
+        }
+    }
+    wave_lds_sync();
+    // ---- backward
+    // dW3[o][h] = sum_s dO[s][o] H2[s][h]
+    for (int nt = 0; nt < 2; nt++) {
+        const f32x16 acc = tile_ll(dOt, kHld, 0, kOut, H2t, kHld, nt * 32, 64, 4, zero16());
+#pragma unroll
+        for (int e = 0; e < 4; e++) gp[P_W3 + (4 * g + e) * kHid + nt * 32 + r] = acc[e];
+    }
+    // dH2^T = W3^T dO^T; dZ2 = dH2 * gelu'(Z2)
+    for (int mt = 0; mt < 2; mt++)
+        for (int nt = 0; nt < 2; nt++) {
+            const f32x16 acc = tile_gl(t.W.frag + F_W3T + mt * 64, dOs, kXld, nt * 32, 1, zero16());
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int h = mt * 32 + 8 * (i >> 2) + 4 * g + (i & 3), s = nt * 32 + r;
+                const float d = acc[i] * gelu_d(Z2[h * kZld + s]);
+                const uint16_t db = f2bf(d);
+                dZ2s[s * kHld + h] = db;
+                dZ2t[h * kHld + s] = db;
+                Z2[h * kZld + s] = d;  // keep FP32 dZ2 for the bias gradient
+            }
+        }
+    wave_lds_sync();
+    // bias 2: sum over samples
+    for (int h = 0; h < kHid; h++) {
+        const float v = Z2[h * kZld + l];
+        const float sv = wave_sum(v);
+        if (l == 0) gp[P_B2 + h] = sv;
+    }
+    // dW2[ho][hi] = sum_s dZ2[ho][s] H1[s][hi]
+    for (int mt = 0; mt < 2; mt++)
+        for (int nt = 0; nt < 2; nt++) {
+            const f32x16 acc = tile_ll(dZ2t, kHld, mt * 32, 64, H1t, kHld, nt * 32, 64, 4, zero16());
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int ho = mt * 32 + 8 * (i >> 2) + 4 * g + (i & 3), hi = nt * 32 + r;
+                gp[P_W2 + ho * kHid + hi] = acc[i];
+            }
+        }
+    // dH1^T = W2^T dZ2; dZ1 = dH1 * gelu'(Z1)
+    for (int mt = 0; mt < 2; mt++)
+        for (int nt = 0; nt < 2; nt++) {
+            const f32x16 acc = tile_gl(t.W.frag + F_W2T + mt * 4 * 64, dZ2s, kHld, nt * 32, 4, zero16());
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int h = mt * 32 + 8 * (i >> 2) + 4 * g + (i & 3), s = nt * 32 + r;
+                const float d = acc[i] * gelu_d(Z1[h * kZld + s]);
+                dZ1t[h * kHld + s] = f2bf(d);
+                Z1[h * kZld + s] = d;
+            }
+        }
+    wave_lds_sync();
+    for (int h = 0; h < kHid; h++) {
+        const float v = Z1[h * kZld + l];
+        const float sv = wave_sum(v);
+        if (l == 0) gp[P_B1 + h] = sv;
+    }
+    // dW1[h][i] = sum_s dZ1[h][s] X[s][i]
+    for (int mt = 0; mt < 2; mt++) {
+        const f32x16 acc = tile_ll(dZ1t, kHld, mt * 32, 64, Xt, kHld, 0, kIn, 4, zero16());
+        if (r < kIn) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int h = mt * 32 + 8 * (i >> 2) + 4 * g + (i & 3);
+                gp[P_W1 + h * kIn + r] = acc[i];
+            }
+        }
+    }
+}
+
+// Sum the blocks' gradients; with lr > 0 also take one Adam step on the
+// mean gradient (betas 0.9, 0.999, eps 1e-8).
+__global__ void k_mlp_reduce_adam(const float* __restrict__ partial, int nblk, const float* __restrict__ loss_part,
+                                  float* __restrict__ par, float* __restrict__ adam_m, float* __restrict__ adam_v,
+                                  float* __restrict__ grad_out, float lr, int t_step) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kNumParams) return;
+    float gsum = 0.f, nval = 0.f;
+    for (int b = 0; b < nblk; b++) {
+        gsum += partial[static_cast<int64_t>(b) * kNumParams + p];
+        nval += loss_part[nblk + b];
+    }
+    if (grad_out) grad_out[p] = gsum;
+    if (lr > 0.f && nval > 0.f) {
+        const float gm = gsum / nval;
+        const float m = 0.9f * adam_m[p] + 0.1f * gm;
+        const float v = 0.999f * adam_v[p] + 0.001f * gm * gm;
+        adam_m[p] = m;
+        adam_v[p] = v;
+        const float mh = m / (1.f - powf(0.9f, static_cast<float>(t_step)));
+        const float vh = v / (1.f - powf(0.999f, static_cast<float>(t_step)));
+        par[p] -= lr * mh / (sqrtf(vh) + 1e-8f);
+    }
+}
+
+// Training set: the (cell, species) of accepted hadrons of a plain pass, in
+// slot order; more than cap of them are thinned evenly.
+__global__ void k_collect_flag(const HSetup* __restrict__ rec, int64_t K, const Hadron* __restrict__ out,
+                               uint32_t* __restrict__ flag) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    flag[i] = out[rec[i].out].pid != 0 ? 1u : 0u;
+}
+__global__ void k_collect_write(const HSetup* __restrict__ rec, int64_t K, const uint32_t* __restrict__ flag,
+                                const int64_t* __restrict__ off, int64_t cap, uint32_t* __restrict__ cell,
+                                uint16_t* __restrict__ spec) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= K || !flag[i]) return;
+    const int64_t o = off[i], n = off[K];
+    int64_t pos = o;
+    if (n > cap) {
+        const int64_t lo = o * cap / n, hi = (o + 1) * cap / n;
+        if (hi == lo) return;
+        pos = lo;
+    }
+    cell[pos] = rec[i].cell;
+    spec[pos] = rec[i].species;
+}
+
+// Band tables for given (cell, species, rho): diagnostics and tests.
+// out[i]: [0] ok, [1] log M, [2..10] c, [11..19] s, [20..28] kappa,
+// [29..37] beta, [38..45] Q (as float of Q / 2^32), [46] last band of positive mass
+constexpr int kBandDiag = 47;
+__global__ void k_band_diag(SampleArgs a, const float* __restrict__ aux, const int32_t* __restrict__ cell,
+                            const int32_t* __restrict__ spec, const float* __restrict__ rho, int64_t n,
+                            float* __restrict__ out, float* __restrict__ lp) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    LParams P;
+    lparams_of(a, aux, cell[i], spec[i], P);
+    float rr[kOut];
+    for (int k = 0; k < kOut; k++) rr[k] = rho[i * kOut + k];
+    BandTable B{};
+    float lm = 0.f;
+    const bool ok = !(P.flags & 3) && band_setup(P, rr, B, &lm, nullptr);
+    float* o = out + i * kBandDiag;
+    o[0] = ok ? 1.f : 0.f;
+    o[1] = lm;
+    for (int k = 0; k < kBands; k++) {
+        float cc, ss, kk, bb;
+        band_unpack(ok ? B.band[k] : make_uint2(0u, 0u), cc, ss, kk, bb);
+        o[2 + k] = cc; o[11 + k] = ss; o[20 + k] = kk; o[29 + k] = bb;
+    }
+    for (int k = 0; k < kBands - 1; k++) o[38 + k] = ok ? static_cast<float>(B.Q[k] * (1.0 / 4294967296.0)) : 0.f;
+    o[46] = ok ? static_cast<float>(B.klast) : 0.f;
+    // the hadron parameters the tables were built from
+    float* q = lp + i * 24;
+    q[0] = P.T; q[1] = P.mu; q[2] = P.m; q[3] = P.sgn; q[4] = P.x; q[5] = P.tmax;
+    q[6] = P.d0; q[7] = P.dn; q[8] = P.D; q[9] = P.l1; q[10] = P.l2; q[11] = P.l3;
+    q[12] = P.dh0; q[13] = P.dh1; q[14] = P.dh2; q[15] = P.ps0; q[16] = P.qi; q[17] = P.q0;
+    q[18] = P.q1; q[19] = P.q2; q[20] = static_cast<float>(P.smode); q[21] = static_cast<float>(P.flags);
+    q[22] = 0.f; q[23] = 0.f;
+}
+
+}  // namespace issdev

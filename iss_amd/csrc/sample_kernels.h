@@ -8,7 +8,10 @@ namespace issdev {
 
 // ============================================================ counting pass
 
-enum StreamTag : uint32_t { TAG_COUNT = 1, TAG_MULTI = 2, TAG_SETUP = 3, TAG_TRIAL = 4 };
+enum StreamTag : uint32_t {
+    TAG_COUNT = 1, TAG_MULTI = 2, TAG_SETUP = 3, TAG_TRIAL = 4,
+    TAG_LEARN = 5, TAG_LEARN2 = 6, TAG_TRAIN = 7
+};
 
 __device__ __forceinline__ uint32_t tag_word(uint32_t tag, uint32_t chunk) {
     return (tag << 24) ^ (chunk & 0x00FFFFFFu);
@@ -126,6 +129,43 @@ struct Hadron {
     float mass, E, px, py, pz, t, x, y, z;
 };
 
+// Boost a rest-frame momentum to the lab frame (src/FSSW.cpp:2001-2010) and
+// write the hadron with its space-time position (src/FSSW.cpp:1969-1996).
+// Boost-invariant surfaces draw y uniform in [y_LB, y_RB] and set
+// eta_s = y - (y - eta_s) (src/FSSW.cpp:1024-1029).
+__device__ __forceinline__ void emit_lab(const SampleArgs& a, int64_t c, uint32_t jj, int pid, float fm,
+                                         float px, float py, float pz, float p0, uint64_t outp) {
+    const int64_t Nc = a.Nc;
+    const float u0 = a.cells[F_U0 * Nc + c], u1 = a.cells[F_U1 * Nc + c],
+                u2 = a.cells[F_U2 * Nc + c], u3 = a.cells[F_U3 * Nc + c];
+    const float pdu = px * u1 + py * u2 + pz * u3;
+    const float k1 = pdu / (u0 + 1.f) + p0;
+    const float lx = px + k1 * u1, ly = py + k1 * u2, lz = pz + k1 * u3;
+    const float mT = sqrtf(lx * lx + ly * ly + fm * fm);
+    const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
+    float eta_s, E, pzl;
+    if (a.boost_invariant) {
+        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk) ^ 0x00800000u);
+        const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
+        eta_s = rap - (asinhf(lz / mT) - eta);
+        E = mT * coshf(rap);
+        pzl = mT * sinhf(rap);
+    } else {
+        eta_s = eta;
+        E = sqrtf(mT * mT + lz * lz);
+        pzl = lz;
+    }
+    Hadron h;
+    h.pid = pid;
+    h.mass = fm;
+    h.E = E; h.px = lx; h.py = ly; h.pz = pzl;
+    h.t = tau * coshf(eta_s);
+    h.x = a.cells[F_X * Nc + c];
+    h.y = a.cells[F_Y * Nc + c];
+    h.z = tau * sinhf(eta_s);
+    static_cast<Hadron*>(a.out)[outp] = h;
+}
+
 // CDF_0, CDF_1, CDF_2 of one |p| table at E~ together: the per-term
 // exponentials become powers of exp(m0 - E~) and exp(-m0), so one hadron's
 // normalisation costs two exponentials and a logarithm.
@@ -195,9 +235,16 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 // Set-up of every hadron slot, one thread per slot (no divergence): the cell
 // (offsets), the species (the cell's FP64 species CDF), the |p| table and its
 // normalisation at E~ = m~ - mu~, and the output slot.
-__global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict__ rec) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.K) return;
+// With a slot list, record r describes slot list[r] for r < *nlist (the
+// learned sampler's fallback hadrons, grid-stride); without one, record i
+// describes slot i.
+__global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict__ rec,
+                                               const int64_t* __restrict__ list,
+                                               const unsigned long long* __restrict__ nlist) {
+    const int64_t n = list ? static_cast<int64_t>(*nlist) : a.K;
+    for (int64_t ri = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ri < n;
+         ri += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = list ? list[ri] : ri;
     const int64_t Nc = a.Nc;
     const int64_t c = upper_bound(a.off + 1, int64_t(0), Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
@@ -220,7 +267,8 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     r.flags = (r.i0 < 0 || r.i0 >= a.pt.len[tb] - 1) ? 1 : 0;
     r.mterm = cm[2] + 2. * q.mut * cm[1] + q.aa * cm[0];
     r.out = a.perm(static_cast<uint64_t>(i));
-    rec[i] = r;
+    rec[ri] = r;
+    }
 }
 
 // Persistent trial kernel with per-lane refill.  Each wavefront owns a
@@ -244,7 +292,8 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int64_t base = a.K / nwaves, rem = a.K % nwaves;
+    const int64_t Kn = a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K;
+    const int64_t base = Kn / nwaves, rem = Kn % nwaves;
     const int64_t beg = wid * base + min(wid, rem);
     const int64_t end = beg + base + (wid < rem ? 1 : 0);
     const int64_t Nc = a.Nc;
@@ -401,37 +450,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
             const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
             const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
             if (ua < w1 * w2) {
-                // boost to the lab frame (src/FSSW.cpp:2001-2010)
-                const float u0 = a.cells[F_U0 * Nc + c], u1 = a.cells[F_U1 * Nc + c],
-                            u2 = a.cells[F_U2 * Nc + c], u3 = a.cells[F_U3 * Nc + c];
-                const float pdu = px * u1 + py * u2 + pz * u3;
-                const float k1 = pdu / (u0 + 1.f) + p0;
-                const float lx = px + k1 * u1, ly = py + k1 * u2, lz = pz + k1 * u3;
-                const float mT = sqrtf(lx * lx + ly * ly + fm * fm);
-                const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
-                float eta_s, E, pzl;
-                if (a.boost_invariant) {
-                    // y uniform in [y_LB, y_RB], eta_s = y - (y - eta_s)
-                    // (src/FSSW.cpp:1024-1029)
-                    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk) ^ 0x00800000u);
-                    const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
-                    eta_s = rap - (asinhf(lz / mT) - eta);
-                    E = mT * coshf(rap);
-                    pzl = mT * sinhf(rap);
-                } else {
-                    eta_s = eta;
-                    E = sqrtf(mT * mT + lz * lz);
-                    pzl = lz;
-                }
-                Hadron h;
-                h.pid = a.sp.pid[s];
-                h.mass = fm;
-                h.E = E; h.px = lx; h.py = ly; h.pz = pzl;
-                h.t = tau * coshf(eta_s);
-                h.x = a.cells[F_X * Nc + c];
-                h.y = a.cells[F_Y * Nc + c];
-                h.z = tau * sinhf(eta_s);
-                static_cast<Hadron*>(a.out)[outp] = h;
+                emit_lab(a, c, jj, a.sp.pid[s], fm, px, py, pz, p0, outp);
                 done = true;
             } else {
                 trials++;
