@@ -1,14 +1,20 @@
-"""Benchmark: sampled hadrons/s and yield-pass GB/s on one MI355X.
+"""Benchmark: sampled hadrons/s, learned proposal against the plain GPU
+sampler, and yield-pass GB/s, on one MI355X.
 
 Workload (the configuration the metric is quoted on): a viscous 3+1D
 freeze-out surface of 200,000 cells (tau in [0.6, 15] fm, eta_s in [-5, 5],
 T in [0.135, 0.160] GeV, mu_B in [0, 0.3] GeV, |u_perp| <= 3, random
 traceless pi with |pi|/(e+P) <= 0.3, bulk Pi/(e+P) in [-0.2, 0], OSU
-14-moment bulk + shear delta f), the 321-species UrQMD hadron list, ~1e4
-hadrons per event.  One step = one yield pass (FP32 series, FP64 species
-CDF) + one sampling pass of 1000 oversampled events, with the surface
-already resident in HBM.  `value` = hadrons sampled per second over the
-timed steps (yields included).
+14-moment bulk + shear delta f), the 321-species UrQMD hadron list,
+normalised to 1e4 hadrons per event.  One step = one yield pass (FP32
+series, FP64 species CDF) + one sampling pass of 1000 oversampled events
+(~1e7 hadrons), surface resident in HBM.  The plain steps run first; the
+network is then trained on the 1e7 hadrons the last plain pass accepted
+(outside the timed region), and the learned steps run on the same surface.
+`value` = hadrons/s of the learned-proposal steps (yields included); the
+plain steps' rate and the ratio are reported beside it.  Both samplers use
+the physical Fermi-Dirac |p| proposal (fermion_cdf2_quirk = 0) so that every
+hadron goes through the proposal being measured.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -27,7 +33,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
-FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed); 78.6 unpacked
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector, packed (78.6 unpacked)
+
+# algorithmic FP32 operations (FMA = 2; add, mul, compare, min/max, select,
+# reciprocal, sqrt, exp, log, sin/cos = 1 each), counted from the kernels'
+# arithmetic; random-number and index arithmetic are not counted
+OPS_PLAIN_PROPOSAL = 115.0     # |p| proposal: table search step terms, ratio test
+OPS_PLAIN_TRIAL = 85.0         # direction, f0, delta f weights, w1 w2 test
+OPS_ACCEPTED = 60.0            # boost to the lab frame and the record
+OPS_PLAIN_SETUP = 190.0        # species pick, |p| normalisation (FP64 counted as FP32)
+OPS_LEARN_SETUP = 2900.0       # 14 inputs, 128 GELU, 9-band envelope, thresholds
+OPS_LEARN_PROPOSAL = 250.0     # band pick, t, f0, direction, weights, envelope ratio
 
 
 def yield_bytes(n_cells: int, n_species: int, with_bulk: bool) -> int:
@@ -36,13 +52,6 @@ def yield_bytes(n_cells: int, n_species: int, with_bulk: bool) -> int:
     per (cell, species) one FP64 species-CDF entry; per cell its FP64 total."""
     per_cell_in = 6 * 4 + (3 * 8 if with_bulk else 0)
     return n_cells * (per_cell_in + 8) + n_cells * n_species * 8
-
-
-def sample_flops(n_trials: int, n_inner: int, n_accepted: int) -> float:
-    """Algorithmic FP32 ops of the sampling pass (FMA = 2, other ops and
-    transcendentals = 1): ~115 per |p| proposal, ~85 per outer trial, ~250
-    per accepted hadron (cell/species selection, boost, record)."""
-    return 115.0 * n_inner + 85.0 * n_trials + 250.0 * n_accepted
 
 
 def cpu_model() -> str:
@@ -55,28 +64,75 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def cpu_baseline(cells, sp, seconds_target=12.0):
-    """The CPU oracle (a 1-thread restatement of the reference algorithm) on a
-    bounded sample of the same workload: the first 2000 cells, 14-moment
-    bulk + shear delta f, sampled for as many events as fit ~10 s."""
+def cpu_baseline(cells, seconds_target=15.0):
+    """The CPU oracle (a 1-thread restatement of the reference algorithm:
+    yields, Poisson multiplicities, |p| tables and rejection, boost) on a
+    bounded sample of the same workload: the first 2000 cells of the
+    surface, as many events as fill ~15 s."""
     from tests import oracle_binding as ob
-    from iss_amd import tables  # noqa: F401
 
     sub = cells[:2000]
-    orc = ob.Oracle(afterburner=1, shear=1, bulk=1, bulk_kind=11)
+    orc = ob.Oracle(afterburner=1, shear=1, bulk=1, bulk_kind=11, fd_quirk=0)
     orc.set_cells_lrf(sub)
     t0 = time.perf_counter()
-    _, h = orc.sample(1, 10)
+    _, h = orc.sample(1, 200)
     dt = time.perf_counter() - t0
+    per_event = max(h.size / 200.0, 1e-3)
     rate0 = max(h.size / max(dt, 1e-9), 1.0)
-    nev = int(max(10, min(100000, seconds_target * rate0 / max(h.size / 10, 1))))
+    nev = int(np.clip(seconds_target * rate0 / per_event, 200, 2_000_000))
     t0 = time.perf_counter()
     _, h = orc.sample(2, nev)
     dt = time.perf_counter() - t0
     return {"value": h.size / dt, "unit": "hadrons/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (yields + sampling, 1 thread) on the first 2000 cells of the "
-                      f"workload, {nev} events, {h.size} hadrons in {dt:.1f} s; host {cpu_model()}, "
+            "sample": f"oracle (yields + sampling, 1 thread) on the first 2000 of the 200,000 cells, "
+                      f"{nev} events, {h.size} hadrons in {dt:.1f} s; host {cpu_model()}, "
                       f"nproc {os.cpu_count()}"}
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated), or None."""
+    f = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(f) as fh:
+            return json.load(fh).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def run_steps(ctx, args, proposal, first_event):
+    def step(i):
+        ctx.compute_yields(precision=args.precision)
+        n = ctx.sample(args.events, seed=12345, event0=first_event + i * args.events,
+                       precision=args.precision, proposal=proposal)
+        return n, ctx.stats()
+
+    for i in range(args.warmup):
+        step(i)
+    acc = {"hadrons": 0, "trials": 0, "inner": 0, "ms_yield": [], "ms_sample": [], "ms_setup": [],
+           "ms_trials": [], "violations": 0, "fallback": 0}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        n, st = step(args.warmup + i)
+        acc["hadrons"] += n
+        acc["trials"] += st.n_trials
+        acc["inner"] += st.n_inner
+        acc["ms_yield"].append(st.ms_yield_kernel)
+        acc["ms_sample"].append(st.ms_sample)
+        acc["ms_setup"].append(st.ms_setup_kernel)
+        acc["ms_trials"].append(st.ms_trials_kernel)
+        acc["violations"] += st.n_violations
+        acc["fallback"] += st.n_fallback
+    acc["elapsed"] = time.perf_counter() - t0
+    for k in ("ms_yield", "ms_sample", "ms_setup", "ms_trials"):
+        acc[k] = float(np.mean(acc[k]))
+    return acc
+
+
+def roofline_valu(kernel, ops, ms):
+    tf = ops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms}
 
 
 def main():
@@ -87,6 +143,7 @@ def main():
     ap.add_argument("--cells", type=int, default=200000)
     ap.add_argument("--events", type=int, default=1000)
     ap.add_argument("--precision", type=int, default=32)
+    ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -101,73 +158,74 @@ def main():
     cells, _ = synth.viscous_lrf(args.cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4)
     ctx = iss_amd.Context(0)
     ctx.set_species(sp)
-    ctx.set_deltaf(shear=1, bulk=1, bulk_kind=11, k11=tables.read_kind11(1))
+    ctx.set_deltaf(shear=1, bulk=1, bulk_kind=11, k11=tables.read_kind11(1), fermion_cdf2_quirk=0)
     ctx.set_surface_lrf(cells)
-    # normalise the surface to 1e4 hadrons per event including delta f
     tot = ctx.compute_yields(precision=64)
     cells[:, 4:8] *= np.float32(1.0e4 / tot.sum())
     ctx.set_surface_lrf(cells)
     nc, ns = ctx.num_cells(), ctx.n_species
 
-    def step(i):
-        ctx.compute_yields(precision=args.precision)
-        n = ctx.sample(args.events, seed=12345, event0=i * args.events, precision=args.precision)
-        return n, ctx.stats()
+    plain = run_steps(ctx, args, 0, 0)
+    # training set: the hadrons accepted by the last plain pass
+    n_train = ctx.mlp_collect(10_000_000)
+    ctx.mlp_init(1)
+    rep = ctx.mlp_train(batch=8192, steps=args.train_steps, lr=3e-3, seed=5)
+    learned = run_steps(ctx, args, 1, (args.warmup + args.steps) * args.events)
 
-    for i in range(args.warmup):
-        step(i)
-    tot_h = 0
-    ky, ks, trials, inner, stot = [], [], 0, 0, []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        n, st = step(args.warmup + i)
-        tot_h += n
-        ky.append(st.ms_yield_kernel)
-        ks.append(st.ms_sample_kernel)
-        stot.append(st.ms_sample)
-        trials += st.n_trials
-        inner += st.n_inner
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_step = 1e3 * elapsed / args.steps
-    ky_ms, ks_ms = float(np.mean(ky)), float(np.mean(ks))
+    v_learn = learned["hadrons"] / learned["elapsed"]
+    v_plain = plain["hadrons"] / plain["elapsed"]
+    hl, hp = learned["hadrons"] / args.steps, plain["hadrons"] / args.steps
     ybytes = yield_bytes(nc, ns, True)
-    y_gbs = ybytes / (ky_ms * 1e-3) / 1e9
-    s_tflops = sample_flops(trials, inner, tot_h) / args.steps / (ks_ms * 1e-3) / 1e12
+    y_gbs = ybytes / (learned["ms_yield"] * 1e-3) / 1e9
     roof_y = {"bound": "hbm", "achieved": y_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-              "frac": y_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_yield",
-              "ms": ky_ms, "bytes": ybytes}
-    roof_s = {"bound": "valu", "achieved": s_tflops, "peak": FP32_VALU_PEAK_TFLOPS,
-              "unit": "TFLOP/s", "frac": s_tflops / FP32_VALU_PEAK_TFLOPS, "traffic": None,
-              "kernel": "k_sample", "ms": ks_ms}
-    dominant = roof_s if ks_ms >= ky_ms else roof_y
+              "frac": y_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles("k_yield"), "kernel": "k_yield",
+              "ms": learned["ms_yield"], "bytes": ybytes}
+    l_setup = roofline_valu("k_learn_setup", OPS_LEARN_SETUP * hl, learned["ms_setup"])
+    l_trials = roofline_valu("k_learn_trials", OPS_LEARN_PROPOSAL * learned["inner"] / args.steps +
+                             OPS_ACCEPTED * hl, learned["ms_trials"])
+    p_setup = roofline_valu("k_setup", OPS_PLAIN_SETUP * hp, plain["ms_setup"])
+    p_trials = roofline_valu("k_trials", (OPS_PLAIN_PROPOSAL * plain["inner"] + OPS_PLAIN_TRIAL * plain["trials"])
+                             / args.steps + OPS_ACCEPTED * hp, plain["ms_trials"])
+    cands = [(learned["ms_yield"], roof_y), (learned["ms_setup"], l_setup), (learned["ms_trials"], l_trials)]
+    dominant = max(cands, key=lambda t: t[0])[1]
     out = {
         "metric": "sampled hadrons/s (learned proposal vs plain GPU) at fixed chi2; yield-pass GB/s",
-        "value": tot_h / elapsed,
+        "value": v_learn,
         "unit": "hadrons/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * learned["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == 32 else "f64",
         "data": "synthetic",
-        "config": {"workload": "viscous 3+1D surface, plain rejection proposal", "cells": nc,
-                   "species": ns, "events_per_step": args.events,
-                   "hadrons_per_event": tot_h / (args.steps * args.events),
-                   "deltaf": "shear + OSU 14-moment bulk", "proposal": "plain"},
+        "config": {"workload": "viscous 3+1D surface, 200k cells, learned proposal", "cells": nc,
+                   "species": ns, "events_per_step": args.events, "hadrons_per_event": hl / args.events,
+                   "deltaf": "shear + OSU 14-moment bulk", "proposal": "learned (trained on the plain "
+                   "sampler's accepted hadrons)", "train_samples": n_train, "train_steps": rep["steps"]},
         "roofline": dominant,
+        "cpu_baseline": None,
+        "plain_hadrons_per_s": v_plain,
+        "learned_vs_plain": v_learn / v_plain,
+        "sample_pass_hadrons_per_s": {"learned": hl / (1e-3 * learned["ms_sample"]),
+                                      "plain": hp / (1e-3 * plain["ms_sample"])},
+        "proposals_per_hadron": {"learned": learned["inner"] / learned["hadrons"],
+                                 "plain": plain["inner"] / plain["hadrons"]},
+        "envelope_violations": learned["violations"],
+        "fallback_hadrons": learned["fallback"],
+        "training": rep,
+        "kernels_ms": {"k_yield": learned["ms_yield"], "k_learn_setup": learned["ms_setup"],
+                       "k_learn_trials": learned["ms_trials"], "k_setup": plain["ms_setup"],
+                       "k_trials": plain["ms_trials"]},
         "roofline_yield": roof_y,
-        "roofline_sample": roof_s,
-        "sample_hadrons_per_s": tot_h / (1e-3 * float(np.sum(stot))),
+        "roofline_sample": {"k_learn_setup": l_setup, "k_learn_trials": l_trials, "k_setup": p_setup,
+                            "k_trials": p_trials},
         "yield_GBs": y_gbs,
-        "trials_per_hadron": trials / max(tot_h, 1),
-        "inner_per_hadron": inner / max(tot_h, 1),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cells, sp)
+        out["cpu_baseline"] = cpu_baseline(cells)
     ctx.close()
     print(json.dumps(out))
 

@@ -117,6 +117,8 @@ typedef struct {
     double ms_yield_kernel;  /* device time of the last yield kernel alone */
     int64_t n_fallback;      /* learned proposal: hadrons handed to the plain sampler */
     int64_t n_violations;    /* learned proposal: draws above the envelope (0 when exact) */
+    double ms_setup_kernel;  /* device time of the last per-hadron set-up kernel */
+    double ms_trials_kernel; /* device time of the last trial kernel(s), fallback included */
 } iss_stats;
 
 /* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */
@@ -198,7 +200,7 @@ int iss_mlp_train(iss_ctx* ctx, const iss_train_opts* opts, iss_train_report* re
 int iss_mlp_eval(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, float* loss,
                  float* dz, float* inputs, float* grad);
 /* Envelope tables for explicit (cell, species, rho[8]): 47 floats per
- * sample (ok, log M, c[9], s[9], kappa[9], beta[9], Q[8] / 2^32, last band)
+ * sample (ok, log M, c[9], s[9], kappa[9], beta[9], Q[8] / 2^16, last band)
  * and the 24 hadron parameters they were built from. */
 int iss_band_diag(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, const float* rho,
                   float* tables, float* params);

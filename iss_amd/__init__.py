@@ -74,7 +74,8 @@ class Stats(C.Structure):
                 ("n_failed", C.c_int64), ("n_range_errors", C.c_int64),
                 ("ms_yield", C.c_double), ("ms_sample", C.c_double),
                 ("ms_sample_kernel", C.c_double), ("ms_yield_kernel", C.c_double),
-                ("n_fallback", C.c_int64), ("n_violations", C.c_int64)]
+                ("n_fallback", C.c_int64), ("n_violations", C.c_int64),
+                ("ms_setup_kernel", C.c_double), ("ms_trials_kernel", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -76,7 +76,7 @@ struct DBuf {
 struct iss_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[7] = {};
     std::string err;
     int num_cu = 256;
     // species
@@ -764,10 +764,12 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         per_cu_trials < 1)
         per_cu_trials = 1;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
     if (o->proposal == 0) {
         HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
         if (K > 0) {
             k_setup<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
+            HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
             const int64_t want = (K + 255) / 256;
             const int nblk = static_cast<int>(
                 std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu_trials)));
@@ -792,6 +794,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             const int64_t nwave = (K + 63) / 64;
             k_learn_setup<<<(nwave + kSetupThreads / 64 - 1) / (kSetupThreads / 64), kSetupThreads, lds, ctx->stream>>>(
                 L, ctx->d_lrec.p);
+            HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
             int per_cu = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_learn_trials, 256, 0) != hipSuccess ||
                 per_cu < 1)
@@ -815,6 +818,11 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     float ms_all = 0.f, ms_k = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
     HIPCHK(hipEventElapsedTime(&ms_k, ctx->ev[4], ctx->ev[5]));
+    float ms_su = 0.f, ms_tr = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms_su, ctx->ev[4], ctx->ev[6]));
+    HIPCHK(hipEventElapsedTime(&ms_tr, ctx->ev[6], ctx->ev[5]));
+    ctx->stats.ms_setup_kernel = ms_su;
+    ctx->stats.ms_trials_kernel = ms_tr;
     ctx->K = K;
     ctx->E = E;
     ctx->stats.n_hadrons = K;

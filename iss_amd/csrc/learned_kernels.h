@@ -126,10 +126,10 @@ struct LParams {
 };
 
 __device__ __forceinline__ float ps_of(const LParams& P, float E) {
-    return P.smode == 2 ? P.ps0 / E : (P.smode == 1 ? P.ps0 : 0.f);
+    return P.smode == 2 ? P.ps0 * __builtin_amdgcn_rcpf(E) : (P.smode == 1 ? P.ps0 : 0.f);
 }
 __device__ __forceinline__ float qb_of(const LParams& P, float E) {
-    return P.qi / E + P.q0 + E * (P.q1 + P.q2 * E);
+    return P.qi * __builtin_amdgcn_rcpf(E) + P.q0 + E * (P.q1 + P.q2 * E);
 }
 
 // The delta-f forms of the plain sampler's trial loop, rewritten as
@@ -274,15 +274,26 @@ __device__ __forceinline__ f32x16 tile_ll(const uint16_t* A, int lda, int m0, in
     return acc;
 }
 
-// tanh-form GELU and its derivative
-__device__ __forceinline__ float gelu_t(float x) {
-    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    return 1.f - 2.f / (expf(2.f * u) + 1.f);
+// Hardware reciprocal, exp2 and log2 (1-2 ulp).  Every bound they enter
+// carries a relative margin of 1e-4, far above their error.
+// (fexp, the exp2-based exponential, comes with the yield kernels)
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+// cube root with sign, through exp2(log2(|x|) / 3)
+__device__ __forceinline__ float fcbrt(float x) {
+    const float r = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fabsf(x)) * (1.f / 3.f));
+    return x < 0.f ? -r : r;
 }
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + gelu_t(x)); }
+
+// tanh-form GELU, 0.5 x (1 + tanh(u)) = x sigmoid(2u), and its derivative
+__device__ __forceinline__ float gelu_sig(float x) {
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-2.8853900817779268f * u));
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
 __device__ __forceinline__ float gelu_d(float x) {
-    const float th = gelu_t(x);
-    return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * x * x);
+    const float sg = gelu_sig(x);
+    return sg + 2.f * x * sg * (1.f - sg) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * x * x);
 }
 
 // parameters, FP32 master layout (row-major [out][in])
@@ -406,7 +417,7 @@ __device__ __forceinline__ float band_edge(int k) {
 // kappa (fp16) and beta (u8, beta = b / 255), both quantised before the
 // bounds are computed.
 struct BandTable {
-    uint32_t Q[8];        // band k is chosen when Q[k-1] <= word < Q[k] (Q[-1] = 0)
+    uint16_t Q[8];        // band k is chosen when Q[k-1] <= word16 < Q[k] (Q[-1] = 0, 16-bit words)
     uint2 band[kBands];   // {c | s << 16, kappa | beta << 16}
     int klast;            // last band of positive mass (chosen for word >= Q[klast - 1])
 };
@@ -433,129 +444,139 @@ __device__ __forceinline__ void band_unpack(uint2 v, float& c, float& s, float& 
 
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.f), 1.f); }
 
+// w2 / (1 + kappa W) <= S2 over W in [l3, l1]: the numerator's bound is
+// linear on W <= 0 (slope cmin) and W >= 0 (slope cmax), then clamped, so
+// the supremum is at an end, at 0 or where the clamp starts.
+__device__ __forceinline__ void s2_try(float W, float Amax, float cmin, float cmax, float kap, float& S2,
+                                       float& Wst, float& Nst) {
+    const float num = clamp01(0.5f * (Amax + (W >= 0.f ? cmax * W : cmin * W)));
+    const float v = fdiv(num, 1.f + kap * W);
+    if (v > S2) { S2 = v; Wst = W; Nst = num; }
+}
+
 // Envelope of one hadron.  rho[k] in (0, 1) scales the band-midpoint
 // optimum kappa0_k.  Returns false when no envelope can be formed.  With
-// dz != nullptr also returns d log(M) / d z_k for rho = sigmoid(z)
-// (M = sum of the band masses, the expected number of proposals per
-// accepted hadron times the target mass).
+// GRAD also returns d log(M) / d z_k for rho = sigmoid(z) (M = sum of the
+// band masses: the expected proposals per accepted hadron times the
+// target's mass).
+template <bool GRAD>
 __device__ bool band_setup(const LParams& P, const float rho[kOut], BandTable& Bt, float* logM, float* dz) {
     const float T = P.T, mu = P.mu, m = P.m, sg = P.sgn, x = P.x;
     const float tau_max = P.tmax - x;
-    double mk[kBands];
+    const float rD = P.D > 0.f ? __builtin_amdgcn_rcpf(P.D) : 0.f;
+    const float rd0 = P.d0 > 0.f ? __builtin_amdgcn_rcpf(P.d0) : 0.f;
+    const bool shear = P.smode != 0 && P.l1 > 0.f;
+    const float klo = shear ? fdiv(-0.98f, P.l1) : 0.f, khi = (shear && P.l3 < 0.f) ? fdiv(0.98f, -P.l3) : 0.f;
+    // quantities at the band ends: edge j < 9 is band_edge(j), edge 9 is the table top
+    float te[kBands + 1], Ee[kBands + 1], pe[kBands + 1], ve[kBands + 1], ee[kBands + 1], ome[kBands + 1];
+    float qe[kBands + 1], pse[kBands + 1];
+#pragma unroll
+    for (int j = 0; j <= kBands; j++) {
+        const float tau = (j < kBands) ? fminf(band_edge(j), tau_max) : tau_max;
+        te[j] = tau;
+        const float E = T * (x + tau) + mu;
+        const float rE = __builtin_amdgcn_rcpf(E);
+        Ee[j] = E;
+        pe[j] = __builtin_amdgcn_sqrtf(fmaxf(E * E - m * m, 0.f));
+        ve[j] = E > 0.f ? pe[j] * rE : 0.f;
+        ee[j] = fexp(-(x + tau));
+        ome[j] = __builtin_amdgcn_rcpf(1.f + sg * ee[j]);   // 1 - sign f0 = e^t f0
+        qe[j] = P.qi * rE + P.q0 + E * (P.q1 + P.q2 * E);
+        pse[j] = P.smode == 2 ? P.ps0 * rE : (P.smode == 1 ? P.ps0 : 0.f);
+    }
+    float mk[kBands];
     float dS2[kOut], Mt_S1[kOut];
     bool ok = tau_max > 0.f;
+#pragma unroll
     for (int k = 0; k < kBands; k++) {
-        mk[k] = 0.;
+        mk[k] = 0.f;
         Bt.band[k] = make_uint2(0u, 0u);
         if (k < kOut) { dS2[k] = 0.f; Mt_S1[k] = 0.f; }
-        const float a = fminf(band_edge(k), tau_max);
-        const float b = (k < kBands - 1) ? fminf(band_edge(k + 1), tau_max) : tau_max;
+        const int jb = k + 1;
+        const float a = te[k], b = te[jb];
         const float w = b - a;
-        if (!(w > 0.f)) continue;
-        const float ta = x + a, tb = x + b;
-        const float Elo = T * ta + mu, Ehi = T * tb + mu;
-        const float plo = sqrtf(fmaxf(Elo * Elo - m * m, 0.f)), phi = sqrtf(fmaxf(Ehi * Ehi - m * m, 0.f));
-        if (!(phi > 0.f) || !(Ehi > 0.f)) continue;
-        const float vlo = Elo > 0.f ? plo / Elo : 0.f, vhi = phi / Ehi;
+        const float Elo = Ee[k], Ehi = Ee[jb], plo = pe[k], phi = pe[jb], vlo = ve[k], vhi = ve[jb];
+        if (!(w > 0.f) || !(phi > 0.f) || !(Ehi > 0.f)) continue;
         // w1 / (1 + beta y) <= S1
         float beta = 0.f;
-        if (P.dn > 0.f) beta = P.d0 > 0.f ? fminf(0.98f, vlo * P.dn / P.d0) : 0.98f;
-        const uint32_t beta_b = static_cast<uint32_t>(floorf(beta * 255.f));
+        if (P.dn > 0.f) beta = P.d0 > 0.f ? fminf(0.98f, vlo * P.dn * rd0) : 0.98f;
+        const uint32_t beta_b = static_cast<uint32_t>(beta * 255.f);
         beta = static_cast<float>(beta_b) * (1.f / 255.f);
-        float S1 = 0.f;
-        if (P.D > 0.f)
-            S1 = fmaxf(clamp01((P.d0 + vhi * P.dn) / P.D) / (1.f + beta),
-                       clamp01((P.d0 - vlo * P.dn) / P.D) / (1.f - beta));
-        // 1 - sign f0 = e^t f0 = 1 / (1 + sign e^-t): monotone in t
-        const float ea = expf(-ta), eb = expf(-tb);
-        const float oma = 1.f / (1.f + sg * ea), omb = 1.f / (1.f + sg * eb);
-        const float om_lo = fminf(oma, omb), om_hi = fmaxf(oma, omb);
+        const float S1 = fmaxf(clamp01((P.d0 + vhi * P.dn) * rD) * __builtin_amdgcn_rcpf(1.f + beta),
+                               clamp01((P.d0 - vlo * P.dn) * rD) * __builtin_amdgcn_rcpf(1.f - beta));
+        const float om_lo = fminf(ome[k], ome[jb]), om_hi = fmaxf(ome[k], ome[jb]);
         // bulk: sup of omf Q(E) over the band
-        float Qmax = fmaxf(qb_of(P, Elo), qb_of(P, Ehi));
+        float Qmax = fmaxf(qe[k], qe[jb]);
         if (P.qi == 0.f && P.q2 != 0.f) {
-            const float Es = -P.q1 / (2.f * P.q2);
+            const float Es = -P.q1 * __builtin_amdgcn_rcpf(2.f * P.q2);
             if (Es > Elo && Es < Ehi) Qmax = fmaxf(Qmax, qb_of(P, Es));
-        } else if (P.qi != 0.f && P.q2 == 0.f && P.q1 != 0.f && P.qi / P.q1 > 0.f) {
-            const float Es = sqrtf(P.qi / P.q1);
+        } else if (P.qi != 0.f && P.q2 == 0.f && P.q1 != 0.f && P.qi * P.q1 > 0.f) {
+            const float Es = __builtin_amdgcn_sqrtf(P.qi * __builtin_amdgcn_rcpf(P.q1));
             if (Es > Elo && Es < Ehi) Qmax = fmaxf(Qmax, qb_of(P, Es));
         }
         const float Amax = 1.f + (Qmax >= 0.f ? om_hi * Qmax : om_lo * Qmax);
         // shear: c(E) = omf p^2 Ps(E) in [cmin, cmax]
         float cmin = 0.f, cmax = 0.f;
-        const bool shear = P.smode != 0 && P.l1 > 0.f;
         if (shear) {
-            const float pa = ps_of(P, fmaxf(Elo, 1e-30f)), pb = ps_of(P, Ehi);
-            const float psmin = fminf(pa, pb), psmax = fmaxf(pa, pb);
+            const float psmin = fminf(pse[k], pse[jb]), psmax = fmaxf(pse[k], pse[jb]);
             if (psmin >= 0.f) { cmin = om_lo * plo * plo * psmin; cmax = om_hi * phi * phi * psmax; }
             else if (psmax <= 0.f) { cmin = om_hi * phi * phi * psmin; cmax = om_lo * plo * plo * psmax; }
             else { cmin = om_hi * phi * phi * psmin; cmax = om_hi * phi * phi * psmax; }
         }
-        // kappa: the network's share of the band-midpoint optimum
-        float kap = 0.f;
-        bool clipped = true;
-        float kap0 = 0.f;
-        if (k < kOut && shear && P.l3 < 0.f) {
-            const float tm = x + 0.5f * (a + b);
-            const float Em = T * tm + mu;
-            const float pm2 = fmaxf(Em * Em - m * m, 0.f);
-            const float omm = 1.f / (1.f + sg * expf(-tm));
-            const float cm = omm * pm2 * ps_of(P, Em);
-            const float am = 1.f + omm * qb_of(P, Em);
-            kap0 = am > 0.f ? cm / am : 0.f;
-            const float raw = rho[k] * kap0;
-            const float lo = -0.98f / P.l1, hi = 0.98f / (-P.l3);
-            kap = h2f(f2h(fminf(fmaxf(raw, lo), hi)));
-            if (1.f + kap * P.l1 < 0.01f || 1.f + kap * P.l3 < 0.01f) kap = 0.f;
-            clipped = !(raw > lo && raw < hi);
-        }
-        // w2 / (1 + kappa W) <= S2: W in [l3, l1]; the numerator's bound is
-        // linear on W <= 0 (slope cmin) and W >= 0 (slope cmax), then clamped
-        float S2 = 0.f, Wst = 0.f, Nst = 0.f;
-        {
-            float cand[5];
-            int nc = 0;
-            cand[nc++] = shear ? P.l3 : 0.f;
-            cand[nc++] = 0.f;
-            cand[nc++] = shear ? P.l1 : 0.f;
-            if (shear && cmax != 0.f) {
-                const float wc = (2.f - Amax) / cmax;
-                if (wc > 0.f && wc < P.l1) cand[nc++] = wc;
-            }
-            if (shear && cmin != 0.f) {
-                const float wc = (2.f - Amax) / cmin;
-                if (wc < 0.f && wc > P.l3) cand[nc++] = wc;
-            }
-            for (int i = 0; i < nc; i++) {
-                const float W = cand[i];
-                const float num = clamp01(0.5f * (Amax + (W >= 0.f ? cmax * W : cmin * W)));
-                const float v = num / (1.f + kap * W);
-                if (v > S2) { S2 = v; Wst = W; Nst = num; }
-            }
-        }
-        // |p| envelope exp(c + s tau)
-        float cc, ss;
-        const float U = -T * T * (Ehi * Ehi + m * m) / (phi * phi * phi * phi) - T * T / (Ehi * Ehi) +
-                        (sg < 0.f ? ea / ((1.f - ea) * (1.f - ea)) : 0.f);
+        // the band's interior point: midpoint (tail: a + 1) for kappa0 and the tangent
         const float tau_s = (k < kBands - 1) ? 0.5f * (a + b) : fminf(a + 1.f, b);
         const float ts = x + tau_s;
         const float Es = T * ts + mu;
         const float ps2 = Es * Es - m * m;
+        const float es = fexp(-ts);
+        const float oms = __builtin_amdgcn_rcpf(1.f + sg * es);
+        const float rEs = __builtin_amdgcn_rcpf(Es);
+        // kappa: the network's share of the band-midpoint optimum
+        float kap = 0.f, kap0 = 0.f;
+        bool clipped = true;
+        if (k < kOut && shear && P.l3 < 0.f) {
+            const float cm = oms * fmaxf(ps2, 0.f) * (P.smode == 2 ? P.ps0 * rEs : P.ps0);
+            const float am = 1.f + oms * (P.qi * rEs + P.q0 + Es * (P.q1 + P.q2 * Es));
+            kap0 = am > 0.f ? cm * __builtin_amdgcn_rcpf(am) : 0.f;
+            const float raw = rho[k] * kap0;
+            kap = h2f(f2h(fminf(fmaxf(raw, klo), khi)));
+            if (1.f + kap * P.l1 < 0.01f || 1.f + kap * P.l3 < 0.01f) kap = 0.f;
+            clipped = !(raw > klo && raw < khi);
+        }
+        float S2 = 0.f, Wst = 0.f, Nst = 0.f;
+        s2_try(0.f, Amax, cmin, cmax, kap, S2, Wst, Nst);
+        if (shear) {
+            s2_try(P.l3, Amax, cmin, cmax, kap, S2, Wst, Nst);
+            s2_try(P.l1, Amax, cmin, cmax, kap, S2, Wst, Nst);
+            if (cmax != 0.f) {
+                const float wc = (2.f - Amax) * __builtin_amdgcn_rcpf(cmax);
+                if (wc > 0.f && wc < P.l1) s2_try(wc, Amax, cmin, cmax, kap, S2, Wst, Nst);
+            }
+            if (cmin != 0.f) {
+                const float wc = (2.f - Amax) * __builtin_amdgcn_rcpf(cmin);
+                if (wc < 0.f && wc > P.l3) s2_try(wc, Amax, cmin, cmax, kap, S2, Wst, Nst);
+            }
+        }
+        if (!(S1 * S2 > 0.f)) continue;
+        // |p| envelope exp(cc + ss tau): the tangent of G at tau_s when G is
+        // concave on the band (G'' <= U <= 0), else sup(p E) sup(e^t f0) e^-tau
+        const float rphi2 = __builtin_amdgcn_rcpf(phi * phi), rEhi = __builtin_amdgcn_rcpf(Ehi);
+        const float ea = ee[k];
+        const float U = -T * T * (Ehi * Ehi + m * m) * rphi2 * rphi2 - T * T * rEhi * rEhi +
+                        (sg < 0.f ? ea * __builtin_amdgcn_rcpf((1.f - ea) * (1.f - ea)) : 0.f);
+        float cc, ss;
         if (U <= 0.f && ps2 > 0.f) {
-            const float es = expf(-ts);
-            const float Gs = logf(sqrtf(ps2) * Es) - tau_s - log1pf(sg * es);
-            ss = T * Es / ps2 + T / Es - 1.f + sg * es / (1.f + sg * es);
+            const float Gs = flog(__builtin_amdgcn_sqrtf(ps2) * Es * oms) - tau_s;
+            ss = T * Es * __builtin_amdgcn_rcpf(ps2) + T * rEs - 1.f + sg * es * oms;
             cc = Gs - ss * tau_s;
         } else {
             ss = -1.f;
-            cc = logf(phi * Ehi * om_hi);
+            cc = flog(phi * Ehi * om_hi);
         }
-        const float sw = ss * w;
-        const float Mt = expf(cc + ss * a) * (fabsf(sw) > 1e-6f ? expm1f(sw) / ss : w);
-        if (!isfinite(cc) || !isfinite(ss) || !isfinite(Mt) || !(Mt >= 0.f)) { ok = false; continue; }
-        if (!(S1 * S2 > 0.f)) continue;
+        if (!isfinite(cc) || !isfinite(ss)) { ok = false; continue; }
         // stored line: fp16 slope, intercept raised to stay above the exact
         // line at both band ends, then rounded up
-        const float c0 = cc + logf(S1 * S2) + kLogMargin;
+        const float c0 = cc + flog(S1 * S2) + kLogMargin;
         const uint16_t sb = f2h(ss);
         const float s16 = h2f(sb);
         const float c1 = c0 + fmaxf(0.f, fmaxf((ss - s16) * a, (ss - s16) * b));
@@ -563,70 +584,85 @@ __device__ bool band_setup(const LParams& P, const float rho[kOut], BandTable& B
         const uint16_t cb = f2h_up(c1);
         const float c16 = h2f(cb);
         const float sw16 = s16 * w;
-        mk[k] = exp(static_cast<double>(c16) + static_cast<double>(s16) * a) *
-                (fabs(sw16) > 1e-6 ? expm1(static_cast<double>(sw16)) / static_cast<double>(s16) : static_cast<double>(w));
+        // integral of exp(c16 + s16 tau) over [a, b]
+        const float em1 = fabsf(sw16) > 1e-2f ? fexp(sw16) - 1.f : sw16 * (1.f + sw16 * (0.5f + sw16 * (1.f / 6.f)));
+        mk[k] = fexp(c16 + s16 * a) * (fabsf(sw16) > 1e-6f ? em1 * __builtin_amdgcn_rcpf(s16) : w);
         Bt.band[k] = make_uint2(static_cast<uint32_t>(cb) | (static_cast<uint32_t>(sb) << 16),
                                 static_cast<uint32_t>(f2h(kap)) | (beta_b << 16));
-        if (k < kOut) {
-            Mt_S1[k] = Mt * S1 * expf(kLogMargin);
-            dS2[k] = (clipped || !shear) ? 0.f : -Nst * Wst / ((1.f + kap * Wst) * (1.f + kap * Wst)) * kap0;
+        if (GRAD && k < kOut) {
+            const float sw = ss * w;
+            const float e1 = fabsf(sw) > 1e-2f ? fexp(sw) - 1.f : sw * (1.f + sw * (0.5f + sw * (1.f / 6.f)));
+            const float Mt = fexp(cc + ss * a) * (fabsf(sw) > 1e-6f ? e1 * __builtin_amdgcn_rcpf(ss) : w);
+            Mt_S1[k] = Mt * S1 * fexp(kLogMargin);
+            dS2[k] = (clipped || !shear) ? 0.f : -fdiv(Nst * Wst, (1.f + kap * Wst) * (1.f + kap * Wst)) * kap0;
         }
     }
-    double M = 0.;
+    float M = 0.f;
+#pragma unroll
     for (int k = 0; k < kBands; k++) M += mk[k];
-    if (!ok || !(M > 0.) || !isfinite(M)) {
+    if (!ok || !(M > 0.f) || !isfinite(M)) {
         if (logM) *logM = 0.f;
-        if (dz) for (int k = 0; k < kOut; k++) dz[k] = 0.f;
+        if (GRAD)
+            for (int k = 0; k < kOut; k++) dz[k] = 0.f;
         return false;
     }
-    if (logM) *logM = static_cast<float>(log(M));
-    if (dz)
-        for (int k = 0; k < kOut; k++) dz[k] = static_cast<float>(Mt_S1[k] * dS2[k] / M) * rho[k] * (1.f - rho[k]);
-    // selection thresholds; every band of positive mass keeps a non-empty
-    // range and the last one reaches 2^32
-    const double two32 = 4294967296.;
+    if (logM) *logM = flog(M);
+    if (GRAD) {
+#pragma unroll
+        for (int k = 0; k < kOut; k++) dz[k] = Mt_S1[k] * dS2[k] / M * rho[k] * (1.f - rho[k]);
+    }
+    // selection thresholds (16-bit words): every band of positive mass keeps
+    // a non-empty range and the last one reaches 2^16
     int klast = 0;
+#pragma unroll
     for (int k = 0; k < kBands; k++)
-        if (mk[k] > 0.) klast = k;
-    uint64_t Qp = 0, Qk[kBands];
-    double cum = 0.;
+        if (mk[k] > 0.f) klast = k;
+    constexpr uint32_t two16 = 1u << 16;
+    uint32_t Qk[kBands];
+    uint32_t Qp = 0;
+    float cum = 0.f;
+    const float iM = 1.f / M;
+#pragma unroll
     for (int k = 0; k < kBands; k++) {
         cum += mk[k];
-        uint64_t q;
+        uint32_t q;
         if (k >= klast) {
-            q = static_cast<uint64_t>(two32);
+            q = two16;
         } else {
-            q = static_cast<uint64_t>(floor(cum / M * two32));
-            if (mk[k] > 0.) q = q > Qp + 1 ? q : Qp + 1;
+            q = static_cast<uint32_t>(fminf(cum * iM, 1.f) * 65536.f);
+            if (mk[k] > 0.f) q = q > Qp + 1 ? q : Qp + 1;
             else q = Qp;
             int after = 0;
-            for (int j = k + 1; j <= klast; j++) after += (mk[j] > 0.) ? 1 : 0;
-            const uint64_t cap = static_cast<uint64_t>(two32) - static_cast<uint64_t>(after);
+#pragma unroll
+            for (int j = k + 1; j < kBands; j++) after += (j <= klast && mk[j] > 0.f) ? 1 : 0;
+            const uint32_t cap = two16 - static_cast<uint32_t>(after);
             if (q > cap) q = cap;
         }
         Qk[k] = q;
         Qp = q;
     }
     Bt.klast = klast;
-    // the chosen probabilities are (Q_k - Q_{k-1}) / 2^32; rescale each
+    // the chosen probabilities are (Q_k - Q_{k-1}) / 2^16; rescale each
     // band's envelope by g_k = m_k / (c_sel_k R), R = max_j m_j / c_sel_j
-    double R = 0.;
+    float R = 0.f;
     Qp = 0;
+#pragma unroll
     for (int k = 0; k < kBands; k++) {
-        const double cs = static_cast<double>(Qk[k] - Qp) / two32;
-        if (mk[k] > 0. && cs > 0.) R = fmax(R, mk[k] / cs);
+        const float cs = static_cast<float>(Qk[k] - Qp) * (1.f / 65536.f);
+        if (mk[k] > 0.f && cs > 0.f) R = fmaxf(R, fdiv(mk[k], cs));
         Qp = Qk[k];
     }
     Qp = 0;
+#pragma unroll
     for (int k = 0; k < kBands; k++) {
-        const double cs = static_cast<double>(Qk[k] - Qp) / two32;
-        if (mk[k] > 0. && cs > 0.) {
+        const float cs = static_cast<float>(Qk[k] - Qp) * (1.f / 65536.f);
+        if (mk[k] > 0.f && cs > 0.f) {
             const float c16 = h2f(static_cast<uint16_t>(Bt.band[k].x & 0xFFFFu));
-            const float cg = c16 - static_cast<float>(log(mk[k] / (cs * R)));
+            const float cg = c16 - flog(fdiv(mk[k], cs * R));
             Bt.band[k].x = (Bt.band[k].x & 0xFFFF0000u) | f2h_up(fmaxf(cg, c16));
         }
         Qp = Qk[k];
-        if (k < kBands - 1) Bt.Q[k] = static_cast<uint32_t>(Qk[k] < 0xFFFFFFFFull ? Qk[k] : 0xFFFFFFFFull);
+        if (k < kBands - 1) Bt.Q[k] = static_cast<uint16_t>(Qk[k] < 0xFFFFu ? Qk[k] : 0xFFFFu);
     }
     return true;
 }
@@ -641,25 +677,27 @@ struct LearnArgs {
     unsigned long long* stats2;       // [0] fallback hadrons, [1] envelope violations
 };
 
-// Per-hadron record of the learned sampler (128 bytes).
+// Per-hadron record of the learned sampler (96 bytes): the band table with
+// 16-bit selection thresholds, fp16 envelope lines and kappas, u8 betas.
 struct alignas(16) LRec {
     uint32_t cell;
     uint32_t j;
     uint16_t species;
     uint8_t flags;        // 1: hole (table range), 2: plain-sampler fallback
     uint8_t klast;
-    uint32_t pad;
-    uint64_t out;
-    uint32_t Q[8];
-    uint2 band[kBands];
+    uint16_t Q[8];
+    uint32_t cs[kBands];  // c | s << 16 (fp16)
+    uint16_t kap[8];      // fp16 (the far tail has kappa = 0)
+    uint8_t beta[kBands];
+    uint8_t pad[7];
 };
-static_assert(sizeof(LRec) == 128, "LRec layout");
+static_assert(sizeof(LRec) == 96, "LRec layout");
 
 constexpr int kSetupThreads = 256;
 constexpr int kMlpBytes = (64 * kXld + 64 * kHld) * 2;
 
 // Set-up of the learned sampler, one wave per 64 consecutive slots: cell,
-// species and output slot exactly as k_setup, then the network inputs, the
+// species exactly as k_setup, then the network inputs, the
 // network on MFMA and the band envelope.
 __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec* __restrict__ rec) {
     extern __shared__ __align__(16) char lds[];
@@ -676,7 +714,6 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
     int64_t c = 0;
     int s = 0;
     uint32_t jj = 0;
-    uint64_t outp = 0;
     LParams P{};
     float f[kIn];
     for (int k = 0; k < kIn; k++) f[k] = 0.f;
@@ -687,7 +724,6 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
         const double* row = a.cdf + c * a.Ns;
         s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
         if (s >= a.Ns) s = a.Ns - 1;
-        outp = a.perm(static_cast<uint64_t>(i));
         lparams_of(a, L.aux, c, s, P);
         features_of(a, c, s, P, f);
     }
@@ -700,23 +736,26 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
     for (int k = 0; k < kOut; k++) rho[k] = 1.f / (1.f + expf(-Os[lane * kOut + k]));
     BandTable B;
     int flags = P.flags & 3;
-    if (!flags && !band_setup(P, rho, B, nullptr, nullptr)) flags = 2;
+    if (!flags && !band_setup<false>(P, rho, B, nullptr, nullptr)) flags = 2;
     LRec r;
     r.cell = static_cast<uint32_t>(c);
     r.j = jj;
     r.species = static_cast<uint16_t>(s);
     r.flags = static_cast<uint8_t>(flags);
     r.klast = static_cast<uint8_t>(flags ? 0 : B.klast);
-    r.pad = 0u;
-    r.out = outp;
-    for (int k = 0; k < 8; k++) r.Q[k] = flags ? 0u : B.Q[k];
-    for (int k = 0; k < kBands; k++) r.band[k] = flags ? make_uint2(0u, 0u) : B.band[k];
+    for (int k = 0; k < 8; k++) r.Q[k] = flags ? 0 : B.Q[k];
+    for (int k = 0; k < kBands; k++) {
+        r.cs[k] = flags ? 0u : B.band[k].x;
+        r.beta[k] = flags ? 0 : static_cast<uint8_t>((B.band[k].y >> 16) & 0xFFu);
+        if (k < 8) r.kap[k] = flags ? 0 : static_cast<uint16_t>(B.band[k].y & 0xFFFFu);
+    }
+    for (int k = 0; k < 7; k++) r.pad[k] = 0;
     rec[i] = r;
 }
 
 // Persistent trial kernel of the learned sampler with per-lane refill (as
 // k_trials).  A lane keeps the hadron's parameters and band thresholds in
-// registers and reads the chosen band's 8 bytes per proposal.  Proposal
+// registers (thresholds, lines, kappas, betas: 20 words).  Proposal
 // `it` of hadron (c, j) uses Philox blocks (c, j, TAG_LEARN, it) and
 // (c, j, TAG_LEARN2, it).
 __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* __restrict__ rec) {
@@ -744,8 +783,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
     uint32_t jj = 0;
     int s = 0, it = 0, klast = 0;
     uint64_t outp = 0;
-    uint32_t Q[8];
-    for (int k = 0; k < 8; k++) Q[k] = 0u;
+    // band table in registers: thresholds (2 x u16 per word), lines, kappas (2 x fp16), betas (4 x u8)
+    uint32_t Qw[4], CS[kBands], KW[4], BW[3];
+    for (int k = 0; k < 4; k++) { Qw[k] = 0u; KW[k] = 0u; }
+    for (int k = 0; k < kBands; k++) CS[k] = 0u;
+    for (int k = 0; k < 3; k++) BW[k] = 0u;
     LParams P{};
 
     while (true) {
@@ -755,15 +797,14 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const int rank = __popcll(need & ((1ull << lane) - 1));
             if (!active && rank < navail) {
                 cur = next + rank;
-                const LRec* r = rec + cur;
-                const uint4 h0 = *reinterpret_cast<const uint4*>(r);
+                const uint4* r = reinterpret_cast<const uint4*>(rec + cur);
+                const uint4 h0 = r[0], h1 = r[1], h2 = r[2], h3 = r[3], h4 = r[4], h5 = r[5];
                 c = h0.x;
                 jj = h0.y;
                 s = static_cast<int>(h0.z & 0xFFFFu);
                 const int flags = static_cast<int>((h0.z >> 16) & 0xFFu);
                 klast = static_cast<int>(h0.z >> 24);
-                const uint4 h1 = *(reinterpret_cast<const uint4*>(r) + 1);
-                outp = static_cast<uint64_t>(h1.x) | (static_cast<uint64_t>(h1.y) << 32);
+                outp = a.perm(static_cast<uint64_t>(cur));
                 if (flags & 1) {
                     // m/T - mu/T outside the table: a hole, as in the plain sampler
                     n_range++;
@@ -774,10 +815,12 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
                     const unsigned long long pos = atomicAdd(L.fb_count, 1ull);
                     L.fb_list[pos] = cur;
                 } else {
-                    Q[0] = h1.z; Q[1] = h1.w;
-                    const uint4 h2 = *(reinterpret_cast<const uint4*>(r) + 2);
-                    const uint2 h3 = *(reinterpret_cast<const uint2*>(r) + 6);
-                    Q[2] = h2.x; Q[3] = h2.y; Q[4] = h2.z; Q[5] = h2.w; Q[6] = h3.x; Q[7] = h3.y;
+                    // bytes 12..27 Q, 28..63 cs, 64..79 kappa, 80..88 beta
+                    Qw[0] = h0.w; Qw[1] = h1.x; Qw[2] = h1.y; Qw[3] = h1.z;
+                    CS[0] = h1.w; CS[1] = h2.x; CS[2] = h2.y; CS[3] = h2.z; CS[4] = h2.w;
+                    CS[5] = h3.x; CS[6] = h3.y; CS[7] = h3.z; CS[8] = h3.w;
+                    KW[0] = h4.x; KW[1] = h4.y; KW[2] = h4.z; KW[3] = h4.w;
+                    BW[0] = h5.x; BW[1] = h5.y; BW[2] = h5.z;
                     lparams_of(a, L.aux, c, s, P);
                     it = 0;
                     active = true;
@@ -795,12 +838,22 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         const u32x4 w2r = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw2, static_cast<uint32_t>(it)}, key0, key1);
         it++;
         n_it++;
+        const uint32_t w16 = w1r.x >> 16;
         int k = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) k += (w1r.x >= Q[j]) ? 1 : 0;
+        for (int j = 0; j < 4; j++) {
+            k += (w16 >= (Qw[j] & 0xFFFFu)) ? 1 : 0;
+            k += (w16 >= (Qw[j] >> 16)) ? 1 : 0;
+        }
         k = min(k, klast);
-        float ck, sk, kk, bk;
-        band_unpack(rec[cur].band[k], ck, sk, kk, bk);
+        uint32_t csk = CS[0], kwk = KW[0], bwk = BW[0];
+#pragma unroll
+        for (int j = 1; j < kBands; j++)
+            if (k == j) { csk = CS[j]; kwk = KW[min(j >> 1, 3)]; bwk = BW[j >> 2]; }
+        const float ck = h2f(static_cast<uint16_t>(csk & 0xFFFFu));
+        const float sk = h2f(static_cast<uint16_t>(csk >> 16));
+        const float kk = (k < kBands - 1) ? h2f(static_cast<uint16_t>((kwk >> ((k & 1) * 16)) & 0xFFFFu)) : 0.f;
+        const float bk = static_cast<float>((bwk >> ((k & 3) * 8)) & 0xFFu) * (1.f / 255.f);
         const float tau_max = P.tmax - P.x;
         const float ba = fminf(band_edge(k), tau_max);
         const float bb = (k < kBands - 1) ? fminf(band_edge(k + 1), tau_max) : tau_max;
@@ -808,25 +861,25 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         // tau from exp(s tau) on [ba, bb]
         const float ut = u24(w1r.y);
         const float sw = sk * bw;
-        float tau = fabsf(sw) > 1e-5f ? ba + log1pf(ut * expm1f(sw)) / sk : ba + ut * bw;
+        float tau = fabsf(sw) > 1e-5f ? ba + fdiv(log1pf(ut * expm1f(sw)), sk) : ba + ut * bw;
         tau = fminf(fmaxf(tau, ba), bb);
         const float t = P.x + tau;
         const float E = P.T * t + P.mu;
         const float p2 = fmaxf(E * E - P.m * P.m, 0.f);
         const float p = sqrtf(p2);
-        const float et = expf(-t);
-        const float fe = 1.f / (1.f + P.sgn * et);     // e^t f0 = 1 - sign f0
-        const float G = logf(p * E) - tau + logf(fe);
+        const float et = fexp(-t);
+        const float fe = fdiv(1.f, 1.f + P.sgn * et);     // e^t f0 = 1 - sign f0
+        const float G = flog(p * E * fe) - tau;
         // direction in the shear eigenframe
         const float P0 = (1.f + kk * P.l1) * (1.f / 3.f), P1 = P0 + (1.f + kk * P.l2) * (1.f / 3.f);
         const float uy = u24(w2r.x);
         int ax;
         float rr;
-        if (uy < P0) { ax = 0; rr = uy / P0; }
-        else if (uy < P1) { ax = 1; rr = (uy - P0) / (P1 - P0); }
-        else { ax = 2; rr = (uy - P1) / (1.f - P1); }
+        if (uy < P0) { ax = 0; rr = fdiv(uy, P0); }
+        else if (uy < P1) { ax = 1; rr = fdiv(uy - P0, P1 - P0); }
+        else { ax = 2; rr = fdiv(uy - P1, 1.f - P1); }
         rr = fminf(fmaxf(rr, 0.f), 1.f);
-        const float cz = cbrtf(2.f * rr - 1.f);
+        const float cz = fcbrt(2.f * rr - 1.f);
         const float sz = sqrtf(fmaxf(1.f - cz * cz, 0.f));
         float sph, cph;
         __sincosf(2.f * static_cast<float>(kPi) * u24(w2r.y), &sph, &cph);
@@ -838,11 +891,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         if (u24(w1r.w) >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }
         const float W = P.l1 * n0 * n0 + P.l2 * n1 * n1 + P.l3 * n2 * n2;
         // the exact Cooper-Frye weights of the plain sampler
-        const float w1 = (P.D > 0.f) ? clamp01((E * P.d0 + p * P.dn * y) / (E * P.D)) : 0.f;
+        const float w1 = (P.D > 0.f) ? clamp01(fdiv(E * P.d0 + p * P.dn * y, E * P.D)) : 0.f;
         const float dfs = fe * p2 * W * ps_of(P, E);
         const float dfb = fe * qb_of(P, E);
         const float w2 = clamp01(0.5f * (1.f + dfs + dfb));
-        const float racc = expf(G - (ck + sk * tau)) * w1 * w2 / ((1.f + kk * W) * (1.f + bk * y));
+        const float racc = fdiv(fexp(G - (ck + sk * tau)) * w1 * w2, (1.f + kk * W) * (1.f + bk * y));
         if (racc > 1.f + 1e-5f) n_viol++;
         bool done = false;
         if (u24(w1r.z) < racc) {
@@ -1006,7 +1059,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs t) {
         float rho[kOut];
         for (int k = 0; k < kOut; k++) rho[k] = 1.f / (1.f + expf(-Os[l * kOut + k]));
         BandTable B;
-        if (!band_setup(P, rho, B, &lm, dz)) { valid = false; lm = 0.f; }
+        if (!band_setup<true>(P, rho, B, &lm, dz)) { valid = false; lm = 0.f; }
     }
     if (t.out_loss && live) t.out_loss[si] = valid ? lm : 0.f;
     if (t.out_dz && live)
@@ -1146,7 +1199,7 @@ __global__ void k_collect_write(const HSetup* __restrict__ rec, int64_t K, const
 
 // Band tables for given (cell, species, rho): diagnostics and tests.
 // out[i]: [0] ok, [1] log M, [2..10] c, [11..19] s, [20..28] kappa,
-// [29..37] beta, [38..45] Q (as float of Q / 2^32), [46] last band of positive mass
+// [29..37] beta, [38..45] Q / 2^16, [46] last band of positive mass
 constexpr int kBandDiag = 47;
 __global__ void k_band_diag(SampleArgs a, const float* __restrict__ aux, const int32_t* __restrict__ cell,
                             const int32_t* __restrict__ spec, const float* __restrict__ rho, int64_t n,
@@ -1159,7 +1212,7 @@ __global__ void k_band_diag(SampleArgs a, const float* __restrict__ aux, const i
     for (int k = 0; k < kOut; k++) rr[k] = rho[i * kOut + k];
     BandTable B{};
     float lm = 0.f;
-    const bool ok = !(P.flags & 3) && band_setup(P, rr, B, &lm, nullptr);
+    const bool ok = !(P.flags & 3) && band_setup<false>(P, rr, B, &lm, nullptr);
     float* o = out + i * kBandDiag;
     o[0] = ok ? 1.f : 0.f;
     o[1] = lm;
@@ -1168,7 +1221,7 @@ __global__ void k_band_diag(SampleArgs a, const float* __restrict__ aux, const i
         band_unpack(ok ? B.band[k] : make_uint2(0u, 0u), cc, ss, kk, bb);
         o[2 + k] = cc; o[11 + k] = ss; o[20 + k] = kk; o[29 + k] = bb;
     }
-    for (int k = 0; k < kBands - 1; k++) o[38 + k] = ok ? static_cast<float>(B.Q[k] * (1.0 / 4294967296.0)) : 0.f;
+    for (int k = 0; k < kBands - 1; k++) o[38 + k] = ok ? static_cast<float>(B.Q[k]) * (1.f / 65536.f) : 0.f;
     o[46] = ok ? static_cast<float>(B.klast) : 0.f;
     // the hadron parameters the tables were built from
     float* q = lp + i * 24;

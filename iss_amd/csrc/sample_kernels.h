@@ -236,8 +236,8 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 // (offsets), the species (the cell's FP64 species CDF), the |p| table and its
 // normalisation at E~ = m~ - mu~, and the output slot.
 // With a slot list, record r describes slot list[r] for r < *nlist (the
-// learned sampler's fallback hadrons, grid-stride); without one, record i
-// describes slot i.
+// learned sampler's fallback hadrons); without one, record i describes slot
+// i.  Grid-stride, one thread per record.
 __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict__ rec,
                                                const int64_t* __restrict__ list,
                                                const unsigned long long* __restrict__ nlist) {
@@ -245,13 +245,15 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     for (int64_t ri = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ri < n;
          ri += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t i = list ? list[ri] : ri;
-    const int64_t Nc = a.Nc;
-    const int64_t c = upper_bound(a.off + 1, int64_t(0), Nc, i);
+    const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
-    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
-    const double* row = a.cdf + c * a.Ns;
-    int s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
-    if (s >= a.Ns) s = a.Ns - 1;
+    int s;
+    {
+        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+        const double* row = a.cdf + c * a.Ns;
+        s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
+        if (s >= a.Ns) s = a.Ns - 1;
+    }
     PState q;
     pstate_of(a, c, s, q);
     const int sign = a.sp.sign[s];

@@ -235,7 +235,7 @@ def _clamp01(v):
 
 def band_setup(P, rho, quantise=True):
     """Envelope of one hadron.  Returns (ok, log M, dz[8], table) where
-    table = dict(c, s, kap, beta (per band), Q (8 thresholds / 2^32), klast,
+    table = dict(c, s, kap, beta (per band), Q (8 thresholds / 2^16), klast,
     a, b (band ends in t - x), mass)."""
     T, mu, m, sg, x = P["T"], P["mu"], P["m"], P["sgn"], P["x"]
     tau_max = P["tmax"] - x
@@ -364,8 +364,8 @@ def band_setup(P, rho, quantise=True):
     logM = math.log(M)
     rho = np.asarray(rho, dtype=np.float64)
     dz = MtS1 * dS2 / M * rho * (1 - rho)
-    # selection thresholds
-    two32 = 4294967296
+    # selection thresholds, in units of 2^-16
+    two32 = 65536
     klast = int(np.nonzero(mk > 0)[0].max())
     Qk = np.zeros(N_BANDS, dtype=np.int64)
     Qp, cum = 0, 0.0

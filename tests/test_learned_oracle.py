@@ -6,7 +6,7 @@
 * the envelope against the exact rest-frame target on random momenta and
   directions in every band (no draw may exceed it);
 * the selection thresholds: every band of positive mass has a non-empty
-  range and the last one ends at 2^32.
+  range and the last one ends at 2^16.
 """
 import ctypes
 import math
@@ -152,4 +152,4 @@ def test_thresholds_cover_every_band():
             width = Qk[k] - prev
             assert (width > 0) == (tab["mass"][k] > 0 or False) or k > tab["klast"]
             prev = Qk[k]
-        assert Qk[tab["klast"]] == 2 ** 32
+        assert Qk[tab["klast"]] == 2 ** 16
