@@ -100,6 +100,20 @@ def traffic_from_profiles(kernel: str):
         return None
 
 
+def calibration_copy(nbytes):
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def run_steps(ctx, args, proposal, first_event):
     def step(i):
         ctx.compute_yields(precision=args.precision)
@@ -145,6 +159,9 @@ def main():
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a known-size kernel for "
+                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the sampler has no multi-GPU path); "
@@ -154,6 +171,8 @@ def main():
     import iss_amd
     from iss_amd import synth, tables
 
+    if args.calibration_copy_mib > 0:
+        calibration_copy(args.calibration_copy_mib << 20)
     sp = tables.species_arrays(1)
     cells, _ = synth.viscous_lrf(args.cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4)
     ctx = iss_amd.Context(0)

@@ -298,38 +298,75 @@ int iSS::generate_samples() {
     so.seed = static_cast<uint64_t>(randomSeed_);
     so.event0 = 0;
     so.n_events = nev;
-    so.proposal = static_cast<int>(paraRdr_ptr->getVal("sampling_proposal", 0));
+    so.proposal = 0;
     so.sampling_model = static_cast<int>(paraRdr_ptr->getVal("dN_dy_sampling_model"));
     so.hydro_mode = hydro_mode;
     so.y_LB = yLB;
     so.y_RB = yRB;
     so.max_trials = 5000;
     so.precision = precision;
-    int64_t nh = 0;
-    rc = iss_sample(ctx_, &so, &nh);
-    if (rc != ISS_OK) {
-        error(std::string("iss_sample: ") + iss_last_error(ctx_));
-        return rc;
-    }
-    std::vector<int64_t> counts(nev);
-    std::vector<iss_hadron> all(static_cast<size_t>(nh));
-    if (iss_get_event_counts(ctx_, counts.data()) != ISS_OK ||
-        iss_get_hadrons(ctx_, all.data(), nh) != ISS_OK) {
-        error(std::string("hadron read-back: ") + iss_last_error(ctx_));
-        return -1;
-    }
     release_hadrons();
-    size_t k = 0;
-    for (int e = 0; e < nev; e++) {
-        auto* v = new std::vector<iSS_Hadron>();
-        v->reserve(counts[e]);
-        for (int64_t i = 0; i < counts[e]; i++, k++) {
-            const iss_hadron& h = all[k];
-            if (h.pid == 0) continue;  // slot that ran out of trials
-            v->push_back(iSS_Hadron{h.pid, h.mass, h.E, h.px, h.py, h.pz, h.t, h.x, h.y, h.z});
+    // one sampling pass over events [event0, event0 + n): appended to the lists
+    auto run_pass = [&](int event0, int n, int proposal) -> int {
+        so.event0 = event0;
+        so.n_events = n;
+        so.proposal = proposal;
+        int64_t nh = 0;
+        int r = iss_sample(ctx_, &so, &nh);
+        if (r != ISS_OK) {
+            error(std::string("iss_sample: ") + iss_last_error(ctx_));
+            return r;
         }
-        hadron_list_.push_back(v);
+        std::vector<int64_t> counts(n);
+        std::vector<iss_hadron> all(static_cast<size_t>(nh));
+        if (iss_get_event_counts(ctx_, counts.data()) != ISS_OK || iss_get_hadrons(ctx_, all.data(), nh) != ISS_OK) {
+            error(std::string("hadron read-back: ") + iss_last_error(ctx_));
+            return -1;
+        }
+        size_t k = 0;
+        for (int e = 0; e < n; e++) {
+            auto* v = new std::vector<iSS_Hadron>();
+            v->reserve(counts[e]);
+            for (int64_t i = 0; i < counts[e]; i++, k++) {
+                const iss_hadron& h = all[k];
+                if (h.pid == 0) continue;  // slot that ran out of trials
+                v->push_back(iSS_Hadron{h.pid, h.mass, h.E, h.px, h.py, h.pz, h.t, h.x, h.y, h.z});
+            }
+            hadron_list_.push_back(v);
+        }
+        return ISS_OK;
+    };
+    const int proposal = static_cast<int>(paraRdr_ptr->getVal("sampling_proposal", 0));
+    if (proposal == 1) {
+        // learned proposal: the first events (about 2e6 hadrons) come from the
+        // plain sampler; the network is trained on their accepted hadrons and
+        // samples the remaining events
+        double per_event = 0.;
+        for (double v : species_dNdy_) per_event += v;
+        if (hydro_mode != 2) per_event *= (yRB - yLB);
+        const int n1 = std::max(1, std::min(nev, static_cast<int>(std::ceil(2.0e6 / std::max(per_event, 1e-9)))));
+        rc = run_pass(0, n1, 0);
+        if (rc != ISS_OK) return rc;
+        if (n1 < nev) {
+            int64_t ntrain = 0;
+            iss_train_opts to{8192, 300, 3e-3f, static_cast<uint64_t>(randomSeed_) + 17u};
+            iss_train_report tr{};
+            if (iss_mlp_collect(ctx_, 10000000, &ntrain) != ISS_OK || iss_mlp_init(ctx_, 1) != ISS_OK ||
+                iss_mlp_train(ctx_, &to, &tr) != ISS_OK) {
+                error(std::string("learned proposal training: ") + iss_last_error(ctx_));
+                return -1;
+            }
+            info("learned proposal trained on " + std::to_string(ntrain) + " hadrons, log envelope mass " +
+                 std::to_string(tr.loss_first) + " -> " + std::to_string(tr.loss_last));
+            rc = run_pass(n1, nev - n1, 1);
+            if (rc != ISS_OK) return rc;
+        }
+    } else {
+        rc = run_pass(0, nev, 0);
+        if (rc != ISS_OK) return rc;
     }
+    int64_t nh = 0;
+    for (auto* v : hadron_list_) nh += static_cast<int64_t>(v->size());
     iss_stats st{};
     iss_get_stats(ctx_, &st);
     if (st.n_failed > 0)

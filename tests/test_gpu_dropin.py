@@ -99,3 +99,18 @@ def test_dropin_oscar_and_gzip(tmp_path):
         n = int(first[0])
         rec = g.readline().split()
     assert n > 10000 and len(rec) == 10
+
+
+@pytest.mark.parametrize("name", ["Viscous1", "Ideal2"])
+def test_dropin_learned_proposal(tmp_path, name):
+    """sampling_proposal = 1: the first events come from the plain sampler,
+    the network is trained on their hadrons and samples the rest; the
+    reference's T^{mu nu} criterion holds for the whole run (the Fermi-Dirac
+    table quirk off, so every species goes through the envelope)."""
+    r = _run(tmp_path, name, ["sampling_proposal=1", "fermion_cdf2_reference_quirk=0",
+                              "number_of_repeated_sampling=1000"])
+    assert "learned proposal trained" in r.stdout, r.stdout[-2000:]
+    data = np.loadtxt(tmp_path / "checkReconstructedTmunu.dat")
+    res = np.mean(np.abs(data[:, 2]))
+    print(name, "learned mean |diff|", res)
+    assert res < 1e-3
