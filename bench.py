@@ -159,6 +159,7 @@ def main():
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -184,11 +185,17 @@ def main():
     ctx.set_surface_lrf(cells)
     nc, ns = ctx.num_cells(), ctx.n_species
 
+    tune = {}
+    if not args.no_tune:
+        ctx.compute_yields(precision=args.precision)
+        tune["plain"] = ctx.autotune(proposal=0, n_events=args.events // 4, seed=7, repeats=1)
     plain = run_steps(ctx, args, 0, 0)
     # training set: the hadrons accepted by the last plain pass
     n_train = ctx.mlp_collect(10_000_000)
     ctx.mlp_init(1)
     rep = ctx.mlp_train(batch=8192, steps=args.train_steps, lr=3e-3, seed=5)
+    if not args.no_tune:
+        tune["learned"] = ctx.autotune(proposal=1, n_events=args.events // 4, seed=7, repeats=1)
     learned = run_steps(ctx, args, 1, (args.warmup + args.steps) * args.events)
 
     v_learn = learned["hadrons"] / learned["elapsed"]
@@ -235,6 +242,7 @@ def main():
         "envelope_violations": learned["violations"],
         "fallback_hadrons": learned["fallback"],
         "training": rep,
+        "autotune": tune,
         "kernels_ms": {"k_yield": learned["ms_yield"], "k_learn_setup": learned["ms_setup"],
                        "k_learn_trials": learned["ms_trials"], "k_setup": plain["ms_setup"],
                        "k_trials": plain["ms_trials"]},

@@ -159,6 +159,29 @@ int iss_get_event_counts(iss_ctx* ctx, int64_t* counts);
 int iss_get_hadrons(iss_ctx* ctx, iss_hadron* dst, int64_t capacity);
 int iss_get_stats(const iss_ctx* ctx, iss_stats* stats);
 
+/* ---- launch autotuner (no counterpart in the reference).  Times short
+ * sampling passes of the current surface for each candidate launch shape of
+ * the trial kernel (idle lanes before a refill, i.e. the draws a lane takes
+ * at once; resident blocks per CU, i.e. the hadrons per wavefront) and keeps
+ * the fastest for that proposal.  The samples do not depend on the launch
+ * shape (counter-based random numbers). ---- */
+typedef struct {
+    int32_t proposal;        /* 0 plain, 1 learned (needs network parameters) */
+    int32_t n_events;        /* events per timed pass */
+    uint64_t seed;
+    int32_t repeats;         /* passes per candidate (the minimum counts) */
+} iss_tune_opts;
+
+typedef struct {
+    int32_t refill_min;      /* chosen */
+    int32_t blocks_per_cu;   /* chosen; 0 = occupancy limit */
+    double ms_best;          /* trial-kernel time of the chosen shape */
+    double ms_default;       /* trial-kernel time of the default shape */
+    int32_t n_tried;
+} iss_tune_report;
+
+int iss_autotune(iss_ctx* ctx, const iss_tune_opts* opts, iss_tune_report* report);
+
 /* ---- learned proposal (no counterpart in the reference; the reference's
  * rejection sampler, src/FSSW.cpp:1852-1966, is the plain proposal).
  * A 14 -> 64 -> 64 -> 8 network (BF16, FP32 accumulation) shapes the

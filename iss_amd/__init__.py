@@ -93,6 +93,18 @@ class TrainReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class _TuneOpts(C.Structure):
+    _fields_ = [("proposal", C.c_int32), ("n_events", C.c_int32), ("seed", C.c_uint64), ("repeats", C.c_int32)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("refill_min", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms_best", C.c_double),
+                ("ms_default", C.c_double), ("n_tried", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # network layout (parameters, FP32, row-major [out][in])
 N_IN, N_HID, N_OUT = 16, 64, 8
 N_BANDS = 9
@@ -115,6 +127,7 @@ _SIGNATURES = {
     "iss_get_event_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "iss_get_hadrons": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "iss_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "iss_autotune": (C.c_int, [C.c_void_p, C.POINTER(_TuneOpts), C.POINTER(TuneReport)]),
     "iss_mlp_num_params": (C.c_int, []),
     "iss_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "iss_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
@@ -269,6 +282,12 @@ class Context:
         s = Stats()
         self._check(self.lib.iss_get_stats(self.h, C.byref(s)), "iss_get_stats")
         return s
+
+    def autotune(self, proposal=0, n_events=200, seed=99, repeats=2) -> dict:
+        o = _TuneOpts(proposal, n_events, seed, repeats)
+        r = TuneReport()
+        self._check(self.lib.iss_autotune(self.h, C.byref(o), C.byref(r)), "iss_autotune")
+        return r.as_dict()
 
     # ------------------------------------------------------ learned proposal
     def num_params(self) -> int:

@@ -134,6 +134,9 @@ struct iss_ctx {
     DBuf<float> d_f32a, d_f32b, d_f32c;
     int64_t n_train = 0;
     bool last_plain = false;  // d_setup / d_out hold the last plain pass
+    // launch shape of the trial kernels per proposal (autotuner)
+    int refill_min[2] = {kRefillMin, kRefillMin};
+    int blocks_per_cu[2] = {0, 0};  // 0: occupancy limit
 };
 
 namespace {
@@ -756,6 +759,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
     a.max_trials = o->max_trials;
+    a.refill_min = ctx->refill_min[o->proposal == 1 ? 1 : 0];
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
     a.Kdev = nullptr;
@@ -763,6 +767,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, 256, 0) != hipSuccess ||
         per_cu_trials < 1)
         per_cu_trials = 1;
+    const int per_cu_plain = ctx->blocks_per_cu[0] > 0 ? std::min(ctx->blocks_per_cu[0], per_cu_trials) : per_cu_trials;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
     if (o->proposal == 0) {
@@ -772,7 +777,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
             const int64_t want = (K + 255) / 256;
             const int nblk = static_cast<int>(
-                std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu_trials)));
+                std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu_plain)));
             k_trials<<<nblk, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
         }
         ctx->last_plain = true;
@@ -799,6 +804,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_learn_trials, 256, 0) != hipSuccess ||
                 per_cu < 1)
                 per_cu = 1;
+            if (ctx->blocks_per_cu[1] > 0) per_cu = std::min(ctx->blocks_per_cu[1], per_cu);
             const int64_t want = (K + 255) / 256;
             const int nblk = static_cast<int>(
                 std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
@@ -861,6 +867,63 @@ int iss_get_hadrons(iss_ctx* ctx, iss_hadron* dst, int64_t capacity) {
 int iss_get_stats(const iss_ctx* ctx, iss_stats* s) {
     if (!ctx || !s) return ISS_ERR_ARG;
     *s = ctx->stats;
+    return ISS_OK;
+}
+
+int iss_autotune(iss_ctx* ctx, const iss_tune_opts* o, iss_tune_report* rep) {
+    if (!ctx || !o || (o->proposal != 0 && o->proposal != 1) || o->n_events <= 0 || o->repeats <= 0)
+        return ISS_ERR_ARG;
+    if (o->proposal == 1 && !ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters");
+    const int p = o->proposal;
+    iss_sample_opts so{};
+    so.seed = o->seed;
+    so.event0 = 0;
+    so.n_events = o->n_events;
+    so.proposal = p;
+    so.sampling_model = 30;
+    so.hydro_mode = 2;
+    so.y_LB = -5.;
+    so.y_RB = 5.;
+    so.max_trials = 5000;
+    so.precision = ctx->yields_valid ? ctx->yield_precision : 32;
+    const int refill[] = {8, 16, 24, 32, 48};
+    const int bpc[] = {0, 3, 2};  // occupancy limit, then fewer resident blocks
+    auto timed = [&](int rf, int bp, double* ms) -> int {
+        ctx->refill_min[p] = rf;
+        ctx->blocks_per_cu[p] = bp;
+        double best = 1e30;
+        for (int r = 0; r < o->repeats; r++) {
+            int64_t n = 0;
+            int rc = iss_sample(ctx, &so, &n);
+            if (rc) return rc;
+            best = std::min(best, ctx->stats.ms_trials_kernel);
+        }
+        *ms = best;
+        return ISS_OK;
+    };
+    double ms_def = 0.;
+    int rc = timed(kRefillMin, 0, &ms_def);
+    if (rc) return rc;
+    double best = ms_def;
+    int best_rf = kRefillMin, best_bp = 0, tried = 1;
+    for (int bp : bpc)
+        for (int rf : refill) {
+            if (rf == kRefillMin && bp == 0) continue;
+            double ms = 0.;
+            rc = timed(rf, bp, &ms);
+            if (rc) return rc;
+            tried++;
+            if (ms < best) { best = ms; best_rf = rf; best_bp = bp; }
+        }
+    ctx->refill_min[p] = best_rf;
+    ctx->blocks_per_cu[p] = best_bp;
+    if (rep) {
+        rep->refill_min = best_rf;
+        rep->blocks_per_cu = best_bp;
+        rep->ms_best = best;
+        rep->ms_default = ms_def;
+        rep->n_tried = tried;
+    }
     return ISS_OK;
 }
 

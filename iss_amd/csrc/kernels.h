@@ -94,6 +94,7 @@ struct SampleArgs {
     int boost_invariant;
     float yLB, yRB;
     int max_trials;
+    int refill_min;          // idle lanes that trigger a refill in the trial kernels
     void* out;               // iss_hadron[K]
     unsigned long long* stats;  // trials, inner, failed, range errors
 };

@@ -191,6 +191,62 @@ __device__ inline void lparams_of(const SampleArgs& a, const float* __restrict__
     if ((sign < 0 && q.x < 1e-3) || (a.pt.quirk && tb >= 3)) P.flags |= 2;
 }
 
+// The same parameters for the trial kernel, in FP32, given x (from the
+// set-up record): mu' = min(m, B muB + S muS + Q muQ) with the float sum of
+// pstate_of.
+__device__ __forceinline__ void lparams_fast(const SampleArgs& a, const float* __restrict__ aux, int64_t c, int s, float x,
+                                    LParams& P) {
+    const int64_t Nc = a.Nc;
+    const float T = a.cells[F_T * Nc + c];
+    const float muB = a.cells[F_MUB * Nc + c], muS = a.cells[F_MUS * Nc + c], muQ = a.cells[F_MUQ * Nc + c];
+    const float Bq = a.sp.B[s], Sq = a.sp.S[s], Qq = a.sp.Q[s];
+    const float m = static_cast<float>(a.sp.mass[s]);
+    const int sign = a.sp.sign[s];
+    P.T = T;
+    P.mu = fminf(m, Bq * muB + Sq * muS + Qq * muQ);
+    P.m = m;
+    P.sgn = static_cast<float>(sign);
+    P.x = x;
+    const int tb = ((sign == -1) ? 0 : 3) + ((x < 30.f) ? 0 : (x < 50.f ? 1 : 2));
+    P.tmax = static_cast<float>(a.pt.Et0[tb] + (a.pt.len[tb] - 1) * a.pt.dEt[tb]);
+    P.flags = 0;
+    P.d0 = a.cells[F_DA0 * Nc + c];
+    P.dn = aux[A_DN * Nc + c];
+    P.D = fabsf(P.d0) + P.dn;
+    P.l1 = aux[A_L1 * Nc + c]; P.l2 = aux[A_L2 * Nc + c]; P.l3 = aux[A_L3 * Nc + c];
+    P.dh0 = aux[A_DH0 * Nc + c]; P.dh1 = aux[A_DH1 * Nc + c]; P.dh2 = aux[A_DH2 * Nc + c];
+    const float Ed = a.cells[F_E * Nc + c], Pd = a.cells[F_P * Nc + c];
+    float cf0 = 0.f, cf1 = 0.f, cf2 = 0.f, cf3 = 0.f, cf4 = 0.f, cf5 = 0.f;
+    if (a.bulk != BULK_NONE || a.neos >= 0) {
+        cf0 = a.coef[c]; cf1 = a.coef[Nc + c]; cf2 = a.coef[2 * Nc + c];
+        cf3 = a.coef[3 * Nc + c]; cf4 = a.coef[4 * Nc + c]; cf5 = a.coef[5 * Nc + c];
+    }
+    P.smode = 0;
+    P.ps0 = 0.f;
+    if (a.shear) {
+        if (a.neos == 1) { P.smode = 2; P.ps0 = 1.f / (2.f * cf2) / T; }
+        else if (a.neos == 0) { P.smode = 1; P.ps0 = cf0; }
+        else { P.smode = 1; P.ps0 = 1.0f / (2.0f * T * T * (Ed + Pd)); }
+    }
+    P.qi = P.q0 = P.q1 = P.q2 = 0.f;
+    if (a.bulk != BULK_NONE) {
+        float Pi = a.cells[F_PI * Nc + c];
+        if (a.bulk == BULK_RTA) Pi = Pi / static_cast<float>(kHbarC);
+        if (a.bulk == BULK_RTA || a.bulk == BULK_CE) {
+            P.qi = -cf0 * Pi * m * m / (3.f * T);
+            P.q1 = cf0 * cf1 * Pi / T;
+        } else if (a.bulk == BULK_14MOM) {
+            P.q0 = Pi * cf0 * m * m;
+            P.q1 = Pi * cf1 * Bq;
+            P.q2 = Pi * cf2;
+        } else if (a.bulk == BULK_22MOM) {
+            P.q0 = Pi * m * m * cf2;
+            P.q1 = Pi * (Bq * cf3 + Sq * cf4 + Qq * cf5);
+            P.q2 = Pi * (cf1 - cf2);
+        }
+    }
+}
+
 // 14 network inputs (two zero pads): T, muB, the five rest-frame shear
 // components and the bulk pressure relative to e + P, mass, degeneracy,
 // baryon number, statistics, and the shear and bulk delta-f at the thermal
@@ -689,7 +745,8 @@ struct alignas(16) LRec {
     uint32_t cs[kBands];  // c | s << 16 (fp16)
     uint16_t kap[8];      // fp16 (the far tail has kappa = 0)
     uint8_t beta[kBands];
-    uint8_t pad[7];
+    uint8_t pad[3];
+    float x;              // (m - mu) / T, the lower end of t
 };
 static_assert(sizeof(LRec) == 96, "LRec layout");
 
@@ -749,15 +806,15 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
         r.beta[k] = flags ? 0 : static_cast<uint8_t>((B.band[k].y >> 16) & 0xFFu);
         if (k < 8) r.kap[k] = flags ? 0 : static_cast<uint16_t>(B.band[k].y & 0xFFFFu);
     }
-    for (int k = 0; k < 7; k++) r.pad[k] = 0;
+    for (int k = 0; k < 3; k++) r.pad[k] = 0;
+    r.x = P.x;
     rec[i] = r;
 }
 
 // Persistent trial kernel of the learned sampler with per-lane refill (as
 // k_trials).  A lane keeps the hadron's parameters and band thresholds in
 // registers (thresholds, lines, kappas, betas: 20 words).  Proposal
-// `it` of hadron (c, j) uses Philox blocks (c, j, TAG_LEARN, it) and
-// (c, j, TAG_LEARN2, it).
+// `it` of hadron (c, j) uses the Philox block (c, j, TAG_LEARN, it).
 __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* __restrict__ rec) {
     __shared__ unsigned long long s_stats[6];
     if (threadIdx.x < 6) s_stats[threadIdx.x] = 0ull;
@@ -771,7 +828,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
     const int64_t end = beg + base + (wid < rem ? 1 : 0);
     const int64_t Nc = a.Nc;
     const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
-    const uint32_t tw1 = tag_word(TAG_LEARN, a.chunk), tw2 = tag_word(TAG_LEARN2, a.chunk);
+    const uint32_t tw1 = tag_word(TAG_LEARN, a.chunk);
     // one proposal is one outer trial of the reference (src/FSSW.cpp:1872)
     const int max_it = a.max_trials;
 
@@ -792,7 +849,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
 
     while (true) {
         const unsigned long long need = __ballot(!active);
-        if (need && next < end && (__popcll(need) >= kRefillMin || need == ~0ull)) {
+        if (need && next < end && (__popcll(need) >= a.refill_min || need == ~0ull)) {
             const int64_t navail = end - next;
             const int rank = __popcll(need & ((1ull << lane) - 1));
             if (!active && rank < navail) {
@@ -821,7 +878,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
                     CS[5] = h3.x; CS[6] = h3.y; CS[7] = h3.z; CS[8] = h3.w;
                     KW[0] = h4.x; KW[1] = h4.y; KW[2] = h4.z; KW[3] = h4.w;
                     BW[0] = h5.x; BW[1] = h5.y; BW[2] = h5.z;
-                    lparams_of(a, L.aux, c, s, P);
+                    lparams_fast(a, L.aux, c, s, __uint_as_float(h5.w), P);
                     it = 0;
                     active = true;
                 }
@@ -833,9 +890,10 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             continue;
         }
         if (!active) continue;
-        // ---- one proposal
+        // ---- one proposal: one Philox block gives the band (16 bits), the
+        // flip (16), t, the axis and cos, phi (24 each) and the acceptance
+        // (the three low bytes)
         const u32x4 w1r = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw1, static_cast<uint32_t>(it)}, key0, key1);
-        const u32x4 w2r = philox4x32(u32x4{static_cast<uint32_t>(c), jj, tw2, static_cast<uint32_t>(it)}, key0, key1);
         it++;
         n_it++;
         const uint32_t w16 = w1r.x >> 16;
@@ -872,7 +930,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         const float G = flog(p * E * fe) - tau;
         // direction in the shear eigenframe
         const float P0 = (1.f + kk * P.l1) * (1.f / 3.f), P1 = P0 + (1.f + kk * P.l2) * (1.f / 3.f);
-        const float uy = u24(w2r.x);
+        const float uy = u24(w1r.z);
         int ax;
         float rr;
         if (uy < P0) { ax = 0; rr = fdiv(uy, P0); }
@@ -882,13 +940,14 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         const float cz = fcbrt(2.f * rr - 1.f);
         const float sz = sqrtf(fmaxf(1.f - cz * cz, 0.f));
         float sph, cph;
-        __sincosf(2.f * static_cast<float>(kPi) * u24(w2r.y), &sph, &cph);
+        __sincosf(2.f * static_cast<float>(kPi) * u24(w1r.w), &sph, &cph);
         float n0, n1, n2;
         if (ax == 0) { n0 = cz; n1 = sz * cph; n2 = sz * sph; }
         else if (ax == 1) { n1 = cz; n2 = sz * cph; n0 = sz * sph; }
         else { n2 = cz; n0 = sz * cph; n1 = sz * sph; }
         float y = n0 * P.dh0 + n1 * P.dh1 + n2 * P.dh2;
-        if (u24(w1r.w) >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }
+        const float uf = (static_cast<float>(w1r.x & 0xFFFFu) + 0.5f) * (1.f / 65536.f);
+        if (uf >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }
         const float W = P.l1 * n0 * n0 + P.l2 * n1 * n1 + P.l3 * n2 * n2;
         // the exact Cooper-Frye weights of the plain sampler
         const float w1 = (P.D > 0.f) ? clamp01(fdiv(E * P.d0 + p * P.dn * y, E * P.D)) : 0.f;
@@ -898,7 +957,8 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         const float racc = fdiv(fexp(G - (ck + sk * tau)) * w1 * w2, (1.f + kk * W) * (1.f + bk * y));
         if (racc > 1.f + 1e-5f) n_viol++;
         bool done = false;
-        if (u24(w1r.z) < racc) {
+        const uint32_t ua = (w1r.y & 0xFFu) | ((w1r.z & 0xFFu) << 8) | ((w1r.w & 0xFFu) << 16);
+        if ((static_cast<float>(ua) + 0.5f) * (1.0f / 16777216.0f) < racc) {
             const float R00 = L.aux[A_R00 * Nc + c], R01 = L.aux[A_R01 * Nc + c], R02 = L.aux[A_R02 * Nc + c];
             const float R10 = L.aux[A_R10 * Nc + c], R11 = L.aux[A_R11 * Nc + c], R12 = L.aux[A_R12 * Nc + c];
             const float R20 = L.aux[A_R20 * Nc + c], R21 = L.aux[A_R21 * Nc + c], R22 = L.aux[A_R22 * Nc + c];

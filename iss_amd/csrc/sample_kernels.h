@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
 // (c, j, tag, it): word 0 is the 32-bit CDF draw, the other 96 bits give four
 // 24-bit uniforms.  Streams depend on (c, j) only, so the output does not
 // depend on the launch shape.
-// free lanes that trigger a refill
+// idle lanes that trigger a refill, unless the autotuner chose otherwise
 constexpr int kRefillMin = 16;
 
 __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) {
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
 
     while (true) {
         const unsigned long long need = __ballot(!active);
-        if (need && next < end && (__popcll(need) >= kRefillMin || need == ~0ull)) {
+        if (need && next < end && (__popcll(need) >= a.refill_min || need == ~0ull)) {
             const int64_t navail = end - next;
             const int rank = __popcll(need & ((1ull << lane) - 1));
             if (!active && rank < navail) {

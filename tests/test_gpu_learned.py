@@ -252,3 +252,23 @@ def test_learned_reference_tmunu(case):
     diff = np.concatenate([Tf - Ts, [Qf[0] - nBs]])
     print(case, "fallback", st.n_fallback, "of", n, "mean|diff|", np.mean(np.abs(diff)))
     assert np.mean(np.abs(diff)) < 1e-3
+
+
+def test_autotuned_launch_shape_keeps_the_samples():
+    """The autotuner only changes launch shapes; with counter-based random
+    numbers the hadrons of a pass are bit-identical before and after."""
+    ctx, sp = _ctx()
+    cells, _ = synth.viscous_lrf(3000, sp, seed=8, target_hadrons=600.0)
+    ctx.set_surface_lrf(cells)
+    ctx.compute_yields(precision=32)
+    ctx.mlp_init(0)
+    ref = {}
+    for prop in (0, 1):
+        ctx.sample(200, seed=21, proposal=prop)
+        ref[prop] = ctx.hadrons().copy()
+    for prop in (0, 1):
+        rep = ctx.autotune(proposal=prop, n_events=100, seed=3, repeats=1)
+        print("proposal", prop, rep)
+        assert rep["n_tried"] >= 10 and rep["ms_best"] <= rep["ms_default"] + 1e-9
+        ctx.sample(200, seed=21, proposal=prop)
+        assert np.array_equal(ctx.hadrons(), ref[prop])
