@@ -228,6 +228,42 @@ int iss_mlp_eval(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* sp
 int iss_band_diag(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* species, const float* rho,
                   float* tables, float* params);
 
+/* ---- smooth spectra: EmissionFunctionArray::calculate_dN_pTdpTdphidy
+ * (src/emissionfunction.cpp:625-844) for several species at once, from the
+ * resident surface with the context's delta f.  dN / (pT dpT dphi dy) in
+ * GeV^-2 on the grid pT[n_pT] (GeV) x phi[n_phi].
+ *   mode 0 (the reference): each cell is integrated over the eta_s table,
+ *          the particle at rapidity y = eta_cell - eta[k] with weight w[k]
+ *          (src/emissionfunction.cpp:232-238, 756-757); out[s][i_pT][i_phi].
+ *   mode 1: no eta_s integration; eta[] are particle rapidities, w unused;
+ *          out[s][i_y][i_pT][i_phi] (rapidity-resolved 3+1D spectra).
+ * species: indices into the context's species table. ---- */
+typedef struct {
+    int32_t precision;        /* 64: FP64 throughout; 32: FP32 integrand, FP64 cell sums */
+    int32_t mode;             /* 0 or 1, above */
+    int32_t all_cells;        /* 1: also the lab cells dropped for u.dsigma < 0 (the
+                                 reference's smooth path uses every cell); 0: sampled cells */
+    int32_t use_pos_dN_only;  /* drop negative contributions (:804-806) */
+    int32_t restrict_deltaf;  /* rescale to |delta f| <= deltaf_max_ratio (:791-799) */
+    double deltaf_max_ratio;
+} iss_spectra_opts;
+
+int iss_spectra(iss_ctx* ctx, const iss_spectra_opts* opts, const int32_t* species, int32_t n_species,
+                const double* pT, int32_t n_pT, const double* phi, int32_t n_phi, const double* eta,
+                const double* eta_weight, int32_t n_eta, double* out);
+/* device time of the last iss_spectra, ms (record + integral + reduction) */
+double iss_spectra_ms(const iss_ctx* ctx);
+
+/* ---- flows: EmissionFunctionArray::calculate_flows
+ * (src/emissionfunction.cpp:875-1023) on one spectrum dN[n_pT][n_phi].
+ * vn_diff [n_pT][3 + 3 to_order]: pT, mT - m, dN/(pT dpT dy) / (2 pi)... as the
+ *   reference's thermal_*_vndata rows (pT, mT - m, norm / 2pi, then per order
+ *   re, im, |v_n|); vn_inte [to_order + 1][6]: order, numerator re, im, v_n re,
+ *   im, |v_n|, row 0 holding the normalisation.  Host arithmetic, no context. ---- */
+int iss_flows(const double* dN, const double* pT, const double* pT_weight, int32_t n_pT, const double* phi,
+              const double* phi_weight, int32_t n_phi, double mass, int32_t to_order, double* vn_diff,
+              double* vn_inte);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,11 @@ class TuneReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class _SpectraOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("mode", C.c_int32), ("all_cells", C.c_int32),
+                ("use_pos_dN_only", C.c_int32), ("restrict_deltaf", C.c_int32), ("deltaf_max_ratio", C.c_double)]
+
+
 # network layout (parameters, FP32, row-major [out][in])
 N_IN, N_HID, N_OUT = 16, 64, 8
 N_BANDS = 9
@@ -138,6 +143,11 @@ _SIGNATURES = {
                                C.c_void_p, C.c_void_p, C.c_void_p]),
     "iss_band_diag": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
+    "iss_spectra": (C.c_int, [C.c_void_p, C.POINTER(_SpectraOpts), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                              C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "iss_spectra_ms": (C.c_double, [C.c_void_p]),
+    "iss_flows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                            C.c_double, C.c_int32, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -337,3 +347,46 @@ class Context:
         self._check(self.lib.iss_band_diag(self.h, cell.size, _ptr(cell), _ptr(species), _ptr(rho),
                                            _ptr(tab), _ptr(par)), "iss_band_diag")
         return tab, par
+
+    # ------------------------------------------------------- smooth spectra
+    def spectra(self, species, pT, phi, eta, eta_weight=None, *, mode=0, precision=64, all_cells=1,
+                use_pos_dN_only=0, restrict_deltaf=1, deltaf_max_ratio=1.0):
+        """dN/(pT dpT dphi dy) [GeV^-2] of the given species (indices into the
+        species table) on pT x phi; mode 0 integrates each cell over the eta_s
+        table, mode 1 evaluates at the rapidities ``eta``.  Returns
+        [n_species, n_pT, n_phi] (mode 0) or [n_species, n_y, n_pT, n_phi]."""
+        species = np.ascontiguousarray(np.atleast_1d(species), dtype=np.int32)
+        pT = np.ascontiguousarray(pT, dtype=np.float64)
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        eta = np.ascontiguousarray(np.atleast_1d(eta), dtype=np.float64)
+        w = None if eta_weight is None else np.ascontiguousarray(eta_weight, dtype=np.float64)
+        o = _SpectraOpts(precision, mode, all_cells, use_pos_dN_only, restrict_deltaf, deltaf_max_ratio)
+        shape = (species.size, pT.size, phi.size) if mode == 0 else (species.size, eta.size, pT.size, phi.size)
+        out = np.zeros(shape, dtype=np.float64)
+        self._check(self.lib.iss_spectra(self.h, C.byref(o), _ptr(species), species.size, _ptr(pT), pT.size,
+                                         _ptr(phi), phi.size, _ptr(eta), _ptr(w), eta.size, _ptr(out)),
+                    "iss_spectra")
+        return out
+
+    def spectra_ms(self):
+        return self.lib.iss_spectra_ms(self.h)
+
+
+def flows(dN, pT, pT_weight, phi, phi_weight, mass, to_order=9):
+    """calculate_flows of one spectrum dN[n_pT, n_phi]: (vn_diff [n_pT, 3 + 3 to_order],
+    vn_inte [to_order + 1, 6]) in the reference's column order."""
+    lib = load_library()
+    dN = np.ascontiguousarray(dN, dtype=np.float64)
+    pT = np.ascontiguousarray(pT, dtype=np.float64)
+    pw = np.ascontiguousarray(pT_weight, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    fw = np.ascontiguousarray(phi_weight, dtype=np.float64)
+    if dN.shape != (pT.size, phi.size):
+        raise ISSError("flows: dN must be [n_pT, n_phi]")
+    diff = np.zeros((pT.size, 3 + 3 * to_order))
+    inte = np.zeros((to_order + 1, 6))
+    rc = lib.iss_flows(_ptr(dN), _ptr(pT), _ptr(pw), pT.size, _ptr(phi), _ptr(fw), phi.size, float(mass),
+                       to_order, _ptr(diff), _ptr(inte))
+    if rc != 0:
+        raise ISSError(f"iss_flows: status {rc}")
+    return diff, inte

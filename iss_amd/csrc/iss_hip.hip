@@ -10,6 +10,8 @@
 //   scan                      cell offsets (int64)
 //   k_multinomial_level       hadrons per event: Multinomial(K, 1/E), binomial tree
 //   k_sample                  species pick, LRF momentum rejection, boost, output
+//   k_spectra_rec + k_spectra smooth dN/(pT dpT dphi dy) on a (pT, phi) grid
+//                             (+ k_spectra_reduce), host-side flows
 //
 // Poisson splitting: the reference draws N_{e,s} ~ Poisson(sum_c N_{c,s}) per
 // event and species and then a cell per hadron from the species' cell CDF
@@ -37,6 +39,7 @@ using namespace issdev;
 #include "yield_kernels.h"
 #include "sample_kernels.h"
 #include "learned_kernels.h"
+#include "spectra_kernels.h"
 
 // =================================================================== host
 
@@ -99,6 +102,10 @@ struct iss_ctx {
     int64_t Nc = 0;
     DBuf<float> d_cells, d_tmp;
     DBuf<double> d_coef;
+    // lab cells dropped for u.dsigma < 0 (kept for the smooth spectra)
+    int64_t Nneg = 0;
+    DBuf<float> d_cells_neg;
+    DBuf<double> d_coef_neg;
     DBuf<uint32_t> d_u32;
     DBuf<int64_t> d_i64, d_scan;
     DBuf<double> d_eos;
@@ -137,6 +144,11 @@ struct iss_ctx {
     // launch shape of the trial kernels per proposal (autotuner)
     int refill_min[2] = {kRefillMin, kRefillMin};
     int blocks_per_cu[2] = {0, 0};  // 0: occupancy limit
+    // smooth spectra
+    DBuf<char> d_srec, d_sgrid;
+    DBuf<double> d_spart, d_sout;
+    DBuf<int32_t> d_sspec;
+    double spectra_ms = 0.;
 };
 
 namespace {
@@ -250,6 +262,12 @@ DeltafDev deltaf_dev(iss_ctx* ctx) {
 }
 
 int cell_prep(iss_ctx* ctx) {
+    if (ctx->Nneg > 0) {
+        HIPCHK(ctx->d_coef_neg.ensure(6 * ctx->Nneg));
+        k_cell_prep<<<(ctx->Nneg + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cells_neg.p, ctx->Nneg, deltaf_dev(ctx),
+                                                                       ctx->d_coef_neg.p);
+        HIPCHK(hipGetLastError());
+    }
     if (ctx->Nc == 0) return ISS_OK;
     HIPCHK(ctx->d_coef.ensure(6 * ctx->Nc));
     const int64_t nb = (ctx->Nc + 255) / 256;
@@ -464,6 +482,97 @@ double mean_loss(iss_ctx* ctx, int nb) {
     return nv > 0. ? ls / nv : 0.;
 }
 
+template <typename R>
+int run_spectra(iss_ctx* ctx, const iss_spectra_opts* o, const int32_t* species, int ns, const double* pT, int npT,
+                const double* phi, int nphi, const double* eta, const double* w, int ny, double* out) {
+    const int64_t Nk = ctx->Nc, Nd = o->all_cells ? ctx->Nneg : 0, N = Nk + Nd;
+    const int64_t npl = static_cast<int64_t>(npT) * nphi;
+    const int64_t npts = npl * (o->mode == 1 ? ny : 1);
+    const int64_t nout = ns * npts;
+    if (npts > (int64_t{1} << 30)) return set_err(ctx, ISS_ERR_ARG, "spectra grid too large");
+    const int64_t wps = (npts + 64 * kSpecPts - 1) / (64 * kSpecPts);
+    if (ns * wps > 65535) return set_err(ctx, ISS_ERR_ARG, "spectra: species x grid points exceed one launch");
+    ctx->spectra_ms = 0.;
+    if (N == 0) {
+        std::fill(out, out + nout, 0.);
+        return ISS_OK;
+    }
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    // per-cell records: the sampled cells, then the dropped ones
+    HIPCHK(ctx->d_srec.ensure(sizeof(double) * kSpecRec * N));
+    double* rec = reinterpret_cast<double*>(ctx->d_srec.p);
+    const int shear = ctx->df.include_shear;
+    const int bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
+    if (Nk > 0)
+        k_spectra_rec<<<(Nk + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->d_coef.p, Nk, shear, bulk,
+                                                                   ctx->neos, rec);
+    if (Nd > 0)
+        k_spectra_rec<<<(Nd + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cells_neg.p, ctx->d_coef_neg.p, Nd, shear,
+                                                                   bulk, ctx->neos, rec + Nk * kSpecRec);
+    HIPCHK(hipGetLastError());
+    // grids: pT, cos phi, sin phi, cosh, sinh, weight of the y - eta_s nodes
+    // (mode 0: y - eta_s = -eta[k]) or of the rapidities (mode 1)
+    std::vector<double> g(static_cast<size_t>(npT) + 2 * nphi + 3 * ny);
+    double* gp = g.data();
+    for (int i = 0; i < npT; i++) gp[i] = pT[i];
+    for (int j = 0; j < nphi; j++) {
+        gp[npT + j] = std::cos(phi[j]);
+        gp[npT + nphi + j] = std::sin(phi[j]);
+    }
+    const int oy = npT + 2 * nphi;
+    for (int k = 0; k < ny; k++) {
+        const double y = o->mode == 1 ? eta[k] : -eta[k];
+        gp[oy + k] = std::cosh(y);
+        gp[oy + ny + k] = std::sinh(y);
+        gp[oy + 2 * ny + k] = o->mode == 1 ? 1. : w[k];
+    }
+    HIPCHK(ctx->d_sgrid.ensure(sizeof(double) * g.size()));
+    HIPCHK(ctx->d_sspec.ensure(ns));
+    HIPCHK(hipMemcpyAsync(ctx->d_sgrid.p, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_sspec.p, species, sizeof(int32_t) * ns, hipMemcpyHostToDevice, ctx->stream));
+    const double* dg = reinterpret_cast<const double*>(ctx->d_sgrid.p);
+    // cell chunks: about 32 waves per CU over all (chunk, point-wave) pairs
+    const int64_t pw = ns * wps;
+    const int64_t target = static_cast<int64_t>(ctx->num_cu) * 32;
+    int64_t nch = std::max<int64_t>(1, std::min<int64_t>((target + pw - 1) / pw, (N + 7) / 8));
+    const int64_t cpc = (N + nch - 1) / nch;
+    nch = (N + cpc - 1) / cpc;
+    HIPCHK(ctx->d_spart.ensure(nch * nout));
+    HIPCHK(ctx->d_sout.ensure(nout));
+    SpectraArgs a{};
+    a.rec = rec;
+    a.N = N;
+    a.cells_per_chunk = cpc;
+    a.sp = species_dev(ctx);
+    a.species = ctx->d_sspec.p;
+    a.ns = ns;
+    a.pT = dg;
+    a.npT = npT;
+    a.cphi = dg + npT;
+    a.sphi = dg + npT + nphi;
+    a.nphi = nphi;
+    a.ych = dg + oy;
+    a.ysh = dg + oy + ny;
+    a.yw = dg + oy + 2 * ny;
+    a.ny = ny;
+    a.mode = o->mode;
+    a.positive_only = o->use_pos_dN_only != 0;
+    a.restrict_ratio = o->restrict_deltaf ? o->deltaf_max_ratio : 0.;
+    a.waves_per_species = static_cast<int>(wps);
+    a.partial = ctx->d_spart.p;
+    k_spectra<R><<<dim3(static_cast<unsigned>(nch), static_cast<unsigned>(pw)), 64, 0, ctx->stream>>>(a);
+    HIPCHK(hipGetLastError());
+    k_spectra_reduce<<<(nout + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_spart.p, nch, nout, ctx->d_sout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, ctx->d_sout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->spectra_ms = ms;
+    return ISS_OK;
+}
+
 }  // namespace
 
 // ================================================================ C-ABI
@@ -509,6 +618,8 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_tpart.release(); ctx->d_tloss.release(); ctx->d_tout.release(); ctx->d_tr_cell.release();
     ctx->d_ev_cell.release(); ctx->d_tr_spec.release(); ctx->d_ev_spec.release(); ctx->d_i32a.release();
     ctx->d_i32b.release(); ctx->d_f32a.release(); ctx->d_f32b.release(); ctx->d_f32c.release();
+    ctx->d_cells_neg.release(); ctx->d_coef_neg.release(); ctx->d_srec.release(); ctx->d_sgrid.release();
+    ctx->d_spart.release(); ctx->d_sout.release(); ctx->d_sspec.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -619,6 +730,7 @@ int iss_set_surface_lab(iss_ctx* ctx, int64_t n, const float* cells34, const iss
     }
     ctx->yields_valid = false;
     ctx->Nc = 0;
+    ctx->Nneg = 0;
     if (n == 0) return ISS_OK;
     DBuf<float> lab;
     HIPCHK(lab.ensure(34 * n));
@@ -637,9 +749,16 @@ int iss_set_surface_lab(iss_ctx* ctx, int64_t n, const float* cells34, const iss
     HIPCHK(ctx->d_cells.ensure(kCellFields * std::max<int64_t>(nk, 1)));
     k_compact<<<nb, 256, 0, ctx->stream>>>(ctx->d_tmp.p, ctx->d_u32.p, ctx->d_i64.p, n, nk, ctx->d_cells.p);
     HIPCHK(hipGetLastError());
+    if (n > nk) {
+        HIPCHK(ctx->d_cells_neg.ensure(kCellFields * (n - nk)));
+        k_compact_dropped<<<nb, 256, 0, ctx->stream>>>(ctx->d_tmp.p, ctx->d_u32.p, ctx->d_i64.p, n, n - nk,
+                                                       ctx->d_cells_neg.p);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     lab.release();
     ctx->Nc = nk;
+    ctx->Nneg = n - nk;
     rc = cell_prep(ctx);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -651,6 +770,7 @@ int iss_set_surface_lrf(iss_ctx* ctx, int64_t n, const float* cells28) {
     (void)hipSetDevice(ctx->device);
     ctx->yields_valid = false;
     ctx->Nc = 0;
+    ctx->Nneg = 0;
     if (n == 0) return ISS_OK;
     HIPCHK(ctx->d_tmp.ensure(kCellFields * n));
     HIPCHK(ctx->d_cells.ensure(kCellFields * n));
@@ -1103,6 +1223,88 @@ int iss_band_diag(iss_ctx* ctx, int64_t n, const int32_t* cell, const int32_t* s
     HIPCHK(hipMemcpyAsync(tables, ctx->d_f32b.p, sizeof(float) * n * kBandDiag, hipMemcpyDeviceToHost, ctx->stream));
     if (params) HIPCHK(hipMemcpyAsync(params, ctx->d_f32c.p, sizeof(float) * n * 24, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ISS_OK;
+}
+
+int iss_spectra(iss_ctx* ctx, const iss_spectra_opts* o, const int32_t* species, int32_t n_species,
+                const double* pT, int32_t n_pT, const double* phi, int32_t n_phi, const double* eta,
+                const double* eta_weight, int32_t n_eta, double* out) {
+    if (!ctx || !o || !species || !pT || !phi || !eta || !out) return ISS_ERR_ARG;
+    if (n_species <= 0 || n_pT <= 0 || n_phi <= 0 || n_eta <= 0)
+        return set_err(ctx, ISS_ERR_ARG, "spectra: empty species list or grid");
+    if (o->mode != 0 && o->mode != 1) return set_err(ctx, ISS_ERR_ARG, "spectra: mode must be 0 or 1");
+    if (o->mode == 0 && !eta_weight) return set_err(ctx, ISS_ERR_ARG, "spectra: mode 0 needs eta weights");
+    if (o->precision != 32 && o->precision != 64) return set_err(ctx, ISS_ERR_ARG, "spectra: precision 32 or 64");
+    if (o->restrict_deltaf && !(o->deltaf_max_ratio > 0.))
+        return set_err(ctx, ISS_ERR_ARG, "spectra: deltaf_max_ratio must be positive");
+    if (ctx->Ns <= 0 || !ctx->tables_ready) return set_err(ctx, ISS_ERR_STATE, "species and delta f must be set");
+    for (int i = 0; i < n_species; i++)
+        if (species[i] < 0 || species[i] >= ctx->Ns) return set_err(ctx, ISS_ERR_ARG, "spectra: species index out of range");
+    for (int i = 0; i < n_pT; i++)
+        if (!(pT[i] >= 0.) || !std::isfinite(pT[i])) return set_err(ctx, ISS_ERR_ARG, "spectra: pT must be finite, >= 0");
+    for (int k = 0; k < n_eta; k++)
+        if (!std::isfinite(eta[k]) || std::fabs(eta[k]) > 20.) return set_err(ctx, ISS_ERR_ARG, "spectra: |eta| <= 20");
+    (void)hipSetDevice(ctx->device);
+    if (o->precision == 64)
+        return run_spectra<double>(ctx, o, species, n_species, pT, n_pT, phi, n_phi, eta, eta_weight, n_eta, out);
+    return run_spectra<float>(ctx, o, species, n_species, pT, n_pT, phi, n_phi, eta, eta_weight, n_eta, out);
+}
+
+double iss_spectra_ms(const iss_ctx* ctx) { return ctx ? ctx->spectra_ms : 0.; }
+
+// calculate_flows (src/emissionfunction.cpp:875-1023): Fourier moments of one
+// spectrum in phi per pT (differential) and weighted by pT dpT (integrated)
+int iss_flows(const double* dN, const double* pT, const double* pT_weight, int32_t n_pT, const double* phi,
+              const double* phi_weight, int32_t n_phi, double mass, int32_t to_order, double* vn_diff,
+              double* vn_inte) {
+    if (!dN || !pT || !pT_weight || !phi || !phi_weight || !vn_diff || !vn_inte) return ISS_ERR_ARG;
+    if (n_pT <= 0 || n_phi <= 0 || to_order < 1) return ISS_ERR_ARG;
+    const int nc = 3 + 3 * to_order;
+    std::vector<double> norm(n_pT, 0.), num(static_cast<size_t>(n_pT) * to_order * 2, 0.);
+    for (int i = 0; i < n_pT; i++) {
+        for (int j = 0; j < n_phi; j++) {
+            const double v = dN[static_cast<size_t>(i) * n_phi + j] * phi_weight[j];
+            norm[i] += v;
+            for (int n = 1; n <= to_order; n++) {
+                num[(static_cast<size_t>(i) * to_order + n - 1) * 2] += v * std::cos(n * phi[j]);
+                num[(static_cast<size_t>(i) * to_order + n - 1) * 2 + 1] += v * std::sin(n * phi[j]);
+            }
+        }
+        norm[i] += 1e-30;
+        double* row = vn_diff + static_cast<size_t>(i) * nc;
+        row[0] = pT[i];
+        row[1] = std::sqrt(mass * mass + pT[i] * pT[i]) - mass;
+        row[2] = norm[i] / (2. * kPi);
+        for (int n = 0; n < to_order; n++) {
+            const double re = num[(static_cast<size_t>(i) * to_order + n) * 2];
+            const double im = num[(static_cast<size_t>(i) * to_order + n) * 2 + 1];
+            row[3 + 3 * n] = re / norm[i];
+            row[4 + 3 * n] = im / norm[i];
+            row[5 + 3 * n] = std::sqrt(re * re + im * im) / norm[i];
+        }
+    }
+    double ni = 0.;
+    std::vector<double> vi(static_cast<size_t>(to_order) * 2, 0.);
+    for (int i = 0; i < n_pT; i++) {
+        const double wt = pT[i] * pT_weight[i];
+        ni += norm[i] * wt;
+        for (int n = 0; n < to_order; n++) {
+            vi[2 * n] += num[(static_cast<size_t>(i) * to_order + n) * 2] * wt;
+            vi[2 * n + 1] += num[(static_cast<size_t>(i) * to_order + n) * 2 + 1] * wt;
+        }
+    }
+    const double row0[6] = {0., ni, 0., 1., 0., 1.};
+    for (int c = 0; c < 6; c++) vn_inte[c] = row0[c];
+    for (int n = 0; n < to_order; n++) {
+        double* row = vn_inte + 6 * (n + 1);
+        const double re = vi[2 * n], im = vi[2 * n + 1];
+        row[0] = n + 1;
+        row[1] = re;
+        row[2] = im;
+        row[3] = re / ni;
+        row[4] = im / ni;
+        row[5] = std::sqrt(re * re + im * im) / ni;
+    }
     return ISS_OK;
 }
 

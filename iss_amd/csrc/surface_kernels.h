@@ -198,6 +198,16 @@ __global__ void k_compact(const float* __restrict__ tmp, const uint32_t* __restr
     for (int f = 0; f < kCellFields; f++) cells[f * nk + o] = tmp[f * n + i];
 }
 
+// the cells dropped for u.dsigma < 0, in input order, into their own record
+__global__ void k_compact_dropped(const float* __restrict__ tmp, const uint32_t* __restrict__ keep,
+                                  const int64_t* __restrict__ pos, int64_t n, int64_t nd,
+                                  float* __restrict__ cells) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || keep[i]) return;
+    const int64_t o = i - pos[i];
+    for (int f = 0; f < kCellFields; f++) cells[f * nd + o] = tmp[f * n + i];
+}
+
 // AoS (28 floats, FO_surf_LRF order) -> field-major
 __global__ void k_aos_to_soa(const float* __restrict__ aos, int64_t n, float* __restrict__ cells) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -10,8 +10,13 @@ Configurations (all seeded, generated in memory):
       Pi/(e+P) in [-0.2, 0], normalisation such that <hadrons/event> ~ 1e4.
   C5  as C2 with 2,000,000 cells and mu_B up to 0.45 GeV.
 
+  LAB viscous lab-frame cells (viscous_lab): eta_s in [-1.5, 1.5], flow with
+      longitudinal and transverse components, time-like and space-like
+      dsigma (a few per cent with u.dsigma < 0), pi^{mu nu} orthogonal to u
+      and traceless, for the smooth-spectra checks.
+
 Cells are returned as rest-frame records (28 floats, FO_surf_LRF order) or,
-for C1, as lab-frame MUSIC records (34 floats, FO_surf order).  e, P, n_B are
+for C1 and LAB, as lab-frame MUSIC records (34 floats, FO_surf order).  e, P, n_B are
 those of an ideal hadron-resonance gas of the same species list.
 """
 from __future__ import annotations
@@ -154,4 +159,69 @@ def ideal_binv_lab(sp=None):
     if sp is not None:
         e, P, nB = hrg_thermo(T, np.zeros(n), sp)
         c[:, 12], c[:, 14] = e, P
+    return c.astype(np.float32)
+
+
+def _boost_to_lab(u):
+    """Lorentz matrices taking rest-frame contravariant vectors to the frame
+    where the fluid moves with 4-velocity u (n x 4)."""
+    n = u.shape[0]
+    B = np.zeros((n, 4, 4))
+    B[:, 0, 0] = u[:, 0]
+    B[:, 0, 1:] = u[:, 1:]
+    B[:, 1:, 0] = u[:, 1:]
+    B[:, 1:, 1:] = np.eye(3)[None] + u[:, 1:, None] * u[:, None, 1:] / (u[:, 0] + 1.0)[:, None, None]
+    return B
+
+
+def viscous_lab(n_cells, sp, seed=7, muB_max=0.2, shear=True, bulk=True, frac_spacelike=0.3,
+                eta_max=1.5, area=1.0):
+    """Lab-frame MUSIC records (34 floats, FO_surf order) of a viscous
+    surface in Milne coordinates.  Components in the local orthonormal frame
+    (tau, x, y, eta): u3 = tau u^eta, da3 the eta_s component (p.dsigma =
+    tau (pt da0 + px da1 + py da2) + pz da3), pi^{mu nu} contravariant with
+    pi03 = tau pi^{tau eta}, pi33 = tau^2 pi^{eta eta}."""
+    rng = np.random.default_rng(seed)
+    n = int(n_cells)
+    c = np.zeros((n, 34), dtype=np.float64)
+    tau = rng.uniform(0.8, 12.0, n)
+    c[:, 0] = tau
+    c[:, 1] = rng.uniform(-8, 8, n)
+    c[:, 2] = rng.uniform(-8, 8, n)
+    c[:, 3] = rng.uniform(-eta_max, eta_max, n)
+    ux = rng.normal(0, 0.8, n)
+    uy = rng.normal(0, 0.8, n)
+    ue = rng.normal(0, 0.3, n)
+    u0 = np.sqrt(1 + ux ** 2 + uy ** 2 + ue ** 2)
+    u = np.stack([u0, ux, uy, ue], axis=1)
+    c[:, 8:12] = u
+    # dsigma_mu (orthonormal, lower index): mostly time-like along u, some space-like
+    D = np.zeros((n, 4))
+    D[:, 0] = area * rng.uniform(0.5, 1.5, n)
+    D[:, 1:] = area * rng.normal(0, 0.25, (n, 3))
+    sl = rng.uniform(0, 1, n) < frac_spacelike
+    D[sl, 1:] = area * rng.normal(0, 1.2, (sl.sum(), 3))
+    D[sl, 0] = area * rng.normal(0, 0.4, sl.sum())
+    c[:, 4] = D[:, 0] / tau
+    c[:, 5] = D[:, 1] / tau
+    c[:, 6] = D[:, 2] / tau
+    c[:, 7] = D[:, 3]
+    T = rng.uniform(0.135, 0.160, n)
+    muB = rng.uniform(0.0, muB_max, n)
+    e, P, nB = hrg_thermo(T, muB, sp)
+    c[:, 12], c[:, 13], c[:, 14], c[:, 15], c[:, 16] = e, T, P, nB, muB
+    epp = e + P
+    if shear:
+        xx, xy, xz, yy, yz = _lrf_shear(rng, n, epp).T
+        PL = np.zeros((n, 4, 4))
+        PL[:, 1, 1], PL[:, 1, 2], PL[:, 1, 3] = xx, xy, xz
+        PL[:, 2, 1], PL[:, 2, 2], PL[:, 2, 3] = xy, yy, yz
+        PL[:, 3, 1], PL[:, 3, 2], PL[:, 3, 3] = xz, yz, -xx - yy
+        B = _boost_to_lab(u)
+        Pm = np.einsum("nab,nbc,ndc->nad", B, PL, B)
+        idx = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+        for k, (a, b) in enumerate(idx):
+            c[:, 19 + k] = Pm[:, a, b]
+    if bulk:
+        c[:, 29] = -rng.uniform(0, 0.15, n) * epp
     return c.astype(np.float32)

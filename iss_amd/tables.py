@@ -72,6 +72,18 @@ def species_arrays(afterburner: int = 1, quantum_statistics: bool = True,
     return sp
 
 
+def bin_tables(table_dir: str = TABLE_DIR) -> dict:
+    """Quadrature grids of the smooth spectra (src/iSS.cpp:151-154): Gauss pT
+    (15 nodes, GeV) and phi (48 nodes) with weights, and the uniform eta_s
+    table (-4..4, step 0.1) with its weights."""
+    d = os.path.join(table_dir, "bin_tables")
+    pt = np.loadtxt(os.path.join(d, "pT_gauss_table.dat"))
+    ph = np.loadtxt(os.path.join(d, "phi_gauss_table.dat"))
+    et = np.loadtxt(os.path.join(d, "eta_uni_table.dat"))
+    return dict(pT=pt[:, 0].copy(), pT_weight=pt[:, 1].copy(), phi=ph[:, 0].copy(), phi_weight=ph[:, 1].copy(),
+                eta=et[:, 0].copy(), eta_weight=et[:, 1].copy())
+
+
 def read_eos(iEOS: int, afterburner: int, table_dir: str = TABLE_DIR) -> dict:
     """HRG EoS rows {e, nB, P, T, muB, muS, muQ} for iEOS 9/91 (1-D in e) or
     12/14 (200 n_B per e)."""

@@ -996,4 +996,148 @@ void tmunu_from_samples(const std::vector<std::vector<Hadron>>& events,
     for (int i = 0; i < 3; i++) Q[i] /= nv;
 }
 
+// EmissionFunctionArray::calculate_dN_pTdpTdphidy (src/emissionfunction.cpp:
+// 625-844), evaluated as the reference does on the lab-frame Milne cells:
+// p.u = pt u^tau - px u^x - py u^y - pz tau u^eta, p.dsigma tau =
+// (pt da0 + px da1 + py da2 + pz da3 / tau) tau, and the shear factor
+// p_mu p_nu pi^{mu nu} from the ten lab components (:759-776).  The
+// delta-f coefficients are FSSW's (the same choice as yields_one_species and
+// sample_from_cell), so that for shear and bulk kind 1 this is the
+// reference's smooth path and otherwise its smooth counterpart of the sampler.
+// mode 0: pt = mT cosh(-eta_k), pz = mT sinh(-eta_k), weight w_k (:232-238,
+// 756-760); mode 1: y - eta_s = y_i - eta_cell, weight 1.
+void smooth_spectra(const FSSWOracle& o, const std::vector<Surf>& lab, int ri, const SpectraOpts& so,
+                    const std::vector<double>& pT, const std::vector<double>& phi,
+                    const std::vector<double>& eta, const std::vector<double>& w,
+                    std::vector<double>& out) {
+    const Particle& p = o.particles[ri];
+    const double mass = p.mass, sign = p.sign, degen = p.gspin;
+    const double prefactor = 1.0 / (8.0 * (M_PI * M_PI * M_PI)) / kHbarC / kHbarC / kHbarC;
+    const int kind = o.cfg.bulk_kind, nk = o.neos_kind();
+    const size_t npT = pT.size(), nphi = phi.size(), ny = eta.size();
+    const size_t nplanes = so.mode == 1 ? ny : 1;
+    out.assign(nplanes * npT * nphi, 0.);
+    for (size_t iy = 0; iy < nplanes; iy++)
+        for (size_t i = 0; i < npT; i++) {
+            const double mT = std::sqrt(mass * mass + pT[i] * pT[i]);
+            for (size_t j = 0; j < nphi; j++) {
+                const double px = pT[i] * std::cos(phi[j]), py = pT[i] * std::sin(phi[j]);
+                double acc = 0.;
+                for (const Surf& s : lab) {
+                    const double T = s.Tdec, tau = s.tau;
+                    const double mu = p.baryon * s.muB + p.strange * s.muS + p.charge * s.muQ;
+                    double c[6] = {0, 0, 0, 0, 0, 0};
+                    double Pi = 0.;
+                    if (nk >= 0) neos_coefficients(o.dft, s.Edec, s.Bn, c);
+                    if (o.cfg.include_bulk == 1) {
+                        if (kind == 21 || kind == 20) {
+                            Pi = s.bulkPi;
+                        } else if (kind == 11) {
+                            Pi = s.bulkPi;
+                            bulk_coefficients_14mom(o.dft, T, s.muB, c);
+                        } else {
+                            Pi = (kind == 0) ? s.bulkPi : s.bulkPi / kHbarC;
+                            bulk_coefficients_T(kind, o.dft, T, c);
+                        }
+                    }
+                    const double Edec = s.Edec, Pdec = s.Pdec;  // double locals (:687-689)
+                    const double dfpref = 1.0 / (2.0 * T * T * (Edec + Pdec));
+                    const size_t nkk = so.mode == 1 ? 1 : ny;
+                    double cell = 0.;
+                    for (size_t k = 0; k < nkk; k++) {
+                        const double yme = so.mode == 1 ? eta[iy] - s.eta : -eta[k];
+                        const double wk = so.mode == 1 ? 1. : w[k];
+                        const double pt = mT * std::cosh(yme), pz = mT * std::sinh(yme);
+                        const double pdotu = pt * s.u0 - px * s.u1 - py * s.u2 - pz * s.u3;
+                        const double f0 = 1. / (std::exp((pdotu - mu) / T) + sign);
+                        const double pdsigma = pt * s.da0 + px * s.da1 + py * s.da2 + pz * s.da3 / tau;
+                        double dfs = 0.;
+                        if (o.cfg.include_shear == 1) {
+                            const double W = pt * pt * s.pi00 - 2.0 * pt * px * s.pi01 - 2.0 * pt * py * s.pi02 -
+                                             2.0 * pt * pz * s.pi03 + px * px * s.pi11 + 2.0 * px * py * s.pi12 +
+                                             2.0 * px * pz * s.pi13 + py * py * s.pi22 + 2.0 * py * pz * s.pi23 +
+                                             pz * pz * s.pi33;
+                            if (nk == 1) dfs = (1. - sign * f0) * W / (2. * c[2]) / (pdotu * T);
+                            else if (nk == 0) dfs = (1. - sign * f0) * W * c[0];
+                            else dfs = (1. - sign * f0) * W * dfpref;
+                        }
+                        double dfb = 0.;
+                        if (o.cfg.include_bulk == 1) {
+                            const double Eo = pdotu / T, mo = mass / T;
+                            if (kind == 1 || kind == 21) {
+                                dfb = -1.0 * (1. - sign * f0) * c[0] * (mo * mo / (3. * Eo) - c[1] * Eo) * Pi;
+                            } else if (kind == 11) {
+                                dfb = (1. - sign * f0) * Pi *
+                                      (c[0] * mass * mass + c[1] * p.baryon * pdotu + c[2] * pdotu * pdotu);
+                            } else if (kind == 20) {
+                                dfb = (1. - sign * f0) * Pi *
+                                      (mass * mass * c[2] +
+                                       pdotu * (p.baryon * c[3] + p.strange * c[4] + p.charge * c[5]) +
+                                       pdotu * pdotu * (c[1] - c[2]));
+                            }
+                        }
+                        double resize = 1.0;
+                        if (so.restrict_deltaf == 1)
+                            resize = std::min(1., so.deltaf_max_ratio / (std::abs(dfs + dfb) + 1e-10));
+                        const double result = prefactor * degen * f0 * pdsigma * tau * (1. + (dfs + dfb) * resize);
+                        if (so.use_pos_dN_only && result < 0.) continue;
+                        cell += result * wk;
+                    }
+                    acc += cell;
+                }
+                out[(iy * npT + i) * nphi + j] = acc;
+            }
+        }
+}
+
+// EmissionFunctionArray::calculate_flows (src/emissionfunction.cpp:875-1023)
+void flows(const std::vector<double>& dN, const std::vector<double>& pT, const std::vector<double>& pTw,
+           const std::vector<double>& phi, const std::vector<double>& phiw, double mass, int to_order,
+           std::vector<double>& diff, std::vector<double>& inte) {
+    const size_t npT = pT.size(), nphi = phi.size();
+    const size_t nc = 3 + 3 * to_order;
+    diff.assign(npT * nc, 0.);
+    inte.assign(6 * (to_order + 1), 0.);
+    std::vector<double> norm(npT, 0.), re(npT * to_order, 0.), im(npT * to_order, 0.);
+    for (size_t i = 0; i < npT; i++) {
+        for (size_t j = 0; j < nphi; j++) {
+            const double dn = dN[i * nphi + j];
+            norm[i] += dn * phiw[j];
+            for (int n = 1; n <= to_order; n++) {
+                re[i * to_order + n - 1] += dn * phiw[j] * std::cos(n * phi[j]);
+                im[i * to_order + n - 1] += dn * phiw[j] * std::sin(n * phi[j]);
+            }
+        }
+        norm[i] = norm[i] + 1e-30;
+        diff[i * nc + 0] = pT[i];
+        diff[i * nc + 1] = std::sqrt(mass * mass + pT[i] * pT[i]) - mass;
+        diff[i * nc + 2] = norm[i] / (2.0 * M_PI);
+        for (int t = 0; t < to_order; t++) {
+            const double a = re[i * to_order + t], b = im[i * to_order + t];
+            diff[i * nc + 3 + 3 * t] = a / norm[i];
+            diff[i * nc + 4 + 3 * t] = b / norm[i];
+            diff[i * nc + 5 + 3 * t] = std::sqrt(a * a + b * b) / norm[i];
+        }
+    }
+    double ni = 0.;
+    std::vector<double> vr(to_order, 0.), vi(to_order, 0.);
+    for (size_t i = 0; i < npT; i++) {
+        ni += norm[i] * pT[i] * pTw[i];
+        for (int t = 0; t < to_order; t++) {
+            vr[t] += re[i * to_order + t] * pT[i] * pTw[i];
+            vi[t] += im[i * to_order + t] * pT[i] * pTw[i];
+        }
+    }
+    inte[0] = 0; inte[1] = ni; inte[2] = 0; inte[3] = 1; inte[4] = 0; inte[5] = 1;
+    for (int t = 0; t < to_order; t++) {
+        double* r = &inte[6 * (t + 1)];
+        r[0] = 1 + t;
+        r[1] = vr[t];
+        r[2] = vi[t];
+        r[3] = vr[t] / ni;
+        r[4] = vi[t] / ni;
+        r[5] = std::sqrt(vr[t] * vr[t] + vi[t] * vi[t]) / ni;
+    }
+}
+
 }  // namespace iss_oracle

@@ -20,6 +20,11 @@
 //                                      draw-level parity UNPINNED (GSL absent)
 //   * full sampler ................... pinned by the reference's T^{mu nu}
 //                                      reconstruction test (1e-3 GeV/fm^3)
+//   * smooth spectra ................. pinned by closed forms (Boltzmann and
+//                                      quantum-statistics K1 series, boosted
+//                                      cell, shear cos 2phi term, bulk K0/K2)
+//   * flows .......................... pinned by Bessel I_n ratios and by
+//                                      pure Fourier inputs
 #pragma once
 
 #include <array>
@@ -203,5 +208,21 @@ class FSSWOracle {
 void tmunu_from_samples(const std::vector<std::vector<Hadron>>& events,
                         const std::vector<Particle>& particles, double volume,
                         double T[16], double Q[3]);
+
+// ------------------------------------------------ smooth spectra and flows
+struct SpectraOpts {
+    int mode = 0;                // 0: eta_s table integral, 1: fixed rapidities
+    int use_pos_dN_only = 0;
+    int restrict_deltaf = 1;
+    double deltaf_max_ratio = 1.0;
+};
+// dN/(pT dpT dphi dy), out[(plane * npT + i) * nphi + j]
+void smooth_spectra(const FSSWOracle& o, const std::vector<Surf>& lab, int real_idx, const SpectraOpts& so,
+                    const std::vector<double>& pT, const std::vector<double>& phi,
+                    const std::vector<double>& eta, const std::vector<double>& w,
+                    std::vector<double>& out);
+void flows(const std::vector<double>& dN, const std::vector<double>& pT, const std::vector<double>& pTw,
+           const std::vector<double>& phi, const std::vector<double>& phiw, double mass, int to_order,
+           std::vector<double>& diff, std::vector<double>& inte);
 
 }  // namespace iss_oracle

@@ -237,4 +237,46 @@ void orc_k1_coeff(void* p, double T, double* out) {
     bulk_coefficients_T(1, static_cast<Handle*>(p)->o.dft, T, out);
 }
 
+// lab cells (34 floats each), regulated without an EoS as the surface reader
+// does, and their rest-frame copy
+long orc_set_cells_lab(void* p, long n, const float* lab34) {
+    Handle* h = static_cast<Handle*>(p);
+    h->lab.resize(n);
+    std::memcpy(h->lab.data(), lab34, sizeof(Surf) * n);
+    regulate_surface(h->lab, nullptr);
+    h->o.cells = transform_to_lrf(h->lab);
+    return static_cast<long>(h->o.cells.size());
+}
+
+// smooth spectra of species (sampling-order indices), out[s][plane][pT][phi]
+void orc_spectra(void* p, const int* species, int ns, int mode, int pos_only, int restrict_df, double ratio,
+                 const double* pT, int npT, const double* phi, int nphi, const double* eta, const double* w,
+                 int ny, double* out) {
+    Handle* h = static_cast<Handle*>(p);
+    SpectraOpts so;
+    so.mode = mode;
+    so.use_pos_dN_only = pos_only;
+    so.restrict_deltaf = restrict_df;
+    so.deltaf_max_ratio = ratio;
+    std::vector<double> vp(pT, pT + npT), vf(phi, phi + nphi), ve(eta, eta + ny), vw;
+    if (w) vw.assign(w, w + ny);
+    else vw.assign(ny, 1.);
+    std::vector<double> r;
+    size_t off = 0;
+    for (int s = 0; s < ns; s++) {
+        smooth_spectra(h->o, h->lab, h->o.order[species[s]], so, vp, vf, ve, vw, r);
+        std::memcpy(out + off, r.data(), sizeof(double) * r.size());
+        off += r.size();
+    }
+}
+
+void orc_flows(const double* dN, const double* pT, const double* pTw, int npT, const double* phi,
+               const double* phiw, int nphi, double mass, int to_order, double* diff, double* inte) {
+    std::vector<double> d(dN, dN + static_cast<size_t>(npT) * nphi), a(pT, pT + npT), aw(pTw, pTw + npT),
+        f(phi, phi + nphi), fw(phiw, phiw + nphi), o1, o2;
+    flows(d, a, aw, f, fw, mass, to_order, o1, o2);
+    std::memcpy(diff, o1.data(), sizeof(double) * o1.size());
+    std::memcpy(inte, o2.data(), sizeof(double) * o2.size());
+}
+
 }  // extern "C"

@@ -50,6 +50,9 @@ def lib():
             "orc_neos_coeff": (None, [v, d, d, v]),
             "orc_k11_coeff": (None, [v, d, d, v]),
             "orc_k1_coeff": (None, [v, d, v]),
+            "orc_set_cells_lab": (L, [v, L, v]),
+            "orc_spectra": (None, [v, v, i, i, i, i, d, v, i, v, i, v, v, i, v]),
+            "orc_flows": (None, [v, v, v, i, v, v, i, d, i, v, v]),
         }
         for k, (r, a) in sig.items():
             f = getattr(o, k)
@@ -132,6 +135,26 @@ class Oracle:
         self.o.orc_surface_tmunu(self.h, _p(T), _p(Q))
         return T, Q
 
+    def set_cells_lab(self, cells):
+        a = np.ascontiguousarray(cells, dtype=np.float32).reshape(-1, 34)
+        self._lab = a
+        return self.o.orc_set_cells_lab(self.h, a.shape[0], _p(a))
+
+    def spectra(self, species, pT, phi, eta, eta_weight=None, mode=0, use_pos_dN_only=0,
+                restrict_deltaf=1, deltaf_max_ratio=1.0):
+        """dN/(pT dpT dphi dy) of species (sampling-order indices) on the lab cells."""
+        sp = np.ascontiguousarray(np.atleast_1d(species), dtype=np.int32)
+        pT = np.ascontiguousarray(pT, dtype=np.float64)
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        eta = np.ascontiguousarray(np.atleast_1d(eta), dtype=np.float64)
+        w = None if eta_weight is None else np.ascontiguousarray(eta_weight, dtype=np.float64)
+        shape = (sp.size, pT.size, phi.size) if mode == 0 else (sp.size, eta.size, pT.size, phi.size)
+        out = np.zeros(shape)
+        self.o.orc_spectra(self.h, _p(sp), sp.size, mode, use_pos_dN_only, restrict_deltaf, deltaf_max_ratio,
+                           _p(pT), pT.size, _p(phi), phi.size, _p(eta), None if w is None else _p(w),
+                           eta.size, _p(out))
+        return out
+
     def psample(self, seed, m, T, mu, sign, n):
         out = np.zeros(n)
         rc = self.o.orc_psample(self.h, seed, m, T, mu, sign, n, _p(out))
@@ -150,6 +173,19 @@ def transform(lab34):
     kept = np.zeros(a.shape[0], dtype=np.int64)
     k = lib().orc_transform(a.shape[0], _p(a), _p(out), _p(kept))
     return out[:k], kept[:k]
+
+
+def flows(dN, pT, pT_weight, phi, phi_weight, mass, to_order=9):
+    dN = np.ascontiguousarray(dN, dtype=np.float64)
+    pT = np.ascontiguousarray(pT, dtype=np.float64)
+    pw = np.ascontiguousarray(pT_weight, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    fw = np.ascontiguousarray(phi_weight, dtype=np.float64)
+    diff = np.zeros((pT.size, 3 + 3 * to_order))
+    inte = np.zeros((to_order + 1, 6))
+    lib().orc_flows(_p(dN), _p(pT), _p(pw), pT.size, _p(phi), _p(fw), phi.size, float(mass), to_order,
+                    _p(diff), _p(inte))
+    return diff, inte
 
 
 def regulate(lab34):
