@@ -4,6 +4,7 @@
 #include "iSS.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
@@ -215,6 +216,7 @@ int iSS::upload_species_and_tables(std::string* err) {
         S[i] = p.strange; Q[i] = p.charge; sg[i] = p.sign;
     }
     species_pid_ = pid;
+    species_index_ = order;
     iss_species_desc sd{n, pid.data(), m.data(), g.data(), B.data(), S.data(), Q.data(), sg.data()};
     int rc = iss_set_species(ctx_, &sd);
     if (rc != ISS_OK) {
@@ -260,8 +262,9 @@ int iSS::upload_species_and_tables(std::string* err) {
 // src/iSS.cpp:130-165 with FSSW::shell (src/FSSW.cpp:344-361): yields,
 // multiplicities and momenta on the GPU, per-event hadron lists, writers.
 int iSS::generate_samples() {
-    if (static_cast<int>(paraRdr_ptr->getVal("MC_sampling")) != 4) {
-        error("only MC_sampling = 4 (FSSW) is provided by this sampler");
+    const int mc = static_cast<int>(paraRdr_ptr->getVal("MC_sampling"));
+    if (mc != 4 && mc != 0) {
+        error("only MC_sampling = 4 (FSSW) and 0 (smooth spectra only) are provided by this sampler");
         return -1;
     }
     if (ensure_context()) return -1;
@@ -272,6 +275,12 @@ int iSS::generate_samples() {
         error(err);
         return rc;
     }
+    // EmissionFunctionArray::shell (src/emissionfunction.cpp:2542-2561)
+    if (static_cast<int>(paraRdr_ptr->getVal("calculate_vn", 0)) == 1) {
+        rc = calculate_spectra_and_flows();
+        if (rc != 0) return rc;
+    }
+    if (mc == 0) return 0;
     const int hydro_mode = static_cast<int>(paraRdr_ptr->getVal("hydro_mode"));
     const double yLB = paraRdr_ptr->getVal("y_LB"), yRB = paraRdr_ptr->getVal("y_RB");
     const int precision = static_cast<int>(paraRdr_ptr->getVal("yield_precision", 64));
@@ -386,6 +395,116 @@ int iSS::generate_samples() {
         iss_io::write_gzip("particle_samples.gz", hadron_list_);
     } else if (static_cast<int>(paraRdr_ptr->getVal("use_binary_format")) == 1) {
         iss_io::write_binary("particle_samples.bin", hadron_list_);
+    }
+    return 0;
+}
+
+namespace {
+// two-column bin table (src/Table.cpp:180-...): node, weight
+bool read_bin_table(const std::string& file, std::vector<double>& x, std::vector<double>& w) {
+    x.clear();
+    w.clear();
+    std::ifstream f(file);
+    if (!f.good()) return false;
+    double a, b;
+    while (f >> a >> b) {
+        x.push_back(a);
+        w.push_back(b);
+    }
+    return !x.empty();
+}
+
+// Table::printTable (src/Table.cpp:166-176): rows of "%15.8e  " fields
+void print_rows(std::ostream& os, const double* v, int rows, int cols, bool transpose) {
+    for (int r = 0; r < rows; r++) {
+        for (int c = 0; c < cols; c++)
+            os << std::scientific << std::setw(15) << std::setprecision(8)
+               << (transpose ? v[static_cast<size_t>(c) * rows + r] : v[static_cast<size_t>(r) * cols + c]) << "  ";
+        os << std::endl;
+    }
+}
+}  // namespace
+
+// EmissionFunctionArray::calculate_dN_pTdpTdphidy_and_flows_4all
+// (src/emissionfunction.cpp:1129-1218) and its historic-format variant
+// (:1031-1120): spectra of all chosen species in one GPU pass over every lab
+// cell, flows per species on the host, the reference's files and layout.
+int iSS::calculate_spectra_and_flows() {
+    std::vector<double> pT, pTw, phi, phiw, eta, etaw;
+    const std::string d = table_path_ + "/bin_tables/";
+    if (!read_bin_table(d + "pT_gauss_table.dat", pT, pTw) || !read_bin_table(d + "phi_gauss_table.dat", phi, phiw) ||
+        !read_bin_table(d + "eta_uni_table.dat", eta, etaw)) {
+        error("Table::loadTableFromFile error: the bin tables in " + d + " cannot be opened.");
+        return -1;
+    }
+    const int to_order = static_cast<int>(paraRdr_ptr->getVal("calculate_vn_to_order", 9));
+    if (static_cast<int>(paraRdr_ptr->getVal("calculate_dN_dphi", 0)) == 1)
+        warn("calculate_dN_dphi = 1 is not supported by this sampler; dN/dphi is not written");
+    if (static_cast<int>(paraRdr_ptr->getVal("include_deltaf_diffusion", 0)) == 1)
+        warn("include_deltaf_diffusion = 1 is not supported by this sampler; spectra without diffusion delta f");
+    const int ns = static_cast<int>(species_pid_.size());
+    const int npT = static_cast<int>(pT.size()), nphi = static_cast<int>(phi.size());
+    std::vector<int32_t> idx(ns);
+    for (int i = 0; i < ns; i++) idx[i] = i;
+    iss_spectra_opts so{};
+    so.precision = static_cast<int>(paraRdr_ptr->getVal("spectra_precision", 64));
+    so.mode = 0;
+    so.all_cells = 1;
+    so.use_pos_dN_only = static_cast<int>(paraRdr_ptr->getVal("use_pos_dN_only", 0));
+    so.restrict_deltaf = static_cast<int>(paraRdr_ptr->getVal("restrict_deltaf", 1));
+    so.deltaf_max_ratio = paraRdr_ptr->getVal("deltaf_max_ratio", 1.0);
+    std::vector<double> dN(static_cast<size_t>(ns) * npT * nphi);
+    int rc = iss_spectra(ctx_, &so, idx.data(), ns, pT.data(), npT, phi.data(), nphi, eta.data(), etaw.data(),
+                         static_cast<int>(eta.size()), dN.data());
+    if (rc != ISS_OK) {
+        error(std::string("iss_spectra: ") + iss_last_error(ctx_));
+        return rc;
+    }
+    info("smooth spectra of " + std::to_string(ns) + " species on " + std::to_string(npT) + " x " +
+         std::to_string(nphi) + " (pT, phi) x " + std::to_string(eta.size()) + " eta_s: " +
+         std::to_string(iss_spectra_ms(ctx_)) + " ms on the GPU");
+    // species of each particle-list entry (-1: not chosen)
+    std::vector<int> of_particle(particle_.size(), -1);
+    for (int i = 0; i < ns; i++)
+        if (of_particle[species_index_[i]] < 0) of_particle[species_index_[i]] = i;
+    const int nc = 3 + 3 * to_order;
+    std::vector<double> diff(static_cast<size_t>(npT) * nc), inte(6 * static_cast<size_t>(to_order + 1));
+    std::vector<double> zero(static_cast<size_t>(npT) * nphi, 0.);
+    auto flows_of = [&](const double* spec, double mass) {
+        return iss_flows(spec, pT.data(), pTw.data(), npT, phi.data(), phiw.data(), nphi, mass, to_order,
+                         diff.data(), inte.data());
+    };
+    const std::string dn_file = path_ + "/dN_pTdpTdphidy.dat";
+    std::remove(dn_file.c_str());
+    std::ofstream odn(dn_file);
+    // dN_pTdpTdphidy tables: one row per phi, one column per pT
+    if (static_cast<int>(paraRdr_ptr->getVal("use_historic_flow_output_format", 0)) == 1) {
+        const std::string f1 = path_ + "/v2data.dat", f2 = path_ + "/v2data-inte.dat";
+        std::ofstream o1(f1), o2(f2);
+        for (size_t n = 0; n < particle_.size(); n++) {
+            const int s = of_particle[n];
+            const double* spec = s >= 0 ? &dN[static_cast<size_t>(s) * npT * nphi] : zero.data();
+            print_rows(odn, spec, nphi, npT, true);
+            o1 << "# Output for particle: " << particle_[n].name << std::endl;
+            o1 << "#                 " << particle_[n].monval << std::endl;
+            o2 << "# For: " << particle_[n].name << std::endl;
+            if (flows_of(spec, particle_[n].mass) != ISS_OK) return -1;
+            print_rows(o1, diff.data(), npT, nc, false);
+            print_rows(o2, inte.data(), to_order + 1, 6, false);
+        }
+        return 0;
+    }
+    for (int s = 0; s < ns; s++) {
+        const particle_info& p = particle_[species_index_[s]];
+        if (flows_of(&dN[static_cast<size_t>(s) * npT * nphi], p.mass) != ISS_OK) return -1;
+        std::ofstream o1(path_ + "/thermal_" + std::to_string(p.monval) + "_vndata.dat");
+        print_rows(o1, diff.data(), npT, nc, false);
+        std::ofstream o2(path_ + "/thermal_" + std::to_string(p.monval) + "_integrated__vndata.dat");
+        print_rows(o2, inte.data(), to_order + 1, 6, false);
+    }
+    for (size_t n = 0; n < particle_.size(); n++) {
+        const int s = of_particle[n];
+        print_rows(odn, s >= 0 ? &dN[static_cast<size_t>(s) * npT * nphi] : zero.data(), nphi, npT, true);
     }
     return 0;
 }

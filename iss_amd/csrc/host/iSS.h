@@ -39,6 +39,7 @@ class iSS {
     std::vector<std::vector<iSS_Hadron>*> hadron_list_;
     std::vector<double> species_dNdy_;  // FP64 per-species totals, sampling order
     std::vector<int> species_pid_;
+    std::vector<int> species_index_;    // sampling order -> index in particle_
 
     int ensure_context();
     int upload_species_and_tables(std::string* err);
@@ -64,6 +65,9 @@ class iSS {
     int shell();
     int read_in_FO_surface();
     int generate_samples();
+    // smooth dN/(pT dpT dphi dy) and v_n of every chosen species on the
+    // bin_tables grids, written as the reference's calculate_vn output
+    int calculate_spectra_and_flows();
 
     int get_number_of_sampled_events() { return static_cast<int>(hadron_list_.size()); }
     int get_number_of_particles(int iev) { return static_cast<int>(hadron_list_[iev]->size()); }

@@ -114,3 +114,63 @@ def test_dropin_learned_proposal(tmp_path, name):
     res = np.mean(np.abs(data[:, 2]))
     print(name, "learned mean |diff|", res)
     assert res < 1e-3
+
+
+def _table_blocks(path, rows):
+    a = np.loadtxt(path)
+    return a.reshape(-1, rows, a.shape[1])
+
+
+@pytest.mark.parametrize("historic", [0, 1])
+def test_dropin_calculate_vn_matches_oracle(tmp_path, historic):
+    """MC_sampling = 0, calculate_vn = 1: the reference's smooth-spectra
+    outputs (src/emissionfunction.cpp:1129-1218, historic :1031-1120) from
+    one GPU pass, against the oracle's lab-frame integral and flows on the
+    same regulated cell.  Bar: the float rest-frame record (5e-5 of the row
+    maximum; the files hold 9 significant digits)."""
+    order = 4
+    _run(tmp_path, "Viscous1", ["MC_sampling=0", "calculate_vn=1", "perform_checks=0",
+                                f"calculate_vn_to_order={order}",
+                                f"use_historic_flow_output_format={historic}"])
+    cfg = ob.case_config("Viscous1")
+    orc = ob.Oracle(afterburner=cfg["afterburner"], shear=cfg["shear"], bulk=cfg["bulk"],
+                    bulk_kind=cfg["bulk_kind"])
+    orc.read_surface(cfg["surface"], bulk=1, rhob=cfg["rhob"], eos_file=cfg["eos_file"], iEOS=cfg["iEOS"])
+    from iss_amd import tables
+    b = tables.bin_tables()
+    sp = orc.species()
+    parts = orc.particles()
+    work = tmp_path / "tests"
+    dn = _table_blocks(work / "dN_pTdpTdphidy.dat", b["phi"].size)
+    assert dn.shape == (parts["pid"].size, b["phi"].size, b["pT"].size)
+    checked = 0
+    for pid in (211, -211, 321, 2212, -2212, 3334):
+        s = int(np.nonzero(sp["pid"] == pid)[0][0])
+        ref = orc.spectra(s, b["pT"], b["phi"], b["eta"], b["eta_weight"])[0]
+        n = int(np.nonzero(parts["pid"] == pid)[0][0])
+        got = dn[n].T
+        rowmax = np.abs(ref).max(axis=1, keepdims=True)
+        assert np.max(np.abs(got - ref) / rowmax) < 5e-5, pid
+        d_ref, i_ref = ob.flows(ref, b["pT"], b["pT_weight"], b["phi"], b["phi_weight"], sp["mass"][s], order)
+        if historic:
+            continue
+        d_got = np.loadtxt(work / f"thermal_{pid}_vndata.dat")
+        i_got = np.loadtxt(work / f"thermal_{pid}_integrated__vndata.dat")
+        assert d_got.shape == (b["pT"].size, 3 + 3 * order) and i_got.shape == (order + 1, 6)
+        np.testing.assert_allclose(d_got[:, :2], d_ref[:, :2], rtol=1e-7)
+        np.testing.assert_allclose(d_got[:, 2], d_ref[:, 2], rtol=5e-5)
+        assert np.max(np.abs(d_got[:, 3:] - d_ref[:, 3:])) < 1e-4
+        np.testing.assert_allclose(i_got[:, 3:], i_ref[:, 3:], atol=1e-5)
+        checked += 1
+    if historic:
+        txt = (work / "v2data.dat").read_text()
+        assert txt.count("# Output for particle:") == parts["pid"].size
+        inte = (work / "v2data-inte.dat").read_text()
+        assert inte.count("# For:") == parts["pid"].size
+    else:
+        assert checked == 6
+    # non-chosen list entries are zero tables
+    chosen = set(sp["pid"].tolist())
+    for n in range(parts["pid"].size):
+        if parts["pid"][n] not in chosen:
+            assert not dn[n].any()
