@@ -254,6 +254,41 @@ int iss_spectra(iss_ctx* ctx, const iss_spectra_opts* opts, const int32_t* speci
 /* device time of the last iss_spectra, ms (record + integral + reduction) */
 double iss_spectra_ms(const iss_ctx* ctx);
 
+/* ---- batched surfaces (no counterpart in the reference, which runs one
+ * surface per iSS instance, src/iSS.cpp:130-168).  Many rest-frame surfaces
+ * of uneven size are held as one record; surface k owns the next n_cells[k]
+ * cells.  iss_sample_batch samples n_events[k] events of every surface k in
+ * rounds of at most events_per_round events per surface; one round is a
+ * fixed sequence of kernels (counts, scan, per-surface event multinomial,
+ * set-up, trials) captured once in a hipGraph and replayed, every size read
+ * on the device.  Each surface's output follows the one-surface law.  The
+ * hadron list (iss_get_hadrons) is round-major, then surface, then event:
+ * iss_get_batch_layout gives segments[(r S + k) 2 + {0, 1}] = (first hadron,
+ * count) of round r, surface k, and event_sizes[(r S + k) R + e] the size of
+ * its e-th event (event r R + e of surface k; 0 beyond the surface's
+ * events).  iss_compute_yields and iss_sample treat the record as one
+ * surface. ---- */
+typedef struct {
+    uint64_t seed;
+    int32_t hydro_mode;          /* 1 boost invariant (y in [y_LB, y_RB]), 2 3+1D */
+    double y_LB, y_RB;
+    int32_t events_per_round;    /* R; 0: the autotuned value (default 16) */
+    int32_t max_trials;
+    int32_t precision;           /* yields: 64 or 32 */
+    int32_t use_graph;           /* 1: one captured round replayed; 0: launched per round */
+} iss_batch_opts;
+
+int iss_set_surfaces_lrf(iss_ctx* ctx, int32_t n_surfaces, const int64_t* n_cells, const float* cells28);
+int iss_num_surfaces(const iss_ctx* ctx);
+int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* opts, const int32_t* n_events, int64_t* n_hadrons);
+int iss_batch_rounds(const iss_ctx* ctx);
+int iss_batch_events_per_round(const iss_ctx* ctx);
+double iss_batch_ms(const iss_ctx* ctx);   /* device time of the last batch (all rounds) */
+int iss_get_batch_layout(iss_ctx* ctx, int64_t* segments, int64_t* event_sizes);
+/* times whole batches for R in {4, 8, 16, 32, 64}, keeps the fastest as the
+ * default events_per_round; report: refill_min = R chosen, ms_* batch times */
+int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* opts, const int32_t* n_events, iss_tune_report* report);
+
 /* ---- flows: EmissionFunctionArray::calculate_flows
  * (src/emissionfunction.cpp:875-1023) on one spectrum dN[n_pT][n_phi].
  * vn_diff [n_pT][3 + 3 to_order]: pT, mT - m, dN/(pT dpT dy) / (2 pi)... as the

@@ -105,6 +105,12 @@ class TuneReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class _BatchOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("hydro_mode", C.c_int32), ("y_LB", C.c_double), ("y_RB", C.c_double),
+                ("events_per_round", C.c_int32), ("max_trials", C.c_int32), ("precision", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
 class _SpectraOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("mode", C.c_int32), ("all_cells", C.c_int32),
                 ("use_pos_dN_only", C.c_int32), ("restrict_deltaf", C.c_int32), ("deltaf_max_ratio", C.c_double)]
@@ -143,6 +149,14 @@ _SIGNATURES = {
                                C.c_void_p, C.c_void_p, C.c_void_p]),
     "iss_band_diag": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
+    "iss_set_surfaces_lrf": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "iss_num_surfaces": (C.c_int, [C.c_void_p]),
+    "iss_sample_batch": (C.c_int, [C.c_void_p, C.POINTER(_BatchOpts), C.c_void_p, C.POINTER(C.c_int64)]),
+    "iss_batch_rounds": (C.c_int, [C.c_void_p]),
+    "iss_batch_events_per_round": (C.c_int, [C.c_void_p]),
+    "iss_batch_ms": (C.c_double, [C.c_void_p]),
+    "iss_get_batch_layout": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "iss_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(_BatchOpts), C.c_void_p, C.POINTER(TuneReport)]),
     "iss_spectra": (C.c_int, [C.c_void_p, C.POINTER(_SpectraOpts), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                               C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "iss_spectra_ms": (C.c_double, [C.c_void_p]),
@@ -347,6 +361,55 @@ class Context:
         self._check(self.lib.iss_band_diag(self.h, cell.size, _ptr(cell), _ptr(species), _ptr(rho),
                                            _ptr(tab), _ptr(par)), "iss_band_diag")
         return tab, par
+
+    # ------------------------------------------------------ batched surfaces
+    def set_surfaces_lrf(self, surfaces):
+        """surfaces: list of [n_k, 28] rest-frame cell arrays, held as one record"""
+        n = np.array([np.asarray(c).reshape(-1, 28).shape[0] for c in surfaces], dtype=np.int64)
+        a = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.float32).reshape(-1, 28) for c in surfaces]))
+        self._check(self.lib.iss_set_surfaces_lrf(self.h, n.size, _ptr(n), _ptr(a)), "iss_set_surfaces_lrf")
+        self.n_surfaces = n.size
+
+    def _batch_opts(self, seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph):
+        return _BatchOpts(seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph)
+
+    def sample_batch(self, n_events, seed=1, hydro_mode=2, y_LB=-5.0, y_RB=5.0, events_per_round=0,
+                     max_trials=5000, precision=32, use_graph=1):
+        ev = np.ascontiguousarray(n_events, dtype=np.int32)
+        o = self._batch_opts(seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph)
+        n = C.c_int64()
+        self._check(self.lib.iss_sample_batch(self.h, C.byref(o), _ptr(ev), C.byref(n)), "iss_sample_batch")
+        self._batch_events = ev.copy()
+        return n.value
+
+    def batch_ms(self):
+        return self.lib.iss_batch_ms(self.h)
+
+    def batch_layout(self):
+        """(segments [rounds, S, 2], event_sizes [rounds, S, R]) of the last batch"""
+        r = self.lib.iss_batch_rounds(self.h)
+        R = self.lib.iss_batch_events_per_round(self.h)
+        S = self._batch_events.size
+        seg = np.zeros((max(r, 0), S, 2), dtype=np.int64)
+        ev = np.zeros((max(r, 0), S, max(R, 1)), dtype=np.int64)
+        if r > 0:
+            self._check(self.lib.iss_get_batch_layout(self.h, _ptr(seg), _ptr(ev)), "iss_get_batch_layout")
+        return seg, ev
+
+    def batch_surface(self, k, hadrons=None, layout=None):
+        """hadrons of surface k in event order and its event sizes"""
+        h = self.hadrons() if hadrons is None else hadrons
+        seg, ev = self.batch_layout() if layout is None else layout
+        parts = [h[o:o + c] for o, c in seg[:, k]]
+        sizes = ev[:, k, :].ravel()[: int(self._batch_events[k])]
+        return (np.concatenate(parts) if parts else h[:0]), sizes
+
+    def autotune_batch(self, n_events, seed=1, hydro_mode=2, precision=32):
+        ev = np.ascontiguousarray(n_events, dtype=np.int32)
+        o = self._batch_opts(seed, hydro_mode, -5.0, 5.0, 0, 5000, precision, 1)
+        r = TuneReport()
+        self._check(self.lib.iss_autotune_batch(self.h, C.byref(o), _ptr(ev), C.byref(r)), "iss_autotune_batch")
+        return r.as_dict()
 
     # ------------------------------------------------------- smooth spectra
     def spectra(self, species, pT, phi, eta, eta_weight=None, *, mode=0, precision=64, all_cells=1,

@@ -40,6 +40,7 @@ using namespace issdev;
 #include "sample_kernels.h"
 #include "learned_kernels.h"
 #include "spectra_kernels.h"
+#include "batch_kernels.h"
 
 // =================================================================== host
 
@@ -144,6 +145,20 @@ struct iss_ctx {
     // launch shape of the trial kernels per proposal (autotuner)
     int refill_min[2] = {kRefillMin, kRefillMin};
     int blocks_per_cu[2] = {0, 0};  // 0: occupancy limit
+    // batched surfaces
+    int S = 0;                       // surfaces of the resident record (0: one surface)
+    std::vector<int64_t> h_surf_off;
+    DBuf<int64_t> d_surf_off;
+    DBuf<int32_t> d_bevents;
+    DBuf<uint32_t> d_bround, d_bevcount;
+    DBuf<unsigned long long> d_bbase;
+    DBuf<Permutation> d_perms;
+    DBuf<int64_t> d_seg;
+    DBuf<double> d_lam;
+    int batch_rounds = 0, batch_R = 0, batch_S = 0;
+    std::vector<int32_t> h_bevents;
+    int batch_R_tuned = 16;
+    double batch_ms = 0., batch_ms_graph_launch = 0.;
     // smooth spectra
     DBuf<char> d_srec, d_sgrid;
     DBuf<double> d_spart, d_sout;
@@ -594,7 +609,7 @@ int iss_ctx_create(int device, iss_ctx** out) {
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
-    if (ctx->d_stats.ensure(6) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
+    if (ctx->d_stats.ensure(8) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
         delete ctx;
         return ISS_ERR_NOMEM;
     }
@@ -620,6 +635,8 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_i32b.release(); ctx->d_f32a.release(); ctx->d_f32b.release(); ctx->d_f32c.release();
     ctx->d_cells_neg.release(); ctx->d_coef_neg.release(); ctx->d_srec.release(); ctx->d_sgrid.release();
     ctx->d_spart.release(); ctx->d_sout.release(); ctx->d_sspec.release();
+    ctx->d_surf_off.release(); ctx->d_bevents.release(); ctx->d_bround.release(); ctx->d_bevcount.release();
+    ctx->d_bbase.release(); ctx->d_perms.release(); ctx->d_seg.release(); ctx->d_lam.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -731,6 +748,7 @@ int iss_set_surface_lab(iss_ctx* ctx, int64_t n, const float* cells34, const iss
     ctx->yields_valid = false;
     ctx->Nc = 0;
     ctx->Nneg = 0;
+    ctx->S = 0;
     if (n == 0) return ISS_OK;
     DBuf<float> lab;
     HIPCHK(lab.ensure(34 * n));
@@ -771,6 +789,7 @@ int iss_set_surface_lrf(iss_ctx* ctx, int64_t n, const float* cells28) {
     ctx->yields_valid = false;
     ctx->Nc = 0;
     ctx->Nneg = 0;
+    ctx->S = 0;
     if (n == 0) return ISS_OK;
     HIPCHK(ctx->d_tmp.ensure(kCellFields * n));
     HIPCHK(ctx->d_cells.ensure(kCellFields * n));
@@ -856,7 +875,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipMemcpyAsync(&K, ctx->d_off.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx->d_out.ensure(std::max<int64_t>(K, 1)));
-    SampleArgs a;
+    SampleArgs a{};
     a.cells = ctx->d_cells.p;
     a.coef = ctx->d_coef.p;
     a.Nc = Nc;
@@ -1304,6 +1323,256 @@ int iss_flows(const double* dN, const double* pT, const double* pT_weight, int32
         row[3] = re / ni;
         row[4] = im / ni;
         row[5] = std::sqrt(re * re + im * im) / ni;
+    }
+    return ISS_OK;
+}
+
+// ------------------------------------------------------------ batched surfaces
+
+int iss_set_surfaces_lrf(iss_ctx* ctx, int32_t n_surfaces, const int64_t* n_cells, const float* cells28) {
+    if (!ctx || n_surfaces <= 0 || !n_cells) return ISS_ERR_ARG;
+    std::vector<int64_t> off(n_surfaces + 1, 0);
+    for (int k = 0; k < n_surfaces; k++) {
+        if (n_cells[k] < 0) return set_err(ctx, ISS_ERR_ARG, "negative cell count");
+        off[k + 1] = off[k] + n_cells[k];
+    }
+    if (off[n_surfaces] > (int64_t{1} << 32) - 2) return set_err(ctx, ISS_ERR_RANGE, "more than 2^32 cells");
+    int rc = iss_set_surface_lrf(ctx, off[n_surfaces], cells28);
+    if (rc) return rc;
+    HIPCHK(ctx->d_surf_off.ensure(n_surfaces + 1));
+    HIPCHK(hipMemcpy(ctx->d_surf_off.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+    ctx->h_surf_off = off;
+    ctx->S = n_surfaces;
+    return ISS_OK;
+}
+
+int iss_num_surfaces(const iss_ctx* ctx) { return ctx ? ctx->S : -1; }
+
+int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_events, int64_t* n_hadrons) {
+    if (!ctx || !o || !n_events) return ISS_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->S <= 0) return set_err(ctx, ISS_ERR_STATE, "no batched surfaces (iss_set_surfaces_lrf)");
+    if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
+    const int S = ctx->S;
+    int maxE = 0;
+    for (int k = 0; k < S; k++) {
+        if (n_events[k] < 0) return set_err(ctx, ISS_ERR_ARG, "negative event count");
+        maxE = std::max(maxE, n_events[k]);
+    }
+    const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned;
+    if (R > 4096) return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most 4096");
+    if (!ctx->yields_valid || (o->precision != ctx->yield_precision && (o->precision == 32 || o->precision == 64))) {
+        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    const int64_t N = ctx->Nc;
+    const int rounds = (maxE + R - 1) / R;
+    const bool binv = o->hydro_mode != 2;
+    const double ymult = binv ? (o->y_RB - o->y_LB) : 1.0;
+    // expected hadrons per round -> reserved output and set-up records
+    HIPCHK(ctx->d_lam.ensure(S));
+    HIPCHK(ctx->d_bevents.ensure(S));
+    HIPCHK(hipMemcpyAsync(ctx->d_bevents.p, n_events, sizeof(int32_t) * S, hipMemcpyHostToDevice, ctx->stream));
+    k_batch_lambda<<<S, 256, 0, ctx->stream>>>(ctx->d_surf_off.p, ctx->d_ctot.p, ctx->d_lam.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> lam(S);
+    HIPCHK(hipMemcpyAsync(lam.data(), ctx->d_lam.p, sizeof(double) * S, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double total = 0., maxr = 0.;
+    for (int r = 0; r < rounds; r++) {
+        double kr = 0.;
+        for (int k = 0; k < S; k++) {
+            const int64_t rem = static_cast<int64_t>(n_events[k]) - static_cast<int64_t>(r) * R;
+            const int n = rem <= 0 ? 0 : static_cast<int>(std::min<int64_t>(rem, R));
+            kr += n * ymult * lam[k];
+        }
+        total += kr;
+        maxr = std::max(maxr, kr);
+    }
+    const int64_t out_cap = static_cast<int64_t>(total + 10. * std::sqrt(total) + 4096.);
+    const int64_t setup_cap = static_cast<int64_t>(maxr + 10. * std::sqrt(maxr) + 4096.);
+    int D = 0;
+    while ((1ll << D) < R) D++;
+    HIPCHK(ctx->d_out.ensure(out_cap));
+    HIPCHK(ctx->d_setup.ensure(setup_cap));
+    HIPCHK(ctx->d_counts.ensure(std::max<int64_t>(N, 1)));
+    HIPCHK(ctx->d_off.ensure(N + 1));
+    HIPCHK(ctx->d_perms.ensure(S));
+    HIPCHK(ctx->d_seg.ensure(2 * static_cast<size_t>(std::max(rounds, 1)) * S));
+    HIPCHK(ctx->d_bevcount.ensure(static_cast<size_t>(std::max(rounds, 1)) * S * R));
+    HIPCHK(ctx->d_tree0.ensure(static_cast<size_t>(S) << D));
+    HIPCHK(ctx->d_tree1.ensure(static_cast<size_t>(S) << D));
+    HIPCHK(ctx->d_scan.ensure(std::max<int64_t>(1, (N + kScanTile - 1) / kScanTile) + 1));
+    HIPCHK(ctx->d_bround.ensure(1));
+    HIPCHK(ctx->d_bbase.ensure(1));
+    HIPCHK(hipMemsetAsync(ctx->d_bround.p, 0, sizeof(uint32_t), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_bbase.p, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    BatchDev b{ctx->d_surf_off.p, ctx->d_bevents.p, S, R, ctx->d_bround.p, ctx->d_bbase.p, ymult};
+    SampleArgs a = base_args(ctx);
+    a.cdf = ctx->d_cdf.p;
+    a.ctot = ctx->d_ctot.p;
+    a.off = ctx->d_off.p;
+    a.K = 0;
+    a.Kdev = reinterpret_cast<const unsigned long long*>(ctx->d_off.p + N);
+    a.seed = o->seed;
+    a.chunk = 0;
+    a.chunk_dev = ctx->d_bround.p;
+    a.surf_off = ctx->d_surf_off.p;
+    a.S = S;
+    a.perms = ctx->d_perms.p;
+    a.out_base = ctx->d_bbase.p;
+    a.out_cap = static_cast<uint64_t>(out_cap);
+    a.setup_cap = setup_cap;
+    a.boost_invariant = binv ? 1 : 0;
+    a.yLB = static_cast<float>(o->y_LB);
+    a.yRB = static_cast<float>(o->y_RB);
+    a.max_trials = o->max_trials;
+    a.refill_min = ctx->refill_min[0];
+    a.out = ctx->d_out.p;
+    a.stats = ctx->d_stats.p;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (ctx->blocks_per_cu[0] > 0) per_cu = std::min(ctx->blocks_per_cu[0], per_cu);
+    const int trial_blocks = ctx->num_cu * per_cu;
+    const int setup_blocks = ctx->num_cu * 8;
+    const int64_t ntiles = std::max<int64_t>(1, (N + kScanTile - 1) / kScanTile);
+    // one round: only kernel launches, every size read on the device
+    auto round = [&]() {
+        if (N > 0) k_batch_counts<<<(N + 255) / 256, 256, 0, ctx->stream>>>(b, ctx->d_ctot.p, N, o->seed, ctx->d_counts.p);
+        k_scan_tiles<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_counts.p, N, ctx->d_scan.p);
+        k_scan_top<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan.p, ntiles);
+        k_scan_apply<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_counts.p, N, ctx->d_scan.p, ctx->d_off.p);
+        k_batch_roots<<<(S + 63) / 64, 64, 0, ctx->stream>>>(b, ctx->d_off.p, o->seed, ctx->d_tree0.p, ctx->d_perms.p,
+                                                             ctx->d_seg.p);
+        int64_t* cur = ctx->d_tree0.p;
+        int64_t* nxt = ctx->d_tree1.p;
+        for (int d = 0; d < D; d++) {
+            const int64_t nodes = static_cast<int64_t>(S) << d;
+            k_batch_multi_level<<<(nodes + 255) / 256, 256, 0, ctx->stream>>>(b, cur, nxt, d, o->seed);
+            std::swap(cur, nxt);
+        }
+        k_batch_multi_leaves<<<((static_cast<int64_t>(S) << D) + 255) / 256, 256, 0, ctx->stream>>>(
+            b, cur, D, ctx->d_bevcount.p);
+        k_setup<<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
+        k_trials<<<trial_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+        k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);
+    };
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (rounds > 0) {
+        if (o->use_graph) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            round();
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+            if (le != hipSuccess || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return set_err(ctx, ISS_ERR_HIP, std::string("round capture: ") +
+                                                     hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return set_err(ctx, ISS_ERR_HIP, std::string("round graph: ") + hipGetErrorString(ie));
+            }
+            for (int r = 0; r < rounds && ie == hipSuccess; r++) ie = hipGraphLaunch(exec, ctx->stream);
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return set_err(ctx, ISS_ERR_HIP, std::string("round replay: ") + hipGetErrorString(ie));
+        } else {
+            for (int r = 0; r < rounds; r++) round();
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    unsigned long long st[8];
+    unsigned long long K = 0;
+    HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&K, ctx->d_bbase.p, sizeof(K), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->batch_ms = ms;
+    ctx->batch_rounds = rounds;
+    ctx->batch_R = R;
+    ctx->batch_S = S;
+    ctx->h_bevents.assign(n_events, n_events + S);
+    ctx->K = static_cast<int64_t>(K);
+    ctx->E = 0;
+    ctx->last_plain = false;
+    ctx->stats.n_hadrons = ctx->K;
+    ctx->stats.n_trials = static_cast<int64_t>(st[0]);
+    ctx->stats.n_inner = static_cast<int64_t>(st[1]);
+    ctx->stats.n_failed = static_cast<int64_t>(st[2]);
+    ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
+    ctx->stats.n_fallback = 0;
+    ctx->stats.n_violations = 0;
+    ctx->stats.ms_sample = ms;
+    ctx->stats.ms_sample_kernel = ms;
+    if (n_hadrons) *n_hadrons = ctx->K;
+    if (st[6] || st[7] || ctx->K > out_cap)
+        return set_err(ctx, ISS_ERR_RANGE, "batched sampling: a round exceeded the reserved output (10 sigma)");
+    if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
+    return ISS_OK;
+}
+
+int iss_batch_rounds(const iss_ctx* ctx) { return ctx ? ctx->batch_rounds : -1; }
+int iss_batch_events_per_round(const iss_ctx* ctx) { return ctx ? ctx->batch_R : -1; }
+double iss_batch_ms(const iss_ctx* ctx) { return ctx ? ctx->batch_ms : 0.; }
+
+int iss_get_batch_layout(iss_ctx* ctx, int64_t* segments, int64_t* event_sizes) {
+    if (!ctx) return ISS_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    const int S = ctx->batch_S, R = ctx->batch_R, rounds = ctx->batch_rounds;
+    if (rounds == 0) return ISS_OK;
+    const size_t nseg = static_cast<size_t>(rounds) * S;
+    if (segments)
+        HIPCHK(hipMemcpy(segments, ctx->d_seg.p, sizeof(int64_t) * 2 * nseg, hipMemcpyDeviceToHost));
+    if (event_sizes) {
+        std::vector<uint32_t> ev(nseg * R);
+        HIPCHK(hipMemcpy(ev.data(), ctx->d_bevcount.p, sizeof(uint32_t) * ev.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < rounds; r++)
+            for (int k = 0; k < S; k++) {
+                const int64_t rem = static_cast<int64_t>(ctx->h_bevents[k]) - static_cast<int64_t>(r) * R;
+                const int n = rem <= 0 ? 0 : static_cast<int>(std::min<int64_t>(rem, R));
+                for (int e = 0; e < R; e++) {
+                    const size_t i = (static_cast<size_t>(r) * S + k) * R + e;
+                    event_sizes[i] = e < n ? static_cast<int64_t>(ev[i]) : 0;
+                }
+            }
+    }
+    return ISS_OK;
+}
+
+int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_events, iss_tune_report* rep) {
+    if (!ctx || !o || !n_events) return ISS_ERR_ARG;
+    const int cand[] = {4, 8, 16, 32, 64};
+    iss_batch_opts bo = *o;
+    double best = 1e30, ms_def = 0.;
+    int best_R = 16, tried = 0;
+    for (int R : cand) {
+        bo.events_per_round = R;
+        double t = 1e30;
+        for (int rep_i = 0; rep_i < 2; rep_i++) {
+            int64_t n = 0;
+            int rc = iss_sample_batch(ctx, &bo, n_events, &n);
+            if (rc) return rc;
+            t = std::min(t, ctx->batch_ms);
+        }
+        tried++;
+        if (R == 16) ms_def = t;
+        if (t < best) { best = t; best_R = R; }
+    }
+    ctx->batch_R_tuned = best_R;
+    if (rep) {
+        rep->refill_min = best_R;   // events per round
+        rep->blocks_per_cu = ctx->blocks_per_cu[0];
+        rep->ms_best = best;
+        rep->ms_default = ms_def;
+        rep->n_tried = tried;
     }
     return ISS_OK;
 }

@@ -97,6 +97,16 @@ struct SampleArgs {
     int refill_min;          // idle lanes that trigger a refill in the trial kernels
     void* out;               // iss_hadron[K]
     unsigned long long* stats;  // trials, inner, failed, range errors
+    // batched surfaces (all null / zero for one surface): the pass (round)
+    // index and the hadrons written by earlier rounds live on the device, so
+    // that one captured round can be replayed
+    const uint32_t* chunk_dev;         // if set, overrides chunk
+    const int64_t* surf_off;           // [S + 1] cell offsets of the surfaces
+    int S;
+    const Permutation* perms;          // [S] per-surface permutation of the round
+    const unsigned long long* out_base;  // hadrons of earlier rounds
+    uint64_t out_cap;                  // hadron slots in out (0: unchecked)
+    int64_t setup_cap;                 // set-up records (0: unchecked)
 };
 
 }  // namespace issdev

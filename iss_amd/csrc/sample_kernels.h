@@ -155,6 +155,10 @@ __device__ __forceinline__ void emit_lab(const SampleArgs& a, int64_t c, uint32_
         E = sqrtf(mT * mT + lz * lz);
         pzl = lz;
     }
+    if (a.out_cap && outp >= a.out_cap) {  // batched rounds: beyond the reserved output
+        atomicAdd(a.stats + 7, 1ull);
+        return;
+    }
     Hadron h;
     h.pid = pid;
     h.mass = fm;
@@ -241,7 +245,12 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict__ rec,
                                                const int64_t* __restrict__ list,
                                                const unsigned long long* __restrict__ nlist) {
-    const int64_t n = list ? static_cast<int64_t>(*nlist) : a.K;
+    if (a.chunk_dev) a.chunk = *a.chunk_dev;
+    int64_t n = list ? static_cast<int64_t>(*nlist) : (a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K);
+    if (a.setup_cap > 0 && n > a.setup_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.stats + 6, 1ull);
+        n = a.setup_cap;
+    }
     for (int64_t ri = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ri < n;
          ri += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t i = list ? list[ri] : ri;
@@ -268,7 +277,15 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     r.i0 = static_cast<int32_t>((q.x - a.pt.Et0[tb]) / a.pt.dEt[tb]);
     r.flags = (r.i0 < 0 || r.i0 >= a.pt.len[tb] - 1) ? 1 : 0;
     r.mterm = cm[2] + 2. * q.mut * cm[1] + q.aa * cm[0];
-    r.out = a.perm(static_cast<uint64_t>(i));
+    if (a.surf_off) {
+        // batched surfaces: slot of the hadron within its surface's block of
+        // this round, after the hadrons of the earlier rounds
+        const int k = upper_bound(a.surf_off + 1, 0, a.S, c);
+        const int64_t h0 = a.off[a.surf_off[k]];
+        r.out = *a.out_base + static_cast<uint64_t>(h0) + a.perms[k](static_cast<uint64_t>(i - h0));
+    } else {
+        r.out = a.perm(static_cast<uint64_t>(i));
+    }
     rec[ri] = r;
     }
 }
@@ -294,7 +311,9 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int64_t Kn = a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K;
+    if (a.chunk_dev) a.chunk = *a.chunk_dev;
+    int64_t Kn = a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K;
+    if (a.setup_cap > 0 && Kn > a.setup_cap) Kn = a.setup_cap;
     const int64_t base = Kn / nwaves, rem = Kn % nwaves;
     const int64_t beg = wid * base + min(wid, rem);
     const int64_t end = beg + base + (wid < rem ? 1 : 0);
