@@ -16,7 +16,16 @@ plain steps' rate and the ratio are reported beside it.  Both samplers use
 the physical Fermi-Dirac |p| proposal (fermion_cdf2_quirk = 0) so that every
 hadron goes through the proposal being measured.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Beside it (outside the timed region): the smooth Cooper-Frye spectra of
+pi+, K+ and p on the reference's (pT, phi, eta_s) quadrature grid over the same
+surface, FP32 and FP64.
+
+--workload c3 measures the batched configuration instead: 256 surfaces of
+log-normal size (median 3e4 cells, 1e3..5e5), each with its own event count
+in [10, 2000]; one step = one yield pass over all of them + one batched
+sampling pass (rounds of autotuned size replayed from a captured hipGraph).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload c2|c3]
 """
 from __future__ import annotations
 
@@ -44,6 +53,12 @@ OPS_ACCEPTED = 60.0            # boost to the lab frame and the record
 OPS_PLAIN_SETUP = 190.0        # species pick, |p| normalisation (FP64 counted as FP32)
 OPS_LEARN_SETUP = 2900.0       # 14 inputs, 128 GELU, 9-band envelope, thresholds
 OPS_LEARN_PROPOSAL = 250.0     # band pick, t, f0, direction, weights, envelope ratio
+# one smooth-spectrum integrand point as the reference writes it
+# (src/emissionfunction.cpp:756-806): pt, pz, p.u, exponent, exp, f0,
+# p.dsigma, the ten-term shear contraction, shear and bulk delta f, the
+# |delta f| restriction, the result and the weighted sum
+OPS_SPECTRA_POINT = 70.0
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 
 
 def yield_bytes(n_cells: int, n_species: int, with_bulk: bool) -> int:
@@ -149,6 +164,99 @@ def roofline_valu(kernel, ops, ms):
             "frac": tf / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms}
 
 
+def spectra_extra(ctx, sp, n_cells):
+    """Smooth spectra of pi+, K+, p on the reference grid, FP32 and FP64."""
+    from iss_amd import tables
+    b = tables.bin_tables()
+    idx = np.array([int(np.nonzero(sp["pid"] == p)[0][0]) for p in (211, 321, 2212)], dtype=np.int32)
+    out = {"species": [211, 321, 2212], "grid": [int(b["pT"].size), int(b["phi"].size), int(b["eta"].size)],
+           "cells": n_cells}
+    points = idx.size * b["pT"].size * b["phi"].size * b["eta"].size * n_cells
+    for prec in (32, 64):
+        ms = []
+        for _ in range(3):
+            ctx.spectra(idx, b["pT"], b["phi"], b["eta"], b["eta_weight"], precision=prec)
+            ms.append(ctx.spectra_ms())
+        t = float(min(ms))
+        tf = OPS_SPECTRA_POINT * points / (t * 1e-3) / 1e12
+        peak = FP32_PEAK_TFLOPS if prec == 32 else FP64_PEAK_TFLOPS
+        out[f"fp{prec}"] = {"ms": t, "points_per_s": points / (t * 1e-3),
+                            "roofline": {"bound": "valu", "achieved": tf, "peak": peak, "unit": "TFLOP/s",
+                                         "frac": tf / peak, "kernel": "k_spectra"}}
+    return out
+
+
+def run_c3(args):
+    import iss_amd
+    from iss_amd import synth, tables
+
+    sp = tables.species_arrays(1)
+    t0 = time.perf_counter()
+    surfaces, ev = synth.batched_surfaces(sp, n_surfaces=args.surfaces)
+    t_gen = time.perf_counter() - t0
+    ncell = np.array([s.shape[0] for s in surfaces])
+    ctx = iss_amd.Context(0)
+    ctx.set_species(sp)
+    ctx.set_deltaf(shear=1, bulk=1, bulk_kind=11, k11=tables.read_kind11(1), fermion_cdf2_quirk=0)
+    ctx.set_surfaces_lrf(surfaces)
+    del surfaces
+    tune = {}
+    if not args.no_tune:
+        ctx.compute_yields(precision=args.precision)
+        tune["plain"] = ctx.autotune(proposal=0, n_events=2, seed=7, repeats=1)
+        small = np.maximum(ev // 10, 1).astype(np.int32)
+        tune["batch"] = ctx.autotune_batch(small, seed=7, precision=args.precision)
+
+    def step(i):
+        ctx.compute_yields(precision=args.precision)
+        n = ctx.sample_batch(ev, seed=100 + i, precision=args.precision, use_graph=1)
+        return n, ctx.stats(), ctx.batch_ms()
+
+    for i in range(args.warmup):
+        step(i)
+    tot, ms_y, ms_b = 0, [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        n, st, mb = step(args.warmup + i)
+        tot += n
+        ms_y.append(st.ms_yield_kernel)
+        ms_b.append(mb)
+    el = time.perf_counter() - t0
+    nc, ns = ctx.num_cells(), ctx.n_species
+    ybytes = yield_bytes(nc, ns, True)
+    my = float(np.mean(ms_y))
+    y_gbs = ybytes / (my * 1e-3) / 1e9
+    out = {
+        "metric": "sampled hadrons/s (learned proposal vs plain GPU) at fixed chi2; yield-pass GB/s",
+        "value": tot / el,
+        "unit": "hadrons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched uneven surfaces (plain proposal)", "surfaces": int(ncell.size),
+                   "cells": int(nc), "cells_min_median_max": [int(ncell.min()), int(np.median(ncell)),
+                                                              int(ncell.max())],
+                   "events_min_max": [int(ev.min()), int(ev.max())], "species": ns,
+                   "hadrons_per_step": tot / args.steps, "events_per_round": ctx.lib.iss_batch_events_per_round(ctx.h),
+                   "rounds": ctx.lib.iss_batch_rounds(ctx.h), "generation_s": t_gen},
+        "roofline": {"bound": "hbm", "achieved": y_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": y_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_yield", "ms": my,
+                     "bytes": ybytes},
+        "cpu_baseline": None,
+        "batch_ms": float(np.mean(ms_b)),
+        "batch_hadrons_per_s": (tot / args.steps) / (1e-3 * float(np.mean(ms_b))),
+        "autotune": tune,
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +268,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-spectra", action="store_true", help="skip the smooth-spectra measurement")
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2")
+    ap.add_argument("--surfaces", type=int, default=256, help="c3: number of surfaces")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -168,6 +279,9 @@ def main():
         print(f"bench.py runs on one GPU only (the sampler has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        run_c3(args)
+        return
 
     import iss_amd
     from iss_amd import synth, tables
@@ -251,6 +365,8 @@ def main():
                             "k_trials": p_trials},
         "yield_GBs": y_gbs,
     }
+    if not args.no_spectra:
+        out["spectra"] = spectra_extra(ctx, sp, nc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells)
     ctx.close()

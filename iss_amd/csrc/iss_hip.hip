@@ -912,7 +912,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     if (o->proposal == 0) {
         HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
         if (K > 0) {
-            k_setup<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
+            k_setup<false><<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
             HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
             const int64_t want = (K + 255) / 256;
             const int nblk = static_cast<int>(
@@ -949,7 +949,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
                 std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
             k_learn_trials<<<nblk, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
             // hadrons without an envelope: the plain sampler, sized on the device
-            k_setup<<<ctx->num_cu * 4, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p, ctx->d_fb_count.p);
+            k_setup<false><<<ctx->num_cu * 4, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p, ctx->d_fb_count.p);
             SampleArgs af = a;
             af.Kdev = ctx->d_fb_count.p;
             k_trials<<<ctx->num_cu * per_cu_trials, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
@@ -1454,7 +1454,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         }
         k_batch_multi_leaves<<<((static_cast<int64_t>(S) << D) + 255) / 256, 256, 0, ctx->stream>>>(
             b, cur, D, ctx->d_bevcount.p);
-        k_setup<<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
+        k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
         k_trials<<<trial_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
         k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);
     };

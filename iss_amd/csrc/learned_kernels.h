@@ -965,7 +965,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const float px = p * (R00 * n0 + R01 * n1 + R02 * n2);
             const float py = p * (R10 * n0 + R11 * n1 + R12 * n2);
             const float pz = p * (R20 * n0 + R21 * n1 + R22 * n2);
-            emit_lab(a, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
+            emit_lab(a, a.chunk, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
             done = true;
         } else if (it >= max_it) {
             n_fail++;

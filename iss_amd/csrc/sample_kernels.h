@@ -133,7 +133,7 @@ struct Hadron {
 // write the hadron with its space-time position (src/FSSW.cpp:1969-1996).
 // Boost-invariant surfaces draw y uniform in [y_LB, y_RB] and set
 // eta_s = y - (y - eta_s) (src/FSSW.cpp:1024-1029).
-__device__ __forceinline__ void emit_lab(const SampleArgs& a, int64_t c, uint32_t jj, int pid, float fm,
+__device__ __forceinline__ void emit_lab(const SampleArgs& a, uint32_t chunk, int64_t c, uint32_t jj, int pid, float fm,
                                          float px, float py, float pz, float p0, uint64_t outp) {
     const int64_t Nc = a.Nc;
     const float u0 = a.cells[F_U0 * Nc + c], u1 = a.cells[F_U1 * Nc + c],
@@ -145,7 +145,7 @@ __device__ __forceinline__ void emit_lab(const SampleArgs& a, int64_t c, uint32_
     const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
     float eta_s, E, pzl;
     if (a.boost_invariant) {
-        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk) ^ 0x00800000u);
+        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, chunk) ^ 0x00800000u);
         const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
         eta_s = rap - (asinhf(lz / mT) - eta);
         E = mT * coshf(rap);
@@ -242,10 +242,16 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 // With a slot list, record r describes slot list[r] for r < *nlist (the
 // learned sampler's fallback hadrons); without one, record i describes slot
 // i.  Grid-stride, one thread per record.
+// BATCH: slots of batched surfaces (per-surface permutations, output base of
+// the round); instantiated separately so that the one-surface kernel keeps
+// its registers.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict__ rec,
                                                const int64_t* __restrict__ list,
                                                const unsigned long long* __restrict__ nlist) {
-    if (a.chunk_dev) a.chunk = *a.chunk_dev;
+    // (the pass index of batched rounds is read here; the kernel argument
+    // block itself is never written, which would copy it to scratch)
+    const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
     int64_t n = list ? static_cast<int64_t>(*nlist) : (a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K);
     if (a.setup_cap > 0 && n > a.setup_cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.stats + 6, 1ull);
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
     int s;
     {
-        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, chunk));
         const double* row = a.cdf + c * a.Ns;
         s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
         if (s >= a.Ns) s = a.Ns - 1;
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     r.i0 = static_cast<int32_t>((q.x - a.pt.Et0[tb]) / a.pt.dEt[tb]);
     r.flags = (r.i0 < 0 || r.i0 >= a.pt.len[tb] - 1) ? 1 : 0;
     r.mterm = cm[2] + 2. * q.mut * cm[1] + q.aa * cm[0];
-    if (a.surf_off) {
+    if (BATCH) {
         // batched surfaces: slot of the hadron within its surface's block of
         // this round, after the hadrons of the earlier rounds
         const int k = upper_bound(a.surf_off + 1, 0, a.S, c);
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (a.chunk_dev) a.chunk = *a.chunk_dev;
+    const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
     int64_t Kn = a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K;
     if (a.setup_cap > 0 && Kn > a.setup_cap) Kn = a.setup_cap;
     const int64_t base = Kn / nwaves, rem = Kn % nwaves;
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     const int64_t Nc = a.Nc;
     const PTabDev& pt = a.pt;
     const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
-    const uint32_t tw = tag_word(TAG_TRIAL, a.chunk);
+    const uint32_t tw = tag_word(TAG_TRIAL, chunk);
     const int max_it = a.max_trials * 64;
 
     int64_t next = beg;
@@ -471,7 +477,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
             const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
             const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
             if (ua < w1 * w2) {
-                emit_lab(a, c, jj, a.sp.pid[s], fm, px, py, pz, p0, outp);
+                emit_lab(a, chunk, c, jj, a.sp.pid[s], fm, px, py, pz, p0, outp);
                 done = true;
             } else {
                 trials++;

@@ -8,6 +8,9 @@ Configurations (all seeded, generated in memory):
       T in [0.135, 0.160] GeV, mu_B in [0, 0.3] GeV, |u_perp| <= 3, random
       traceless transverse pi^{mu nu} with |pi|/(e+P) <= 0.3, bulk
       Pi/(e+P) in [-0.2, 0], normalisation such that <hadrons/event> ~ 1e4.
+  C3  256 surfaces generated as C2 with seeds 1000 + k, cell counts
+      clip(round(exp(N(ln 3e4, 1))), 1e3, 5e5), oversampled events
+      U_int[10, 2000], and C2's hadron density per cell (0.05 per event).
   C5  as C2 with 2,000,000 cells and mu_B up to 0.45 GeV.
 
   LAB viscous lab-frame cells (viscous_lab): eta_s in [-1.5, 1.5], flow with
@@ -54,20 +57,26 @@ def hrg_thermo(T, muB, sp, nterms=10):
     return e / c3, P / c3, nB / c3
 
 
-def hrg_thermo_fast(T, muB, sp):
+def hrg_grid(sp, t0, t1, m0, m1, n=64):
+    """hrg_thermo tabulated on an n x n (T, mu_B) grid (for hrg_thermo_fast)."""
+    tg = np.linspace(t0, t1, n)
+    mg = np.linspace(m0, m1, n)
+    TT, MM = np.meshgrid(tg, mg, indexing="ij")
+    return (t0, t1, m0, m1, n, hrg_thermo(TT.ravel(), MM.ravel(), sp))
+
+
+def hrg_thermo_fast(T, muB, sp, grid=None):
     """hrg_thermo on a 64 x 64 (T, mu_B) grid with bilinear interpolation for
-    large surfaces (relative interpolation error ~1e-5)."""
+    large surfaces (relative interpolation error ~1e-5); `grid` (hrg_grid)
+    reuses one table for many surfaces."""
     T = np.asarray(T, dtype=np.float64)
     muB = np.asarray(muB, dtype=np.float64)
-    if T.size < 5000:
+    if grid is None and T.size < 5000:
         return hrg_thermo(T, muB, sp)
-    t0, t1 = T.min(), T.max() + 1e-9
-    m0, m1 = muB.min(), muB.max() + 1e-9
-    nt = nm = 64
-    tg = np.linspace(t0, t1, nt)
-    mg = np.linspace(m0, m1, nm)
-    TT, MM = np.meshgrid(tg, mg, indexing="ij")
-    grids = hrg_thermo(TT.ravel(), MM.ravel(), sp)
+    if grid is None:
+        grid = hrg_grid(sp, T.min(), T.max() + 1e-9, muB.min(), muB.max() + 1e-9)
+    t0, t1, m0, m1, nt, grids = grid
+    nm = nt
     ft = (T - t0) / (t1 - t0) * (nt - 1)
     fm = (muB - m0) / (m1 - m0) * (nm - 1)
     it = np.clip(ft.astype(int), 0, nt - 2)
@@ -92,7 +101,7 @@ def _lrf_shear(rng, n, epp, frac_max=0.3):
     return comp * scale[:, None]
 
 
-def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=True, shear=True):
+def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=True, shear=True, grid=None):
     """C2-style rest-frame surface (28 floats per cell) and the area scale."""
     rng = np.random.default_rng(seed)
     n = int(n_cells)
@@ -118,7 +127,7 @@ def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=Tru
     c[:, 9] = ux
     c[:, 10] = uy
     c[:, 11] = utau * np.sinh(eta)
-    e, P, nB = hrg_thermo_fast(T, muB, sp)
+    e, P, nB = hrg_thermo_fast(T, muB, sp, grid)
     c[:, 12], c[:, 13], c[:, 14], c[:, 15], c[:, 16] = e, T, P, nB, muB
     epp = e + P
     if bulk:
@@ -225,3 +234,18 @@ def viscous_lab(n_cells, sp, seed=7, muB_max=0.2, shear=True, bulk=True, frac_sp
     if bulk:
         c[:, 29] = -rng.uniform(0, 0.15, n) * epp
     return c.astype(np.float32)
+
+
+def batched_surfaces(sp, n_surfaces=256, seed=3, median_cells=3.0e4, sigma=1.0, min_cells=1000,
+                     max_cells=500000, events=(10, 2000), hadrons_per_cell=0.05):
+    """C3: (list of rest-frame surfaces, events per surface)."""
+    rng = np.random.default_rng(seed)
+    n = np.clip(np.round(np.exp(rng.normal(np.log(median_cells), sigma, n_surfaces))), min_cells,
+                max_cells).astype(np.int64)
+    ev = rng.integers(events[0], events[1] + 1, n_surfaces).astype(np.int32)
+    grid = hrg_grid(sp, 0.135, 0.160 + 1e-9, 0.0, 0.3 + 1e-9)
+    surfaces = []
+    for k in range(n_surfaces):
+        c, _ = viscous_lrf(int(n[k]), sp, seed=1000 + k, target_hadrons=hadrons_per_cell * float(n[k]), grid=grid)
+        surfaces.append(c)
+    return surfaces, ev
