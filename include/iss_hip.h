@@ -119,6 +119,8 @@ typedef struct {
     int64_t n_violations;    /* learned proposal: draws above the envelope (0 when exact) */
     double ms_setup_kernel;  /* device time of the last per-hadron set-up kernel */
     double ms_trials_kernel; /* device time of the last trial kernel(s), fallback included */
+    double ms_decay;         /* device time of the last iss_decay */
+    int64_t n_decay_generations;
 } iss_stats;
 
 /* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */
@@ -288,6 +290,38 @@ int iss_get_batch_layout(iss_ctx* ctx, int64_t* segments, int64_t* event_sizes);
 /* times whole batches for R in {4, 8, 16, 32, 64}, keeps the fastest as the
  * default events_per_round; report: refill_min = R chosen, ms_* batch times */
 int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* opts, const int32_t* n_events, iss_tune_report* report);
+
+/* ---- resonance decays: particle_decay (src/particle_decay.cpp:33-533) driven
+ * to stable particles as FSSW::perform_resonance_feed_down
+ * (src/FSSW.cpp:1746-1779).  The table is the PDG list with its channels as
+ * particle_decay reads it (anti-baryon entries with conjugated daughters);
+ * entry i owns channels [first_channel[i], first_channel[i] + n_channels[i]),
+ * daughters[5 j .. 5 j + 4] the monvals of channel j (0 = none).  A particle
+ * whose first channel has one daughter is stable.  iss_decay replaces the
+ * hadron list of the last iss_sample by its stable descendants, event by
+ * event (iss_get_event_counts / iss_get_hadrons read the result): two- and
+ * three-body channels at the pole mass, other channel sizes leave no
+ * daughters, as in the reference. ---- */
+typedef struct {
+    int32_t n;
+    const int32_t* monval;
+    const double* mass;
+    const double* width;
+    const int32_t* stable;
+    const int32_t* first_channel;
+    const int32_t* n_channels;
+    int32_t n_channels_total;
+    const int32_t* npart;
+    const double* branching_ratio;
+    const int32_t* daughters;
+} iss_decay_table;
+
+int iss_set_decay_table(iss_ctx* ctx, const iss_decay_table* table);
+int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons);
+/* an external hadron list (n_events events of counts[e] hadrons, grouped by
+ * event) as the context's list, e.g. to decay it (no counterpart: the
+ * reference decays its own Hadron_list) */
+int iss_set_hadrons(iss_ctx* ctx, int32_t n_events, const int64_t* counts, const iss_hadron* hadrons);
 
 /* ---- flows: EmissionFunctionArray::calculate_flows
  * (src/emissionfunction.cpp:875-1023) on one spectrum dN[n_pT][n_phi].

@@ -75,7 +75,8 @@ class Stats(C.Structure):
                 ("ms_yield", C.c_double), ("ms_sample", C.c_double),
                 ("ms_sample_kernel", C.c_double), ("ms_yield_kernel", C.c_double),
                 ("n_fallback", C.c_int64), ("n_violations", C.c_int64),
-                ("ms_setup_kernel", C.c_double), ("ms_trials_kernel", C.c_double)]
+                ("ms_setup_kernel", C.c_double), ("ms_trials_kernel", C.c_double),
+                ("ms_decay", C.c_double), ("n_decay_generations", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -103,6 +104,13 @@ class TuneReport(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class _DecayTable(C.Structure):
+    _fields_ = [("n", C.c_int32), ("monval", C.c_void_p), ("mass", C.c_void_p), ("width", C.c_void_p),
+                ("stable", C.c_void_p), ("first_channel", C.c_void_p), ("n_channels", C.c_void_p),
+                ("n_channels_total", C.c_int32), ("npart", C.c_void_p), ("branching_ratio", C.c_void_p),
+                ("daughters", C.c_void_p)]
 
 
 class _BatchOpts(C.Structure):
@@ -149,6 +157,9 @@ _SIGNATURES = {
                                C.c_void_p, C.c_void_p, C.c_void_p]),
     "iss_band_diag": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
+    "iss_set_decay_table": (C.c_int, [C.c_void_p, C.POINTER(_DecayTable)]),
+    "iss_decay": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int64)]),
+    "iss_set_hadrons": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "iss_set_surfaces_lrf": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "iss_num_surfaces": (C.c_int, [C.c_void_p]),
     "iss_sample_batch": (C.c_int, [C.c_void_p, C.POINTER(_BatchOpts), C.c_void_p, C.POINTER(C.c_int64)]),
@@ -361,6 +372,36 @@ class Context:
         self._check(self.lib.iss_band_diag(self.h, cell.size, _ptr(cell), _ptr(species), _ptr(rho),
                                            _ptr(tab), _ptr(par)), "iss_band_diag")
         return tab, par
+
+    # ------------------------------------------------------ resonance decays
+    def set_decay_table(self, t: dict):
+        """t: tables.decay_table(...) (entries with their channels)"""
+        arrs = {"monval": np.ascontiguousarray(t["monval"], dtype=np.int32),
+                "mass": np.ascontiguousarray(t["mass"], dtype=np.float64),
+                "width": np.ascontiguousarray(t["width"], dtype=np.float64),
+                "stable": np.ascontiguousarray(t["stable"], dtype=np.int32),
+                "first_channel": np.ascontiguousarray(t["first_channel"], dtype=np.int32),
+                "n_channels": np.ascontiguousarray(t["n_channels"], dtype=np.int32),
+                "npart": np.ascontiguousarray(t["npart"], dtype=np.int32),
+                "branching_ratio": np.ascontiguousarray(t["branching_ratio"], dtype=np.float64),
+                "daughters": np.ascontiguousarray(t["daughters"], dtype=np.int32).reshape(-1)}
+        d = _DecayTable(arrs["monval"].size, _ptr(arrs["monval"]), _ptr(arrs["mass"]), _ptr(arrs["width"]),
+                        _ptr(arrs["stable"]), _ptr(arrs["first_channel"]), _ptr(arrs["n_channels"]),
+                        arrs["npart"].size, _ptr(arrs["npart"]), _ptr(arrs["branching_ratio"]),
+                        _ptr(arrs["daughters"]))
+        self._check(self.lib.iss_set_decay_table(self.h, C.byref(d)), "iss_set_decay_table")
+
+    def set_hadrons(self, counts, hadrons):
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        h = np.ascontiguousarray(hadrons, dtype=HADRON_DTYPE)
+        self._check(self.lib.iss_set_hadrons(self.h, c.size, _ptr(c), _ptr(h)), "iss_set_hadrons")
+        self.n_events = c.size
+
+    def decay(self, seed=1):
+        """replace the last sampled hadron list by its stable descendants"""
+        n = C.c_int64()
+        self._check(self.lib.iss_decay(self.h, seed, C.byref(n)), "iss_decay")
+        return n.value
 
     # ------------------------------------------------------ batched surfaces
     def set_surfaces_lrf(self, surfaces):

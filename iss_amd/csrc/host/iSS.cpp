@@ -315,8 +315,36 @@ int iSS::generate_samples() {
     so.max_trials = 5000;
     so.precision = precision;
     release_hadrons();
+    const bool decays = static_cast<int>(paraRdr_ptr->getVal("perform_decays", 0)) == 1;
+    if (decays) {
+        // the decay table as particle_decay reads it (src/particle_decay.cpp:33-170)
+        std::vector<int32_t> mv, st, c0, nc, np, dd;
+        std::vector<double> ms, wd, br;
+        for (const auto& p : particle_) {
+            mv.push_back(p.monval);
+            ms.push_back(p.mass);
+            wd.push_back(p.width);
+            st.push_back(p.stable);
+            c0.push_back(static_cast<int32_t>(np.size()));
+            nc.push_back(static_cast<int32_t>(p.decay_channels.size()));
+            for (const auto* d : p.decay_channels) {
+                np.push_back(d->decay_Npart);
+                br.push_back(d->branching_ratio);
+                for (int k = 0; k < 5; k++) dd.push_back(d->decay_part[k]);
+            }
+        }
+        iss_decay_table dt{static_cast<int32_t>(mv.size()), mv.data(), ms.data(), wd.data(), st.data(), c0.data(),
+                           nc.data(), static_cast<int32_t>(np.size()), np.data(), br.data(), dd.data()};
+        if (iss_set_decay_table(ctx_, &dt) != ISS_OK) {
+            error(std::string("iss_set_decay_table: ") + iss_last_error(ctx_));
+            return -1;
+        }
+    }
     // one sampling pass over events [event0, event0 + n): appended to the lists
-    auto run_pass = [&](int event0, int n, int proposal) -> int {
+    // (train: keep the pass's accepted hadrons as the network's training set,
+    // before any decays replace them)
+    int64_t ntrain = 0;
+    auto run_pass = [&](int event0, int n, int proposal, bool train) -> int {
         so.event0 = event0;
         so.n_events = n;
         so.proposal = proposal;
@@ -325,6 +353,18 @@ int iSS::generate_samples() {
         if (r != ISS_OK) {
             error(std::string("iss_sample: ") + iss_last_error(ctx_));
             return r;
+        }
+        if (train && iss_mlp_collect(ctx_, 10000000, &ntrain) != ISS_OK) {
+            error(std::string("learned proposal training set: ") + iss_last_error(ctx_));
+            return -1;
+        }
+        if (decays) {
+            // FSSW::shell -> perform_resonance_feed_down (src/FSSW.cpp:344-361, 1746-1779)
+            r = iss_decay(ctx_, so.seed ^ 0xD3CA7ull ^ (static_cast<uint64_t>(event0) << 20), &nh);
+            if (r != ISS_OK) {
+                error(std::string("iss_decay: ") + iss_last_error(ctx_));
+                return r;
+            }
         }
         std::vector<int64_t> counts(n);
         std::vector<iss_hadron> all(static_cast<size_t>(nh));
@@ -354,24 +394,22 @@ int iSS::generate_samples() {
         for (double v : species_dNdy_) per_event += v;
         if (hydro_mode != 2) per_event *= (yRB - yLB);
         const int n1 = std::max(1, std::min(nev, static_cast<int>(std::ceil(2.0e6 / std::max(per_event, 1e-9)))));
-        rc = run_pass(0, n1, 0);
+        rc = run_pass(0, n1, 0, n1 < nev);
         if (rc != ISS_OK) return rc;
         if (n1 < nev) {
-            int64_t ntrain = 0;
             iss_train_opts to{8192, 300, 3e-3f, static_cast<uint64_t>(randomSeed_) + 17u};
             iss_train_report tr{};
-            if (iss_mlp_collect(ctx_, 10000000, &ntrain) != ISS_OK || iss_mlp_init(ctx_, 1) != ISS_OK ||
-                iss_mlp_train(ctx_, &to, &tr) != ISS_OK) {
+            if (iss_mlp_init(ctx_, 1) != ISS_OK || iss_mlp_train(ctx_, &to, &tr) != ISS_OK) {
                 error(std::string("learned proposal training: ") + iss_last_error(ctx_));
                 return -1;
             }
             info("learned proposal trained on " + std::to_string(ntrain) + " hadrons, log envelope mass " +
                  std::to_string(tr.loss_first) + " -> " + std::to_string(tr.loss_last));
-            rc = run_pass(n1, nev - n1, 1);
+            rc = run_pass(n1, nev - n1, 1, false);
             if (rc != ISS_OK) return rc;
         }
     } else {
-        rc = run_pass(0, nev, 0);
+        rc = run_pass(0, nev, 0, false);
         if (rc != ISS_OK) return rc;
     }
     int64_t nh = 0;
@@ -381,8 +419,7 @@ int iSS::generate_samples() {
     if (st.n_failed > 0)
         warn(std::to_string(st.n_failed) + " hadrons exhausted the maximum number of trials");
     info("sampled " + std::to_string(nh) + " hadrons in " + std::to_string(nev) + " events");
-    if (static_cast<int>(paraRdr_ptr->getVal("perform_decays", 0)) == 1)
-        warn("perform_decays = 1 is not supported by this sampler; hadrons are primary");
+
     if (static_cast<int>(paraRdr_ptr->getVal("include_spectators", 0)) != 0)
         warn("include_spectators is not supported by this sampler");
     // writers (src/FSSW.cpp:354-360)

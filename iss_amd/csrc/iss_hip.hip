@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -41,6 +42,7 @@ using namespace issdev;
 #include "learned_kernels.h"
 #include "spectra_kernels.h"
 #include "batch_kernels.h"
+#include "decay_kernels.h"
 
 // =================================================================== host
 
@@ -159,6 +161,12 @@ struct iss_ctx {
     std::vector<int32_t> h_bevents;
     int batch_R_tuned = 16;
     double batch_ms = 0., batch_ms_graph_launch = 0.;
+    // resonance decay table
+    int dec_n = 0;
+    DBuf<int> d_dec_int;       // key, key_idx, stable, ch0, nch ([5][n]); npart, dpid[5], didx[5] ([11][nc])
+    DBuf<double> d_dec_dbl;    // mass, width ([2][n]); cumulative br [nc]
+    DecayDev dec{};
+    int64_t dec_failed = 0;
     // smooth spectra
     DBuf<char> d_srec, d_sgrid;
     DBuf<double> d_spart, d_sout;
@@ -637,6 +645,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_spart.release(); ctx->d_sout.release(); ctx->d_sspec.release();
     ctx->d_surf_off.release(); ctx->d_bevents.release(); ctx->d_bround.release(); ctx->d_bevcount.release();
     ctx->d_bbase.release(); ctx->d_perms.release(); ctx->d_seg.release(); ctx->d_lam.release();
+    ctx->d_dec_int.release(); ctx->d_dec_dbl.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1574,6 +1583,223 @@ int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_e
         rep->ms_default = ms_def;
         rep->n_tried = tried;
     }
+    return ISS_OK;
+}
+
+// ------------------------------------------------------------ resonance decays
+
+int iss_set_decay_table(iss_ctx* ctx, const iss_decay_table* t) {
+    if (!ctx || !t || t->n <= 0 || t->n_channels_total < 0) return ISS_ERR_ARG;
+    if (!t->monval || !t->mass || !t->width || !t->stable || !t->first_channel || !t->n_channels ||
+        (t->n_channels_total > 0 && (!t->npart || !t->branching_ratio || !t->daughters)))
+        return ISS_ERR_ARG;
+    const int n = t->n, nc = t->n_channels_total;
+    for (int i = 0; i < n; i++)
+        if (t->first_channel[i] < 0 || t->n_channels[i] < 0 || t->first_channel[i] + t->n_channels[i] > nc)
+            return set_err(ctx, ISS_ERR_ARG, "decay table: channel range out of bounds");
+    // ascending monval keys; for a repeated monval the first entry wins, as
+    // the reference's linear searches do
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t->monval[a] < t->monval[b]; });
+    std::vector<int> key, kidx;
+    for (int i : order)
+        if (key.empty() || key.back() != t->monval[i]) {
+            key.push_back(t->monval[i]);
+            kidx.push_back(i);
+        }
+    const int nk = static_cast<int>(key.size());
+    auto first_of = [&](int mv) {
+        const auto it = std::lower_bound(key.begin(), key.end(), mv);
+        return (it != key.end() && *it == mv) ? kidx[it - key.begin()] : -1;
+    };
+    std::vector<int> ints(5 * static_cast<size_t>(n) + 11 * static_cast<size_t>(std::max(nc, 1)), 0);
+    std::vector<double> dbl(2 * static_cast<size_t>(n) + std::max(nc, 1), 0.);
+    for (int i = 0; i < nk; i++) {
+        ints[i] = key[i];
+        ints[n + i] = kidx[i];
+    }
+    for (int i = 0; i < n; i++) {
+        ints[2 * n + i] = t->stable[i];
+        ints[3 * n + i] = t->first_channel[i];
+        ints[4 * n + i] = t->n_channels[i];
+        dbl[i] = t->mass[i];
+        dbl[n + i] = t->width[i];
+        double cum = 0.;
+        for (int j = t->first_channel[i]; j < t->first_channel[i] + t->n_channels[i]; j++) {
+            cum += t->branching_ratio[j];
+            dbl[2 * n + j] = cum;
+        }
+    }
+    int* ci = ints.data() + 5 * n;
+    for (int j = 0; j < nc; j++) {
+        ci[j] = t->npart[j];
+        for (int k = 0; k < 5; k++) {
+            const int mv = t->daughters[5 * j + k];
+            ci[nc + 5 * j + k] = mv;
+            ci[6 * nc + 5 * j + k] = mv != 0 ? first_of(mv) : -1;
+        }
+    }
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(ctx->d_dec_int.ensure(ints.size()));
+    HIPCHK(ctx->d_dec_dbl.ensure(dbl.size()));
+    HIPCHK(hipMemcpy(ctx->d_dec_int.p, ints.data(), sizeof(int) * ints.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_dec_dbl.p, dbl.data(), sizeof(double) * dbl.size(), hipMemcpyHostToDevice));
+    DecayDev& D = ctx->dec;
+    const int* I = ctx->d_dec_int.p;
+    const double* Dd = ctx->d_dec_dbl.p;
+    D.n = nk;
+    D.key = I;
+    D.key_idx = I + n;
+    D.stable = I + 2 * n;
+    D.ch0 = I + 3 * n;
+    D.nch = I + 4 * n;
+    D.npart = I + 5 * n;
+    D.dpid = I + 5 * n + nc;
+    D.didx = I + 5 * n + 6 * nc;
+    D.mass = Dd;
+    D.width = Dd + n;
+    D.cumbr = Dd + 2 * n;
+    ctx->dec_n = n;
+    return ISS_OK;
+}
+
+namespace {
+struct DecayGen {
+    const Hadron* P = nullptr;
+    int64_t n = 0;
+    DBuf<Hadron> own;
+    DBuf<uint8_t> kind;
+    DBuf<int32_t> chan;
+    DBuf<uint32_t> nchild, size;
+    DBuf<int64_t> off, pos, parent;
+};
+}  // namespace
+
+int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
+    if (!ctx) return ISS_ERR_ARG;
+    if (ctx->dec_n == 0) return set_err(ctx, ISS_ERR_STATE, "no decay table (iss_set_decay_table)");
+    if (ctx->E <= 0 && ctx->K > 0)
+        return set_err(ctx, ISS_ERR_UNSUPPORTED, "decays apply to the hadron list of iss_sample (not to batches)");
+    (void)hipSetDevice(ctx->device);
+    const int E = ctx->E;
+    std::vector<std::unique_ptr<DecayGen>> gens;
+    gens.emplace_back(new DecayGen);
+    gens[0]->P = ctx->d_out.p;
+    gens[0]->n = ctx->K;
+    HIPCHK(ctx->d_stats.ensure(8));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p + 6, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int g = 0;; g++) {
+        DecayGen& G = *gens[g];
+        const int64_t n = G.n;
+        HIPCHK(G.kind.ensure(n));
+        HIPCHK(G.chan.ensure(n));
+        HIPCHK(G.nchild.ensure(n));
+        HIPCHK(G.size.ensure(n));
+        HIPCHK(G.off.ensure(n + 1));
+        HIPCHK(G.pos.ensure(n + 1));
+        if (n > 0)
+            k_decay_pick<<<(n + 255) / 256, 256, 0, ctx->stream>>>(ctx->dec, G.P, n, g, seed, G.kind.p, G.chan.p,
+                                                                   G.nchild.p, ctx->d_stats.p + 6);
+        HIPCHK(hipGetLastError());
+        int rc = launch_scan(ctx, G.nchild.p, n, G.off.p);
+        if (rc) return rc;
+        int64_t nn = 0;
+        HIPCHK(hipMemcpyAsync(&nn, G.off.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (nn == 0) break;
+        if (g + 1 >= kDecayMaxGen) return set_err(ctx, ISS_ERR_RANGE, "decay chains deeper than 12 generations");
+        gens.emplace_back(new DecayGen);
+        DecayGen& H = *gens[g + 1];
+        HIPCHK(H.own.ensure(nn));
+        HIPCHK(H.parent.ensure(nn));
+        H.P = H.own.p;
+        H.n = nn;
+        k_decay_emit<<<(n + 255) / 256, 256, 0, ctx->stream>>>(ctx->dec, G.P, n, g, seed, G.kind.p, G.chan.p,
+                                                               G.off.p, H.own.p, H.parent.p, ctx->d_stats.p + 7);
+        HIPCHK(hipGetLastError());
+    }
+    // final descendants, bottom-up
+    const int last = static_cast<int>(gens.size()) - 1;
+    for (int g = last; g >= 0; g--) {
+        DecayGen& G = *gens[g];
+        if (G.n == 0) continue;
+        k_decay_size<<<(G.n + 255) / 256, 256, 0, ctx->stream>>>(G.kind.p, G.off.p, G.nchild.p,
+                                                                 g < last ? gens[g + 1]->size.p : nullptr, G.n,
+                                                                 G.size.p);
+    }
+    HIPCHK(hipGetLastError());
+    int rc = launch_scan(ctx, gens[0]->size.p, gens[0]->n, gens[0]->pos.p);
+    if (rc) return rc;
+    int64_t F = 0;
+    HIPCHK(hipMemcpyAsync(&F, gens[0]->pos.p + gens[0]->n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int g = 0; g < last; g++) {
+        DecayGen& G = *gens[g];
+        if (G.n == 0) continue;
+        k_decay_place<<<(G.n + 255) / 256, 256, 0, ctx->stream>>>(G.kind.p, G.off.p, G.nchild.p, G.pos.p,
+                                                                  gens[g + 1]->size.p, G.n, gens[g + 1]->pos.p);
+    }
+    DBuf<Hadron> out;
+    HIPCHK(out.ensure(std::max<int64_t>(F, 1)));
+    for (int g = 0; g <= last; g++) {
+        DecayGen& G = *gens[g];
+        if (G.n == 0) continue;
+        k_decay_write<<<(G.n + 255) / 256, 256, 0, ctx->stream>>>(G.P, G.kind.p, G.pos.p, G.n, out.p);
+    }
+    if (E > 0) {
+        HIPCHK(ctx->d_tree0.ensure(E + 1));
+        k_decay_events<<<(E + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_evoff.p, E, gens[0]->pos.p,
+                                                                      ctx->d_tree0.p);
+        HIPCHK(hipMemcpyAsync(ctx->d_evoff.p, ctx->d_tree0.p, sizeof(int64_t) * (E + 1), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    unsigned long long nfb[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(nfb, ctx->d_stats.p + 6, sizeof(nfb), hipMemcpyDeviceToHost, ctx->stream));
+    const unsigned long long nbelow = nfb[0], nf = nfb[1];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    gens.clear();
+    std::swap(ctx->d_out.p, out.p);
+    std::swap(ctx->d_out.cap, out.cap);
+    out.release();
+    ctx->K = F;
+    ctx->stats.n_hadrons = F;
+    ctx->stats.ms_decay = ms;
+    ctx->stats.n_decay_generations = last + 1;
+    ctx->dec_failed = static_cast<int64_t>(nf);
+    ctx->last_plain = false;
+    if (n_hadrons) *n_hadrons = F;
+    if (nf) return set_err(ctx, ISS_ERR_RANGE, "three-body decay found no kinematically allowed point");
+    if (nbelow)
+        return set_err(ctx, ISS_ERR_RANGE, std::to_string(nbelow) +
+                                               " decays with the pole mass below the daughters' masses "
+                                               "(the reference exits; the mothers were dropped)");
+    return ISS_OK;
+}
+
+int iss_set_hadrons(iss_ctx* ctx, int32_t n_events, const int64_t* counts, const iss_hadron* hadrons) {
+    if (!ctx || n_events < 0 || (n_events > 0 && !counts)) return ISS_ERR_ARG;
+    std::vector<int64_t> off(n_events + 1, 0);
+    for (int e = 0; e < n_events; e++) {
+        if (counts[e] < 0) return set_err(ctx, ISS_ERR_ARG, "negative event size");
+        off[e + 1] = off[e] + counts[e];
+    }
+    const int64_t K = off[n_events];
+    if (K > 0 && !hadrons) return ISS_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(ctx->d_out.ensure(std::max<int64_t>(K, 1)));
+    HIPCHK(ctx->d_evoff.ensure(n_events + 1));
+    if (K > 0) HIPCHK(hipMemcpy(ctx->d_out.p, hadrons, sizeof(Hadron) * K, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_evoff.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+    ctx->K = K;
+    ctx->E = n_events;
+    ctx->stats.n_hadrons = K;
+    ctx->last_plain = false;
     return ISS_OK;
 }
 

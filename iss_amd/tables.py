@@ -72,6 +72,62 @@ def species_arrays(afterburner: int = 1, quantum_statistics: bool = True,
     return sp
 
 
+def decay_table(afterburner: int = 1, table_dir: str = TABLE_DIR) -> dict:
+    """The PDG list with its decay channels as particle_decay reads it
+    (src/particle_decay.cpp:33-170): an anti-baryon entry after every baryon
+    with B = 1, its daughters conjugated unless neutral in B, Q and S; stable
+    when the first channel has one daughter."""
+    toks = open(os.path.join(table_dir, AFTERBURNER_FILES[afterburner][0])).read().split()
+    ent, chans = [], []
+    i = 0
+    while i + 12 <= len(toks):
+        mon = int(toks[i])
+        mass, width = float(toks[i + 2]), float(toks[i + 3])
+        B, S, Q, nd = int(toks[i + 5]), int(toks[i + 6]), int(toks[i + 10]), int(toks[i + 11])
+        i += 12
+        ch = []
+        for _ in range(nd):
+            c = toks[i:i + 8]
+            i += 8
+            ch.append((int(c[1]), float(c[2]), [int(x) for x in c[3:8]]))
+        ent.append(dict(monval=mon, mass=mass, width=width, B=B, S=S, Q=Q, ch=ch))
+        if B == 1:
+            conj = []
+            for npart, br, parts in ch:
+                new = []
+                for d in parts:
+                    if d == 0:
+                        new.append(0)
+                        continue
+                    q = next((e for e in ent if e["monval"] == d), None)
+                    neutral = q is not None and q["B"] == 0 and q["Q"] == 0 and q["S"] == 0
+                    new.append(d if neutral else -d)
+                conj.append((npart, br, new))
+            ent.append(dict(monval=-mon, mass=mass, width=width, B=-B, S=-S, Q=-Q, ch=conj))
+    out = {k: [] for k in ("monval", "mass", "width", "stable", "first_channel", "n_channels", "baryon",
+                           "strange", "charge")}
+    npart, br, dau = [], [], []
+    for e in ent:
+        out["monval"].append(e["monval"])
+        out["mass"].append(e["mass"])
+        out["width"].append(e["width"])
+        out["stable"].append(1 if e["ch"] and e["ch"][0][0] == 1 else 0)
+        out["first_channel"].append(len(npart))
+        out["n_channels"].append(len(e["ch"]))
+        out["baryon"].append(e["B"])
+        out["strange"].append(e["S"])
+        out["charge"].append(e["Q"])
+        for c in e["ch"]:
+            npart.append(c[0])
+            br.append(c[1])
+            dau.append(c[2])
+    res = {k: np.array(v, dtype=np.float64 if k in ("mass", "width") else np.int32) for k, v in out.items()}
+    res["npart"] = np.array(npart, dtype=np.int32)
+    res["branching_ratio"] = np.array(br, dtype=np.float64)
+    res["daughters"] = np.array(dau, dtype=np.int32).reshape(-1, 5)
+    return res
+
+
 def bin_tables(table_dir: str = TABLE_DIR) -> dict:
     """Quadrature grids of the smooth spectra (src/iSS.cpp:151-154): Gauss pT
     (15 nodes, GeV) and phi (48 nodes) with weights, and the uniform eta_s

@@ -1140,4 +1140,199 @@ void flows(const std::vector<double>& dN, const std::vector<double>& pT, const s
     }
 }
 
+// ---------------------------------------------------------- resonance decays
+
+// One entry per particle with all its channels; an anti-baryon entry follows
+// every baryon with B = 1, its daughters conjugated unless neutral in B, Q
+// and S (src/particle_decay.cpp:33-170).  A particle is stable when its first
+// channel has one daughter.
+bool Decayer::load(const std::string& file) {
+    table.clear();
+    std::ifstream f(file);
+    if (!f.good()) return false;
+    while (true) {
+        DecayParticle p;
+        std::string name;
+        int gspin, charm, bottom, giso, ndec;
+        if (!(f >> p.monval)) break;
+        f >> name >> p.mass >> p.width >> gspin >> p.baryon >> p.strange >> charm >> bottom >> giso >>
+            p.charge >> ndec;
+        for (int j = 0; j < ndec; j++) {
+            DecayChannel c;
+            int dummy;
+            f >> dummy >> c.npart >> c.br >> c.part[0] >> c.part[1] >> c.part[2] >> c.part[3] >> c.part[4];
+            p.ch.push_back(c);
+        }
+        if (!f) break;
+        p.stable = (!p.ch.empty() && p.ch[0].npart == 1) ? 1 : 0;
+        table.push_back(p);
+        if (p.baryon == 1) {
+            DecayParticle a = p;
+            a.monval = -p.monval;
+            a.baryon = -p.baryon;
+            a.strange = -p.strange;
+            a.charge = -p.charge;
+            for (auto& c : a.ch)
+                for (int k = 0; k < 5; k++) {
+                    const int d = c.part[k];
+                    if (d == 0) continue;
+                    const DecayParticle* dp = nullptr;
+                    for (const auto& q : table)
+                        if (q.monval == d) { dp = &q; break; }
+                    // a daughter missing from the list is conjugated
+                    const bool neutral = dp && dp->baryon == 0 && dp->charge == 0 && dp->strange == 0;
+                    c.part[k] = neutral ? d : -d;
+                }
+            table.push_back(a);
+        }
+    }
+    return !table.empty();
+}
+
+const DecayParticle* Decayer::find(int monval) const {
+    for (const auto& p : table)
+        if (p.monval == monval) return &p;
+    return nullptr;
+}
+
+namespace {
+// boost a rest-frame momentum by the mother's velocity (src/particle_decay.cpp:376-394)
+void boost_to_lab(double vx, double vy, double vz, double E, double px, double py, double pz, Hadron& h) {
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double gamma = 1. / std::sqrt(1. - v2);
+    const double gm1 = gamma - 1.;
+    const double vp = vx * px + vy * py + vz * pz;
+    const double k = v2 > 0. ? gm1 * vp / v2 + gamma * E : 0.;  // at rest: no boost (the reference: 0/0)
+    h.E = static_cast<float>(gamma * (E + vp));
+    h.px = static_cast<float>(px + k * vx);
+    h.py = static_cast<float>(py + k * vy);
+    h.pz = static_cast<float>(pz + k * vz);
+}
+}  // namespace
+
+// Channel by cumulative branching ratio, then the two-body
+// (src/particle_decay.cpp:345-418) or three-body (:422-533) decay at the pole
+// mass (the Breit-Wigner draw is off there), isotropic in the rest frame, an
+// exponential lifetime of mean E / (M Gamma) (hbar c = 0.19733 GeV fm), the
+// decay point on the mother's straight line.  Channels of any other size
+// leave no daughters, as in the reference.
+void Decayer::decay(const Hadron& m, std::vector<Hadron>& out, std::mt19937& g,
+                    std::uniform_real_distribution<double>& U) const {
+    const DecayParticle* M = find(m.pid);
+    if (!M || M->stable) {
+        out.push_back(m);
+        return;
+    }
+    const double r = U(g);
+    double cum = 0.;
+    const DecayChannel* c = nullptr;
+    for (const auto& ch : M->ch) {
+        cum += ch.br;
+        if (cum > r) { c = &ch; break; }
+    }
+    if (!c) c = &M->ch.back();   // branching ratios summing below r (the reference dereferences null)
+    const double Mp = m.mass, W = M->width;
+    const double vx = m.px / m.E, vy = m.py / m.E, vz = m.pz / m.E;
+    if (c->npart == 2) {
+        Hadron d1{}, d2{};
+        d1.pid = c->part[0];
+        d2.pid = c->part[1];
+        d1.mass = static_cast<float>(find(d1.pid) ? find(d1.pid)->mass : 0.);
+        d2.mass = static_cast<float>(find(d2.pid) ? find(d2.pid)->mass : 0.);
+        const double m1 = d1.mass, m2 = d2.mass;
+        if (Mp < m1 + m2) {
+            n_below++;
+            return;
+        }
+        const double t = Mp * Mp - m1 * m1 - m2 * m2;
+        const double p = std::sqrt(t * t - 4 * m1 * m1 * m2 * m2) / (2 * Mp);
+        const double phi = U(g) * 2 * M_PI;
+        const double cth = 2. * (U(g) - 0.5);
+        const double sth = std::sqrt(1. - cth * cth);
+        const double px = p * sth * std::cos(phi), py = p * sth * std::sin(phi), pz = p * cth;
+        boost_to_lab(vx, vy, vz, std::sqrt(p * p + m1 * m1), px, py, pz, d1);
+        boost_to_lab(vx, vy, vz, std::sqrt(p * p + m2 * m2), -px, -py, -pz, d2);
+        double life = 1e10;
+        if (W > 1e-10) life = -(m.E / Mp / W) * std::log(U(g)) * 0.19733;
+        for (Hadron* d : {&d1, &d2}) {
+            d->t = static_cast<float>(m.t + life);
+            d->x = static_cast<float>(m.x + m.px / m.E * life);
+            d->y = static_cast<float>(m.y + m.py / m.E * life);
+            d->z = static_cast<float>(m.z + m.pz / m.E * life);
+        }
+        out.push_back(d1);
+        out.push_back(d2);
+    } else if (c->npart == 3) {
+        Hadron d[3]{};
+        double ms[3];
+        for (int k = 0; k < 3; k++) {
+            d[k].pid = c->part[k];
+            d[k].mass = static_cast<float>(find(d[k].pid) ? find(d[k].pid)->mass : 0.);
+            ms[k] = d[k].mass;
+        }
+        if (Mp < ms[0] + ms[1] + ms[2]) {
+            n_below++;
+            return;
+        }
+        double E1, E2, E3, p1, p2, c12;
+        do {
+            do {
+                E1 = U(g) * (Mp - ms[0] - ms[1] - ms[2]) + ms[0];
+                E2 = U(g) * (Mp - ms[0] - ms[1] - ms[2]) + ms[1];
+            } while (E1 + E2 > Mp);
+            p1 = std::sqrt(E1 * E1 - ms[0] * ms[0]);
+            p2 = std::sqrt(E2 * E2 - ms[1] * ms[1]);
+            E3 = Mp - E1 - E2;
+            c12 = (E3 * E3 - p1 * p1 - p2 * p2 - ms[2] * ms[2]) / (2. * p1 * p2);
+        } while (c12 < -1.0 || c12 > 1.0);
+        double life = 1e10;
+        if (W > 1e-10) life = -(m.E / Mp / W) * std::log(U(g)) * 0.19733;
+        // daughters 2 and 3 in the plane of daughter 1 (along z), then an
+        // Euler rotation (phi, ksi, theta)
+        const double t2x = p2 * std::sqrt(1. - c12 * c12), t2z = p2 * c12;
+        const double t3x = -t2x, t3z = -(p1 + t2z);
+        const double phi = 2. * M_PI * U(g);
+        const double ksi = 2. * M_PI * U(g);
+        const double cth = 2. * U(g) - 1.0;
+        const double sp = std::sin(phi), cp = std::cos(phi), sk = std::sin(ksi), ck = std::cos(ksi);
+        const double sth = std::sqrt(1. - cth * cth);
+        const double q[3][3] = {
+            {-p1 * sth * ck, p1 * sth * sk, p1 * cth},
+            {t2x * (cp * cth * ck - sp * sk) - t2z * sth * ck, t2x * (-cp * cth * sk - sp * ck) + t2z * sth * sk,
+             t2x * (cp * sth) + t2z * cth},
+            {t3x * (cp * cth * ck - sp * sk) - t3z * sth * ck, t3x * (-cp * cth * sk - sp * ck) + t3z * (sth * sk),
+             t3x * cp * sth + t3z * cth}};
+        for (int k = 0; k < 3; k++) {
+            const double Ek = std::sqrt(ms[k] * ms[k] + q[k][0] * q[k][0] + q[k][1] * q[k][1] + q[k][2] * q[k][2]);
+            boost_to_lab(vx, vy, vz, Ek, q[k][0], q[k][1], q[k][2], d[k]);
+            d[k].t = static_cast<float>(m.t + life);
+            d[k].x = static_cast<float>(m.x + m.px / m.E * life);
+            d[k].y = static_cast<float>(m.y + m.py / m.E * life);
+            d[k].z = static_cast<float>(m.z + m.pz / m.E * life);
+        }
+        for (int k = 0; k < 3; k++) out.push_back(d[k]);
+    }
+}
+
+// Decays every particle of the event, appending unstable daughters to the
+// work list and stable ones to the event, in the reference's order.
+void Decayer::feed_down(std::vector<Hadron>& event, std::mt19937& g,
+                        std::uniform_real_distribution<double>& U) const {
+    std::vector<Hadron> work(event);
+    event.clear();
+    std::vector<Hadron> ds;
+    for (size_t i = 0; i < work.size(); i++) {
+        ds.clear();
+        const Hadron m = work[i];
+        decay(m, ds, g, U);
+        for (const auto& d : ds) {
+            const DecayParticle* q = find(d.pid);
+            // particles missing from the list stay final (the reference would
+            // dereference a null entry for them)
+            if (!q || q->stable == 1) event.push_back(d);
+            else work.push_back(d);
+        }
+    }
+}
+
 }  // namespace iss_oracle

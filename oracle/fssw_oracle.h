@@ -25,6 +25,9 @@
 //                                      cell, shear cos 2phi term, bulk K0/K2)
 //   * flows .......................... pinned by Bessel I_n ratios and by
 //                                      pure Fourier inputs
+//   * resonance decays ............... pinned draw-for-draw vs oracle/_ref
+//                                      (the reference's particle_decay sources)
+//                                      and by conservation laws
 #pragma once
 
 #include <array>
@@ -224,5 +227,35 @@ void smooth_spectra(const FSSWOracle& o, const std::vector<Surf>& lab, int real_
 void flows(const std::vector<double>& dN, const std::vector<double>& pT, const std::vector<double>& pTw,
            const std::vector<double>& phi, const std::vector<double>& phiw, double mass, int to_order,
            std::vector<double>& diff, std::vector<double>& inte);
+
+// ---------------------------------------------------------- resonance decays
+struct DecayChannel {
+    int npart = 0;
+    double br = 0.;
+    int part[5] = {0, 0, 0, 0, 0};
+};
+struct DecayParticle {
+    int monval = 0;
+    double mass = 0., width = 0.;
+    int baryon = 0, strange = 0, charge = 0, stable = 0;
+    std::vector<DecayChannel> ch;
+};
+class Decayer {
+ public:
+    std::vector<DecayParticle> table;
+    // decays below threshold (pole mass under the daughters' masses): the
+    // reference exits there (src/particle_decay.cpp:351-357, 431-441); here
+    // the mother is dropped and counted
+    mutable long n_below = 0;
+    // particle_decay::read_resonances_list (src/particle_decay.cpp:33-170)
+    bool load(const std::string& pdg_file);
+    const DecayParticle* find(int monval) const;
+    // particle_decay::perform_decays (src/particle_decay.cpp:262-320)
+    void decay(const Hadron& m, std::vector<Hadron>& out, std::mt19937& g,
+               std::uniform_real_distribution<double>& U) const;
+    // FSSW::perform_resonance_feed_down (src/FSSW.cpp:1746-1779) on one event
+    void feed_down(std::vector<Hadron>& event, std::mt19937& g,
+                   std::uniform_real_distribution<double>& U) const;
+};
 
 }  // namespace iss_oracle

@@ -279,4 +279,68 @@ void orc_flows(const double* dN, const double* pT, const double* pTw, int npT, c
     std::memcpy(inte, o2.data(), sizeof(double) * o2.size());
 }
 
+// resonance feed-down of events (hadrons grouped by event, counts[nev]) with
+// one mt19937 stream over all events, as the reference's shared generator
+static Decayer* g_decayer = nullptr;
+static std::string g_decayer_file;
+
+long orc_decay(const char* pdg_file, unsigned seed, long nev, const long* counts, const void* in, long cap,
+               long* counts_out, void* out) {
+    if (!g_decayer || g_decayer_file != pdg_file) {
+        delete g_decayer;
+        g_decayer = new Decayer;
+        if (!g_decayer->load(pdg_file)) return -1;
+        g_decayer_file = pdg_file;
+    }
+    std::mt19937 g(seed);
+    std::uniform_real_distribution<double> U(0., 1.);
+    g_decayer->n_below = 0;
+    const Hadron* h = static_cast<const Hadron*>(in);
+    Hadron* o = static_cast<Hadron*>(out);
+    long k = 0, w = 0;
+    for (long e = 0; e < nev; e++) {
+        std::vector<Hadron> ev(h + k, h + k + counts[e]);
+        k += counts[e];
+        g_decayer->feed_down(ev, g, U);
+        counts_out[e] = static_cast<long>(ev.size());
+        for (const auto& x : ev) {
+            if (w < cap) o[w] = x;
+            w++;
+        }
+    }
+    return w;
+}
+
+long orc_decay_below() { return g_decayer ? g_decayer->n_below : 0; }
+
+// the decay table as the decayer holds it: per entry monval, mass, width,
+// stable, baryon, strange, charge, first channel; per channel npart, br, 5 daughters
+long orc_decay_table(const char* pdg_file, int* ints, double* dbls, int* chan_ints, double* chan_br, long cap_p,
+                     long cap_c, long* n_chan) {
+    Decayer d;
+    if (!d.load(pdg_file)) return -1;
+    long nc = 0;
+    for (size_t i = 0; i < d.table.size(); i++) {
+        const auto& p = d.table[i];
+        if (static_cast<long>(i) < cap_p) {
+            int* q = ints + 7 * i;
+            q[0] = p.monval; q[1] = p.stable; q[2] = p.baryon; q[3] = p.strange; q[4] = p.charge;
+            q[5] = static_cast<int>(nc); q[6] = static_cast<int>(p.ch.size());
+            dbls[2 * i] = p.mass;
+            dbls[2 * i + 1] = p.width;
+        }
+        for (const auto& c : p.ch) {
+            if (nc < cap_c) {
+                int* q = chan_ints + 6 * nc;
+                q[0] = c.npart;
+                for (int k = 0; k < 5; k++) q[1 + k] = c.part[k];
+                chan_br[nc] = c.br;
+            }
+            nc++;
+        }
+    }
+    *n_chan = nc;
+    return static_cast<long>(d.table.size());
+}
+
 }  // extern "C"

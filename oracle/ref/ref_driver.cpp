@@ -5,6 +5,8 @@
 //   * MomentumSamplerShell (src/MomentumSamplerShell.cpp:6-48) + Random
 //   * RandomVariable1DArray (src/RandomVariable1DArray.cpp:25-74)
 //   * read_FOdata reader + regulation + PDG list (src/readindata.cpp)
+//   * particle_decay (src/particle_decay.cpp) driven as
+//     FSSW::perform_resonance_feed_down does (src/FSSW.cpp:1746-1779)
 #include <cstring>
 #include <memory>
 #include <string>
@@ -12,6 +14,7 @@
 
 #include "MomentumSamplerShell.h"
 #include "ParameterReader.h"
+#include "particle_decay.h"
 #include "Random.h"
 #include "RandomVariable1DArray.h"
 #include "readindata.h"
@@ -75,6 +78,40 @@ void ref_particles(int* pid, double* mass, int* g, int* B, int* S, int* Q, int* 
         B[i] = g_parts[i].baryon; S[i] = g_parts[i].strange; Q[i] = g_parts[i].charge;
         sign[i] = g_parts[i].sign;
     }
+}
+
+// resonance feed-down of events (counts[nev] hadrons each, iSS_Hadron
+// records) with the reference's decayer and one generator for all events
+long ref_decay(const char* table_dir, int afterburner, long seed, long nev, const long* counts,
+               const void* in, long cap, long* counts_out, void* out) {
+    auto rng = std::make_shared<RandomUtil::Random>(seed);
+    const AfterburnerType ab = afterburner == 2 ? AfterburnerType::SMASH
+                               : (afterburner == 1 ? AfterburnerType::UrQMD : AfterburnerType::PDG_Decay);
+    particle_decay dec(rng, ab, table_dir);
+    const iSS_Hadron* h = static_cast<const iSS_Hadron*>(in);
+    iSS_Hadron* o = static_cast<iSS_Hadron*>(out);
+    long k = 0, w = 0;
+    for (long e = 0; e < nev; e++) {
+        std::vector<iSS_Hadron> work(h + k, h + k + counts[e]);
+        k += counts[e];
+        long n = 0;
+        for (size_t i = 0; i < work.size(); i++) {
+            std::vector<iSS_Hadron> ds;
+            iSS_Hadron m = work[i];
+            dec.perform_decays(&m, &ds);
+            for (auto& d : ds) {
+                if (dec.check_particle_stable(&d) == 1) {
+                    if (w < cap) o[w] = d;
+                    w++;
+                    n++;
+                } else {
+                    work.push_back(d);
+                }
+            }
+        }
+        counts_out[e] = n;
+    }
+    return w;
 }
 
 }  // extern "C"

@@ -53,6 +53,9 @@ def lib():
             "orc_set_cells_lab": (L, [v, L, v]),
             "orc_spectra": (None, [v, v, i, i, i, i, d, v, i, v, i, v, v, i, v]),
             "orc_flows": (None, [v, v, v, i, v, v, i, d, i, v, v]),
+            "orc_decay": (L, [s, C.c_uint, L, v, v, L, v, v]),
+            "orc_decay_table": (L, [s, v, v, v, v, L, L, v]),
+            "orc_decay_below": (L, []),
         }
         for k, (r, a) in sig.items():
             f = getattr(o, k)
@@ -188,6 +191,53 @@ def flows(dN, pT, pT_weight, phi, phi_weight, mass, to_order=9):
     return diff, inte
 
 
+PDG_FILES = {0: "pdg-s95pv1.dat", 1: "pdg-urqmd_v3.3+.dat", 2: "pdg-SMASH.dat"}
+
+
+def _decay_run(fn, counts, hadrons, cap_factor=8):
+    counts = np.ascontiguousarray(counts, dtype=np.int64)
+    h = np.ascontiguousarray(hadrons, dtype=HADRON_DTYPE)
+    cap = max(int(h.size * cap_factor), 16)
+    out = np.zeros(cap, dtype=HADRON_DTYPE)
+    cout = np.zeros(counts.size, dtype=np.int64)
+    n = fn(counts, h, cap, cout, out)
+    if n < 0:
+        raise RuntimeError("decay table missing")
+    if n > cap:
+        return _decay_run(fn, counts, hadrons, cap_factor * 4)
+    return cout, out[:n]
+
+
+def decay(counts, hadrons, seed, afterburner=1):
+    """the oracle's resonance feed-down of events (one mt19937 stream)"""
+    f = os.path.join(TABLE_DIR, PDG_FILES[afterburner]).encode()
+    return _decay_run(lambda c, h, cap, co, o: lib().orc_decay(f, seed, c.size, _p(c), _p(h), cap, _p(co), _p(o)),
+                      counts, hadrons)
+
+
+def ref_decay(counts, hadrons, seed, afterburner=1):
+    """the reference's particle_decay, driven as its feed-down loop"""
+    r = ref_lib()
+    if r is None:
+        return None
+    return _decay_run(lambda c, h, cap, co, o: r.ref_decay(TABLE_DIR.encode(), afterburner, seed, c.size, _p(c),
+                                                          _p(h), cap, _p(co), _p(o)), counts, hadrons)
+
+
+def decay_table(afterburner=1):
+    f = os.path.join(TABLE_DIR, PDG_FILES[afterburner]).encode()
+    ip = np.zeros((2000, 7), dtype=np.int32)
+    dp = np.zeros((2000, 2))
+    ci = np.zeros((20000, 6), dtype=np.int32)
+    cb = np.zeros(20000)
+    nc = C.c_long()
+    n = lib().orc_decay_table(f, _p(ip), _p(dp), _p(ci), _p(cb), 2000, 20000, C.byref(nc))
+    return dict(monval=ip[:n, 0].copy(), stable=ip[:n, 1].copy(), B=ip[:n, 2].copy(), S=ip[:n, 3].copy(),
+                Q=ip[:n, 4].copy(), ch0=ip[:n, 5].copy(), nch=ip[:n, 6].copy(), mass=dp[:n, 0].copy(),
+                width=dp[:n, 1].copy(), npart=ci[:nc.value, 0].copy(), daughters=ci[:nc.value, 1:].copy(),
+                br=cb[:nc.value].copy())
+
+
 def regulate(lab34):
     a = np.array(lab34, dtype=np.float32).reshape(-1, 34)
     lib().orc_regulate(a.shape[0], _p(a))
@@ -220,6 +270,8 @@ def ref_lib():
         r.ref_get_cells.argtypes = [v]
         r.ref_num_particles.restype = L
         r.ref_particles.argtypes = [v] * 7
+        r.ref_decay.argtypes = [s, i, L, L, v, v, L, v, v]
+        r.ref_decay.restype = L
         _r = r
     return _r
 

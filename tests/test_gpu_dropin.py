@@ -174,3 +174,51 @@ def test_dropin_calculate_vn_matches_oracle(tmp_path, historic):
     for n in range(parts["pid"].size):
         if parts["pid"][n] not in chosen:
             assert not dn[n].any()
+
+
+def _read_bin(path):
+    raw = path.read_bytes()
+    off, evs = 0, []
+    while off < len(raw):
+        (n,) = struct.unpack_from("<i", raw, off)
+        off += 4
+        ev = []
+        for _ in range(n):
+            pid = struct.unpack_from("<i", raw, off)[0]
+            vals = struct.unpack_from("<9f", raw, off + 4)
+            ev.append((pid,) + vals)
+            off += 40
+        evs.append(ev)
+    return evs
+
+
+def test_dropin_perform_decays(tmp_path):
+    """perform_decays = 1: the sampled hadrons are replaced by their stable
+    descendants (FSSW::perform_resonance_feed_down, src/FSSW.cpp:1746-1779)
+    before the writer; charge is conserved on average against the primary
+    sample of the same seed (decays conserve it, the vanishing 4-body and
+    '-2' channels of the table aside)."""
+    _run(tmp_path, "Ideal2", ["use_binary_format=1", "number_of_repeated_sampling=20", "perform_checks=0",
+                              "perform_decays=1"])
+    dec = _read_bin(tmp_path / "particle_samples.bin")
+    (tmp_path / "tests").rename(tmp_path / "t_dec")
+    for f in ("iSS_tables", "iSS_parameters_ideal.dat", "particle_samples.bin"):
+        os.remove(tmp_path / f)
+    _run(tmp_path, "Ideal2", ["use_binary_format=1", "number_of_repeated_sampling=20", "perform_checks=0"])
+    prim = _read_bin(tmp_path / "particle_samples.bin")
+    assert len(dec) == len(prim) == 20
+    t = ob.decay_table(1)
+    stable = set(t["monval"][t["stable"] == 1].tolist())
+    q = {int(m): int(t["Q"][i]) for i, m in reversed(list(enumerate(t["monval"])))}
+    nd = sum(len(e) for e in dec)
+    npr = sum(len(e) for e in prim)
+    assert nd > 1.3 * npr
+    assert all(h[0] in stable for e in dec for h in e)
+    for e in dec[:3]:
+        for h in e:
+            pid, m, tt, x, y, z, E, px, py, pz = h
+            assert abs(E * E - px * px - py * py - pz * pz - m * m) < 1e-3 * E * E + 1e-4
+    qd = np.mean([sum(q[h[0]] for h in e) for e in dec])
+    qp = np.mean([sum(q.get(h[0], 0) for h in e) for e in prim])
+    print("hadrons primary", npr, "decayed", nd, "mean charge", qp, qd)
+    assert abs(qd - qp) < 5 * np.sqrt(npr / 20) / np.sqrt(20) + 1
