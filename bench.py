@@ -16,9 +16,10 @@ plain steps' rate and the ratio are reported beside it.  Both samplers use
 the physical Fermi-Dirac |p| proposal (fermion_cdf2_quirk = 0) so that every
 hadron goes through the proposal being measured.
 
-Beside it (outside the timed region): the smooth Cooper-Frye spectra of
-pi+, K+ and p on the reference's (pT, phi, eta_s) quadrature grid over the same
-surface, FP32 and FP64.
+Beside it (outside the timed region): the resonance feed-down of a step's
+hadrons to stable particles, and the smooth Cooper-Frye spectra of pi+, K+ and
+p on the reference's (pT, phi, eta_s) quadrature grid over the same surface,
+FP32 and FP64.
 
 --workload c3 measures the batched configuration instead: 256 surfaces of
 log-normal size (median 3e4 cells, 1e3..5e5), each with its own event count
@@ -365,6 +366,19 @@ def main():
                             "k_trials": p_trials},
         "yield_GBs": y_gbs,
     }
+    # resonance feed-down of the last learned step's hadrons (outside the timed region)
+    ctx.set_decay_table(tables.decay_table(1))
+    n_in = ctx.stats().n_hadrons
+    ms_dec = []
+    for i in range(3):
+        if i:
+            ctx.sample(args.events, seed=777 + i, precision=args.precision, proposal=1)
+            n_in = ctx.stats().n_hadrons
+        n_out = ctx.decay(seed=99 + i)
+        ms_dec.append(ctx.stats().ms_decay)
+    st = ctx.stats()
+    out["decays"] = {"hadrons_in": int(n_in), "hadrons_out": int(n_out), "generations": int(st.n_decay_generations),
+                     "ms": float(min(ms_dec)), "hadrons_per_s": n_in / (1e-3 * float(min(ms_dec)))}
     if not args.no_spectra:
         out["spectra"] = spectra_extra(ctx, sp, nc)
     if not args.no_cpu:

@@ -63,7 +63,7 @@ typedef struct {
     int32_t include_shear;
     int32_t include_bulk;
     int32_t bulk_kind;
-    int32_t include_diffusion;     /* must be 0 */
+    int32_t include_diffusion;     /* baryon-diffusion delta f (needs qmu_table) */
     /* kind 11: grid of nT x nmu, values [iT * nmu + imu] */
     int32_t k11_nT, k11_nmu;
     double k11_T0, k11_mu0, k11_dT, k11_dmu;
@@ -76,6 +76,9 @@ typedef struct {
     /* keep FermionMomentumSampler::CDF_2's missing alternating sign
      * (src/FermionMomentumSampler.cpp:80-93); 1 = as the reference */
     int32_t fermion_cdf2_quirk;
+    /* diffusion: Coefficients_RTA_diffusion.dat, 150 x 100 values [iT][imu] on
+     * T = 0.05 + 0.001 iT, mu_B = 0.007892 imu (src/FSSW.cpp:1546-1566) */
+    const double* qmu_table;
 } iss_deltaf_desc;
 
 /* Hadron-resonance-gas EoS used to regulate T, mu, P from (e, n_B)

@@ -50,7 +50,7 @@ class _Deltaf(C.Structure):
                 ("k11_dT", C.c_double), ("k11_dmu", C.c_double),
                 ("k11_c0", C.c_void_p), ("k11_c1", C.c_void_p), ("k11_c2", C.c_void_p),
                 ("neos_ncol", C.c_int32), ("neos", C.c_void_p),
-                ("fermion_cdf2_quirk", C.c_int32)]
+                ("fermion_cdf2_quirk", C.c_int32), ("qmu_table", C.c_void_p)]
 
 
 class _Eos(C.Structure):
@@ -238,10 +238,10 @@ class Context:
         self.species = arrs
 
     def set_deltaf(self, shear=0, bulk=0, bulk_kind=1, k11=None, neos=None,
-                   fermion_cdf2_quirk=1):
+                   fermion_cdf2_quirk=1, diffusion=0, qmu_table=None):
         d = _Deltaf()
         d.include_shear, d.include_bulk, d.bulk_kind = int(shear), int(bulk), int(bulk_kind)
-        d.include_diffusion = 0
+        d.include_diffusion = int(diffusion)
         d.fermion_cdf2_quirk = int(fermion_cdf2_quirk)
         keep = []
         if k11 is not None:
@@ -256,6 +256,10 @@ class Context:
             keep.append(a)
             d.neos_ncol = a.shape[1]
             d.neos = _ptr(a)
+        if qmu_table is not None:
+            a = np.ascontiguousarray(qmu_table, dtype=np.float64).reshape(150, 100)
+            keep.append(a)
+            d.qmu_table = _ptr(a)
         self._check(self.lib.iss_set_deltaf(self.h, C.byref(d)), "iss_set_deltaf")
 
     # -------------------------------------------------------------- surfaces

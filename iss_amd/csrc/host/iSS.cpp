@@ -232,7 +232,15 @@ int iSS::upload_species_and_tables(std::string* err) {
     df.fermion_cdf2_quirk = static_cast<int>(paraRdr_ptr->getVal("fermion_cdf2_reference_quirk", 1));
     const bool smash = afterburner_type_ == AfterburnerType::SMASH;
     iss_io::Kind11 k11;
-    std::vector<double> neos;
+    std::vector<double> neos, qmu;
+    if (df.include_diffusion == 1) {
+        const std::string f = table_path_ + "/deltaf_tables/Coefficients_RTA_diffusion.dat";
+        if (!iss_io::read_qmu_table(f, qmu)) {
+            *err = "Can not found file: " + f;
+            return ISS_ERR_ARG;
+        }
+        df.qmu_table = qmu.data();
+    }
     if (df.include_bulk == 1 && df.bulk_kind == 11) {
         const std::string d = table_path_ + "/deltaf_tables" + (smash ? "/smash_box" : "/urqmd");
         if (!iss_io::read_kind11(d, k11)) {
@@ -477,8 +485,6 @@ int iSS::calculate_spectra_and_flows() {
     const int to_order = static_cast<int>(paraRdr_ptr->getVal("calculate_vn_to_order", 9));
     if (static_cast<int>(paraRdr_ptr->getVal("calculate_dN_dphi", 0)) == 1)
         warn("calculate_dN_dphi = 1 is not supported by this sampler; dN/dphi is not written");
-    if (static_cast<int>(paraRdr_ptr->getVal("include_deltaf_diffusion", 0)) == 1)
-        warn("include_deltaf_diffusion = 1 is not supported by this sampler; spectra without diffusion delta f");
     const int ns = static_cast<int>(species_pid_.size());
     const int npT = static_cast<int>(pT.size()), nphi = static_cast<int>(phi.size());
     std::vector<int32_t> idx(ns);

@@ -318,6 +318,18 @@ bool read_neos(const std::string& file, int ncol, std::vector<double>& rows) {
     return true;
 }
 
+bool read_qmu_table(const std::string& file, std::vector<double>& D) {
+    std::ifstream f(file);
+    if (!f.good()) return false;
+    D.assign(150 * 100, 0.);
+    for (int imu = 0; imu < 100; imu++)
+        for (int iT = 0; iT < 150; iT++) {
+            double T, mu;
+            if (!(f >> T >> mu >> D[iT * 100 + imu])) return false;
+        }
+    return true;
+}
+
 bool write_binary(const std::string& file, const std::vector<std::vector<iSS_Hadron>*>& ev) {
     FILE* o = std::fopen(file.c_str(), "wb");
     if (!o) return false;

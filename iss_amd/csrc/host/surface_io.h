@@ -47,6 +47,9 @@ struct Kind11 {
 };
 bool read_kind11(const std::string& dir, Kind11& t);
 bool read_neos(const std::string& file, int ncol, std::vector<double>& rows);
+// Coefficients_RTA_diffusion.dat (src/FSSW.cpp:1546-1566): 100 mu_B blocks of
+// 150 T rows (T, mu_B, D); returned as D[iT * 100 + imu]
+bool read_qmu_table(const std::string& file, std::vector<double>& D);
 
 // writers (src/FSSW.cpp:365-561)
 bool write_binary(const std::string& file, const std::vector<std::vector<iSS_Hadron>*>& events);

@@ -93,7 +93,7 @@ struct iss_ctx {
     // delta f
     iss_deltaf_desc df{};
     int bulk_mode = BULK_NONE, neos = -1;
-    DBuf<double> d_k11, d_neos;
+    DBuf<double> d_k11, d_neos, d_qmu, d_etab, d_dnq;
     double neos_de = 0.;
     bool tables_ready = false;
     DBuf<double> d_ktab;
@@ -270,6 +270,8 @@ DeltafDev deltaf_dev(iss_ctx* ctx) {
     DeltafDev d{};
     d.bulk_kind = ctx->df.bulk_kind;
     d.include_bulk = ctx->df.include_bulk;
+    d.include_diffusion = ctx->df.include_diffusion;
+    d.qmu_tab = ctx->d_qmu.p;
     d.neos = ctx->neos;
     d.k11_nT = ctx->df.k11_nT;
     d.k11_nmu = ctx->df.k11_nmu;
@@ -286,13 +288,13 @@ DeltafDev deltaf_dev(iss_ctx* ctx) {
 
 int cell_prep(iss_ctx* ctx) {
     if (ctx->Nneg > 0) {
-        HIPCHK(ctx->d_coef_neg.ensure(6 * ctx->Nneg));
+        HIPCHK(ctx->d_coef_neg.ensure(kCoefRows * ctx->Nneg));
         k_cell_prep<<<(ctx->Nneg + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cells_neg.p, ctx->Nneg, deltaf_dev(ctx),
                                                                        ctx->d_coef_neg.p);
         HIPCHK(hipGetLastError());
     }
     if (ctx->Nc == 0) return ISS_OK;
-    HIPCHK(ctx->d_coef.ensure(6 * ctx->Nc));
+    HIPCHK(ctx->d_coef.ensure(kCoefRows * ctx->Nc));
     const int64_t nb = (ctx->Nc + 255) / 256;
     k_cell_prep<<<nb, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->Nc, deltaf_dev(ctx), ctx->d_coef.p);
     HIPCHK(ctx->d_aux.ensure(static_cast<size_t>(kAuxFields) * ctx->Nc));
@@ -378,11 +380,18 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     a.ctot = ctx->d_ctot.p;
     a.raw = raw ? static_cast<void*>(ctx->d_raw.p) : nullptr;
     a.partial = ctx->d_partial.p;
+    a.dnq = nullptr;
     HIPCHK(ctx->d_oob.ensure(1));
     a.oob = ctx->d_oob.p;
     HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     int nblk = 1;
+    if (Nc > 0 && ctx->df.include_diffusion) {
+        HIPCHK(ctx->d_dnq.ensure(static_cast<size_t>(Nc) * Ns));
+        k_diffusion_dn<<<ctx->num_cu * 8, 256, 0, ctx->stream>>>(a, ctx->d_etab.p, ctx->d_dnq.p);
+        HIPCHK(hipGetLastError());
+        a.dnq = ctx->d_dnq.p;
+    }
     if (Nc > 0) {
         int rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, false) : launch_yield<float>(ctx, a, raw, false);
         if (rc) return set_err(ctx, rc, "yield kernel launch failed");
@@ -432,6 +441,7 @@ SampleArgs base_args(iss_ctx* ctx) {
     a.shear = ctx->df.include_shear;
     a.bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
     a.neos = ctx->neos;
+    a.diff = ctx->df.include_diffusion;
     a.max_trials = 5000;
     return a;
 }
@@ -528,10 +538,11 @@ int run_spectra(iss_ctx* ctx, const iss_spectra_opts* o, const int32_t* species,
     const int bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
     if (Nk > 0)
         k_spectra_rec<<<(Nk + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->d_coef.p, Nk, shear, bulk,
-                                                                   ctx->neos, rec);
+                                                                   ctx->neos, ctx->df.include_diffusion, rec);
     if (Nd > 0)
         k_spectra_rec<<<(Nd + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_cells_neg.p, ctx->d_coef_neg.p, Nd, shear,
-                                                                   bulk, ctx->neos, rec + Nk * kSpecRec);
+                                                                   bulk, ctx->neos, ctx->df.include_diffusion,
+                                                                   rec + Nk * kSpecRec);
     HIPCHK(hipGetLastError());
     // grids: pT, cos phi, sin phi, cosh, sinh, weight of the y - eta_s nodes
     // (mode 0: y - eta_s = -eta[k]) or of the rapidities (mode 1)
@@ -645,7 +656,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_spart.release(); ctx->d_sout.release(); ctx->d_sspec.release();
     ctx->d_surf_off.release(); ctx->d_bevents.release(); ctx->d_bround.release(); ctx->d_bevcount.release();
     ctx->d_bbase.release(); ctx->d_perms.release(); ctx->d_seg.release(); ctx->d_lam.release();
-    ctx->d_dec_int.release(); ctx->d_dec_dbl.release();
+    ctx->d_dec_int.release(); ctx->d_dec_dbl.release(); ctx->d_qmu.release(); ctx->d_etab.release(); ctx->d_dnq.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -691,7 +702,17 @@ int iss_set_species(iss_ctx* ctx, const iss_species_desc* sp) {
 int iss_set_deltaf(iss_ctx* ctx, const iss_deltaf_desc* d) {
     if (!ctx || !d) return ISS_ERR_ARG;
     (void)hipSetDevice(ctx->device);
-    if (d->include_diffusion) return set_err(ctx, ISS_ERR_UNSUPPORTED, "baryon-diffusion delta f is not supported");
+    if (d->include_diffusion) {
+        if (!d->qmu_table) return set_err(ctx, ISS_ERR_ARG, "diffusion delta f needs its coefficient table");
+        HIPCHK(ctx->d_qmu.ensure(150 * 100));
+        HIPCHK(hipMemcpyAsync(ctx->d_qmu.p, d->qmu_table, sizeof(double) * 150 * 100, hipMemcpyHostToDevice,
+                              ctx->stream));
+        if (!ctx->d_etab.p) {
+            HIPCHK(ctx->d_etab.ensure(static_cast<size_t>(kEnTerms) * kKLen));
+            k_expint_table<<<(kKLen + 127) / 128, 128, 0, ctx->stream>>>(ctx->d_etab.p);
+            HIPCHK(hipGetLastError());
+        }
+    }
     int mode = BULK_NONE, neos = -1;
     const int k = d->bulk_kind;
     if (k == 21) neos = 1;
@@ -724,6 +745,7 @@ int iss_set_deltaf(iss_ctx* ctx, const iss_deltaf_desc* d) {
     ctx->df = *d;
     ctx->df.k11_c0 = ctx->df.k11_c1 = ctx->df.k11_c2 = nullptr;
     ctx->df.neos = nullptr;
+    ctx->df.qmu_table = nullptr;
     ctx->bulk_mode = mode;
     ctx->neos = neos;
     int rc = build_ptables(ctx, d->fermion_cdf2_quirk ? 1 : 0);
@@ -903,6 +925,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.shear = ctx->df.include_shear;
     a.bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
     a.neos = ctx->neos;
+    a.diff = ctx->df.include_diffusion;
     a.boost_invariant = binv ? 1 : 0;
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);

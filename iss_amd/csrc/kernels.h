@@ -2,7 +2,8 @@
 //
 // HBM layout (one context):
 //   cells   float  [kCellFields][Nc]  rest-frame surface, field-major (SoA)
-//   coef    double [6][Nc]            per-cell delta-f coefficients
+//   coef    double [8][Nc]            per-cell delta-f coefficients (6 bulk /
+//                                     NEoS, 1 / D_qmu, n_B / (e + P))
 //   cdf     double [Nc][Ns]           per-cell inclusive species prefix of the
 //                                     clamped yields (the species-selection CDF)
 //   ctot    double [Nc]               per-cell total yield (last cdf entry)
@@ -31,6 +32,9 @@ enum CellField {
 constexpr double kKx0 = 0.5, kKdx = 0.05, kKxmax = 400.;
 constexpr int kKLen = 7991;
 constexpr int kKLds = 1024;      // rows staged in LDS (args up to 51.65)
+
+constexpr int kEnTerms = 9;     // E_2, E_4, ..., E_18 (src/FSSW.cpp:1611, 1637-1642)
+constexpr int kCoefRows = 8;    // per-cell coefficients: 6 bulk / NEoS, 1 / D_qmu, n_B / (e + P)
 
 constexpr int kMaxSpecies = 512;
 constexpr int kMaxChunks = kMaxSpecies / kWave;
@@ -71,6 +75,7 @@ struct YieldArgs {
     void* raw;       // optional per-(cell, species) yields in working precision
     double* partial; // [gridDim.x][Ns]
     int* oob;        // set when a Bessel argument leaves the table
+    const double* dnq;   // [Nc][Ns] diffusion corrections (k_diffusion_dn); null: none
 };
 
 struct SampleArgs {
@@ -91,6 +96,7 @@ struct SampleArgs {
     uint32_t chunk;
     Permutation perm;
     int shear, bulk, neos;   // neos: -1 none, 1 CE, 0 22-moment
+    int diff;                // diffusion delta f
     int boost_invariant;
     float yLB, yRB;
     int max_trials;

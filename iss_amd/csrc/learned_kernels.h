@@ -188,7 +188,9 @@ __device__ inline void lparams_of(const SampleArgs& a, const float* __restrict__
             P.q2 = Pi * (cf1 - cf2);
         }
     }
-    if ((sign < 0 && q.x < 1e-3) || (a.pt.quirk && tb >= 3)) P.flags |= 2;
+    // the envelope does not bound the diffusion term: those hadrons take the
+    // plain sampler
+    if ((sign < 0 && q.x < 1e-3) || (a.pt.quirk && tb >= 3) || a.diff) P.flags |= 2;
 }
 
 // The same parameters for the trial kernel, in FP32, given x (from the

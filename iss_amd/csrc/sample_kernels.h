@@ -343,6 +343,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     float Bq = 0.f, Sq = 0.f, Qq = 0.f;
     float pixx = 0.f, pixy = 0.f, pixz = 0.f, piyy = 0.f, piyz = 0.f;
     float cf0 = 0.f, cf1 = 0.f, cf2 = 0.f, cf3 = 0.f, cf4 = 0.f, cf5 = 0.f;
+    float qx = 0.f, qy = 0.f, qz = 0.f, pqm = 0.f;  // diffusion: q / D_qmu, n_B / (e + P)
 
     while (true) {
         const unsigned long long need = __ballot(!active);
@@ -377,6 +378,13 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
                 if (a.bulk != BULK_NONE || a.neos >= 0) {
                     cf0 = a.coef[c]; cf1 = a.coef[Nc + c]; cf2 = a.coef[2 * Nc + c];
                     cf3 = a.coef[3 * Nc + c]; cf4 = a.coef[4 * Nc + c]; cf5 = a.coef[5 * Nc + c];
+                }
+                if (a.diff) {
+                    const float iq = static_cast<float>(a.coef[6 * Nc + c]);
+                    qx = a.cells[F_QX * Nc + c] * iq;
+                    qy = a.cells[F_QY * Nc + c] * iq;
+                    qz = a.cells[F_QZ * Nc + c] * iq;
+                    pqm = static_cast<float>(a.coef[7 * Nc + c]);
                 }
                 it = 0;
                 trials = 1;
@@ -472,8 +480,13 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
             } else if (a.bulk == BULK_22MOM) {
                 dfb = omf * Pi * (fm * fm * cf2 + p0 * (Bq * cf3 + Sq * cf4 + Qq * cf5) + p0 * p0 * (cf1 - cf2));
             }
+            float dfq = 0.f;
+            if (a.diff) {
+                // (src/FSSW.cpp:1931-1937)
+                dfq = omf * (pqm - Bq / p0) * (-(px * qx + py * qy + pz * qz));
+            }
             const float w1 = fminf(fmaxf(pds / p0 / dsfac, 0.f), 1.f);
-            const float w2 = fminf(fmaxf((1.f + dfs + dfb) * 0.5f, 0.f), 1.f);
+            const float w2 = fminf(fmaxf((1.f + dfs + dfb + dfq) * 0.5f, 0.f), 1.f);
             const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
             const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
             if (ua < w1 * w2) {

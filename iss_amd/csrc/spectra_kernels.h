@@ -29,7 +29,7 @@
 
 namespace issdev {
 
-constexpr int kSpecRec = 32;
+constexpr int kSpecRec = 36;
 enum SpecSlot {
     RS_IT = 0,                          // 1 / T
     RS_MB, RS_MS, RS_MQ,                // mu_B / T, mu_S / T, mu_Q / T
@@ -40,12 +40,13 @@ enum SpecSlot {
     RS_BM2, RS_BE0, RS_BB, RS_BS, RS_BQ, RS_BE2, RS_BM2E,
     // bulk: omf * (BM2 m^2 + (BE0 + B BB + S BS + Q BQ) E + BE2 E^2 + BM2E m^2 / E)
     RS_CHE, RS_SHE,                     // cosh, sinh of the cell's eta_s
+    RS_QX, RS_QY, RS_QZ, RS_PQ,         // diffusion: q / D_qmu, n_B / (e + P)
     kSpecUsed
 };
 static_assert(kSpecUsed <= kSpecRec, "spectra record");
 
 __global__ void k_spectra_rec(const float* __restrict__ cells, const double* __restrict__ coef, int64_t N,
-                              int shear, int bulk, int neos, double* __restrict__ rec) {
+                              int shear, int bulk, int neos, int diff, double* __restrict__ rec) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double r[kSpecRec];
@@ -99,6 +100,14 @@ __global__ void k_spectra_rec(const float* __restrict__ cells, const double* __r
     const double eta = cells[F_ETA * N + i];
     r[RS_CHE] = cosh(eta);
     r[RS_SHE] = sinh(eta);
+    if (diff) {
+        // (src/emissionfunction.cpp:777-786): (1 - s f0) (n_B/(e+P) - B/E) p.q / D_qmu
+        const double iq = coef[6 * N + i];
+        r[RS_QX] = cells[F_QX * N + i] * iq;
+        r[RS_QY] = cells[F_QY * N + i] * iq;
+        r[RS_QZ] = cells[F_QZ * N + i] * iq;
+        r[RS_PQ] = coef[7 * N + i];
+    }
     for (int k = 0; k < kSpecRec; k++) rec[i * kSpecRec + k] = r[k];
 }
 
@@ -137,6 +146,7 @@ __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(
 template <typename R>
 struct SpecCell {
     R xx, xy2, xz2, yy, yz2, sa, sb, cm, be, be2, cme, sg, lo, hi;
+    R qx, qy, qz, pq, bq;   // diffusion (zero when off)
 };
 
 // f0 p.dsigma (1 + delta f) from the rest-frame energy E, momentum (lx, ly,
@@ -152,7 +162,8 @@ __device__ __forceinline__ R spectra_integrand(const SpecCell<R>& c, R E, R lx, 
     const R lz2 = lz * lz;
     const R W = c.xx * (lx * lx - lz2) + c.yy * (ly * ly - lz2) + lx * (c.xy2 * ly + c.xz2 * lz) + c.yz2 * ly * lz;
     const R iE = frcp(E);
-    R df = omf * (W * (c.sa + c.sb * iE) + c.cm + E * (c.be + c.be2 * E) + c.cme * iE);
+    R df = omf * (W * (c.sa + c.sb * iE) + c.cm + E * (c.be + c.be2 * E) + c.cme * iE -
+                  (c.pq - c.bq * iE) * (lx * c.qx + ly * c.qy + lz * c.qz));
     df = fmin(fmax(df, c.lo), c.hi);
     R r = f0 * pds * (R(1) + df);
     if (positive_only && r < R(0)) r = R(0);
@@ -198,6 +209,7 @@ __global__ __launch_bounds__(64) void k_spectra(SpectraArgs a) {
     }
     SpecCell<R> c;
     c.sg = static_cast<R>(a.sp.sign[s]);
+    c.bq = static_cast<R>(B);
     const R inf = static_cast<R>(INFINITY);
     c.hi = a.restrict_ratio > 0. ? static_cast<R>(a.restrict_ratio) : inf;
     c.lo = -c.hi;
@@ -219,6 +231,10 @@ __global__ __launch_bounds__(64) void k_spectra(SpectraArgs a) {
         c.be = static_cast<R>(v[RS_BE0] + B * v[RS_BB] + S * v[RS_BS] + Q * v[RS_BQ]);
         c.be2 = static_cast<R>(v[RS_BE2]);
         c.cme = static_cast<R>(v[RS_BM2E] * m * m);
+        c.qx = static_cast<R>(v[RS_QX]);
+        c.qy = static_cast<R>(v[RS_QY]);
+        c.qz = static_cast<R>(v[RS_QZ]);
+        c.pq = static_cast<R>(v[RS_PQ]);
         if (a.mode == 0) {
             R mTr[kSpecPts], Pe[kSpecPts], Px[kSpecPts], Py[kSpecPts], Pz[kSpecPts], Pds[kSpecPts], X0[kSpecPts];
             R cell[kSpecPts];

@@ -2,6 +2,7 @@
 // (one translation unit: included by iss_hip.hip)
 #pragma once
 
+#include "expint.h"
 #include "kernels.h"
 
 namespace issdev {
@@ -225,6 +226,8 @@ __global__ void k_soa_to_aos(const float* __restrict__ cells, int64_t n, float* 
 
 struct DeltafDev {
     int bulk_kind, include_bulk, neos;  // neos: -1, 0 (22-mom), 1 (CE)
+    int include_diffusion;
+    const double* qmu_tab;  // [150][100] diffusion coefficients [iT][imu]
     int k11_nT, k11_nmu;
     double k11_T0, k11_mu0, k11_dT, k11_dmu;
     const double* k11;   // [3][nT*nmu]
@@ -306,7 +309,33 @@ __device__ void coef_neos(const DeltafDev& d, double Ed, double nB, double c[6])
     }
 }
 
-// Per-cell coefficients as the yield loop selects them (src/FSSW.cpp:597-620)
+// get_deltaf_qmu_coeff (src/FSSW.cpp:1569-1606): bilinear in (T, mu_B) on
+// T = 0.05 + 0.001 i (150 rows), mu_B = 0.007892 j (100 columns); 1e30
+// (no diffusion correction) outside the table
+__device__ double coef_qmu(const DeltafDev& d, double T, double muB) {
+    const double T0 = 0.05, dT = 0.001, mu0 = 0.0, dmu = 0.007892;
+    const int nT = 150, nmu = 100;
+    const int iT = static_cast<int>((T - T0) / dT);
+    const int imu = static_cast<int>((muB - mu0) / dmu);
+    const double xf = (T - T0) / dT - iT, yf = (muB - mu0) / dmu - imu;
+    if (imu > nmu - 2 || iT > nT - 2 || imu < 0 || iT < 0) return 1e30;
+    const double* t = d.qmu_tab;
+    const double f1 = t[iT * nmu + imu], f2 = t[iT * nmu + imu + 1];
+    const double f3 = t[(iT + 1) * nmu + imu + 1], f4 = t[(iT + 1) * nmu + imu];
+    return f1 * (1. - xf) * (1. - yf) + f2 * (1. - xf) * yf + f3 * xf * yf + f4 * xf * (1. - yf);
+}
+
+// E_2, E_4, ..., E_18 on the K-table grid (src/FSSW.cpp:1637-1642)
+__global__ void k_expint_table(double* etab) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kKLen) return;
+    const double x = kKx0 + i * kKdx;
+    etab[i] = expint_en(2, x);
+    for (int k = 1; k < kEnTerms; k++) etab[k * kKLen + i] = expint_en(2 * k + 2, x);
+}
+
+// Per-cell coefficients as the yield loop selects them (src/FSSW.cpp:597-637);
+// rows 6, 7: 1 / deltaf_qmu_coeff and n_B / (e + P) of the diffusion delta f
 __global__ void k_cell_prep(const float* __restrict__ cells, int64_t Nc, DeltafDev d,
                             double* __restrict__ coef) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -320,6 +349,13 @@ __global__ void k_cell_prep(const float* __restrict__ cells, int64_t Nc, DeltafD
         else if (d.bulk_kind == 1) coef_rta(T, c);
     }
     for (int k = 0; k < 6; k++) coef[k * Nc + i] = c[k];
+    double iq = 0., pq = 0.;
+    if (d.include_diffusion) {
+        iq = 1. / coef_qmu(d, T, muB);
+        pq = nB / (Ed + static_cast<double>(cells[F_P * Nc + i]));
+    }
+    coef[6 * Nc + i] = iq;
+    coef[7 * Nc + i] = pq;
 }
 
 

@@ -77,10 +77,42 @@ __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x
 __device__ __forceinline__ double frcp(double x) { return 1. / x; }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// E_2, ..., E_18 at x: the table's linear interpolation, exact outside it
+// (src/FSSW.cpp:1700-1721)
+__device__ __forceinline__ void en_lookup(const double* etab, double x, double* e) {
+    if (x < kKx0 || x > kKxmax - kKdx) {
+        e[0] = expint_en(2, x);
+        for (int k = 1; k < kEnTerms; k++) e[k] = expint_en(2 * k + 2, x);
+        return;
+    }
+    const int idx = static_cast<int>((x - kKx0) / kKdx);
+    const double fr = (x - kKx0 - idx * kKdx) / kKdx;
+    for (int k = 0; k < kEnTerms; k++)
+        e[k] = (1. - fr) * etab[k * kKLen + idx] + fr * etab[k * kKLen + idx + 1];
+}
+
+// the diffusion series term I_1 of calculate_dN_analytic (src/FSSW.cpp:
+// 785-809) at x = n m / T, before the factor -(m / T)^3
+__device__ __forceinline__ double diffusion_i1(const double* etab, double x) {
+    double e[kEnTerms];
+    en_lookup(etab, x, e);
+    double v = exp(-x) / x * (2. / (x * x) + 2. / x - 0.5) + 3. / 8. * e[0];
+    double dfact = 1., fact = 2., p2 = 4.;
+    for (int k = 3; k <= kEnTerms + 1; k++) {
+        dfact *= (2 * k - 5);
+        fact *= k;
+        p2 *= 2.;
+        v += 3. * dfact / p2 / fact * e[k - 2];
+    }
+    return v;
+}
+
+// dnq: the diffusion correction of this (cell, species), computed by
+// k_diffusion_dn (0 without diffusion delta f)
 template <typename R, int MODE, typename KL>
 __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, R T, R beta,
                                        R muB, R muS, R muQ, R dsu, R Pi, const R* c,
-                                       const KL& kl, bool* ill) {
+                                       const KL& kl, bool* ill, R dnq = R(0)) {
     const R mu = Bq * muB + Sq * muS + Qq * muQ;
     const R lambda = fexp(beta * mu);
     const int nt = (mass < R(0.7) && T > R(0.05)) ? 10 : 1;
@@ -128,8 +160,9 @@ __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, 
         const R b1 = m2 * T * r1, b2 = m2 * T2 * r2, b3 = m2 * mass * T2 * r3;
         dN = pd * Pi * (b1 * m2 * c[2] + b2 * (Bq * c[3] + Sq * c[4] + Qq * c[5]) + b3 * (c[1] - c[2]));
     }
+    dN += dnq;
     const R N = Neq + dN;
-    // ill-conditioned: the bulk correction nearly cancels N_eq
+    // ill-conditioned: the corrections nearly cancel N_eq
     if (ill) *ill = (fabs(Neq) + fabs(dN)) > R(4) * fabs(N);
     return N > R(0) ? N : R(0);
 }
@@ -153,7 +186,7 @@ __device__ double yield_fp64_global(const YieldArgs& a, int64_t c, int s) {
     return yield_one<double, MODE>(a.sp.mass[s], pref, a.sp.B[s], a.sp.S[s], a.sp.Q[s], a.sp.sign[s], T,
                                    1. / T, (double)a.cells[F_MUB * Nc + c], (double)a.cells[F_MUS * Nc + c],
                                    (double)a.cells[F_MUQ * Nc + c], (double)a.cells[F_DA0 * Nc + c], Pi, cd,
-                                   k64, (bool*)nullptr);
+                                   k64, (bool*)nullptr, a.dnq ? a.dnq[c * a.Ns + s] : 0.);
 }
 
 constexpr int kYieldThreads = 256;
@@ -263,17 +296,18 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a) {
                 const R* q0 = s_cell + cl;
                 const R p[13] = {q0[0], q0[TC], q0[2 * TC], q0[3 * TC], q0[4 * TC], q0[5 * TC], q0[6 * TC],
                                  q0[7 * TC], q0[8 * TC], q0[9 * TC], q0[10 * TC], q0[11 * TC], q0[12 * TC]};
+                const R dnq = a.dnq ? static_cast<R>(a.dnq[(c0 + cl) * Ns + s]) : R(0);
                 const char4 q = s_q[s];
                 if constexpr (sizeof(R) == sizeof(double)) {
                     const KLookup64<EXACT> kl{reinterpret_cast<const double*>(s_k), a.ktab, a.oob};
                     y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, p[0], p[1], p[2],
-                                           p[3], p[4], p[5], p[6], p + 7, kl, (bool*)nullptr);
+                                           p[3], p[4], p[5], p[6], p + 7, kl, (bool*)nullptr, dnq);
                 } else {
                     const KLookup32<EXACT> kl{reinterpret_cast<const float2*>(s_k), a.ktabf, a.oob};
                     bool ill = false;
                     y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, p[0], p[1], p[2],
                                            p[3], p[4], p[5], p[6], p + 7, kl,
-                                           MODE == BULK_NONE ? (bool*)nullptr : &ill);
+                                           MODE == BULK_NONE ? (bool*)nullptr : &ill, dnq);
                     if (MODE != BULK_NONE && ill) {
                         const int k = atomicAdd(&s_fix[0], 1);
                         if (k < kFixupCap) s_fix[1 + k] = (cl << 16) | s;
@@ -347,5 +381,55 @@ __global__ void k_species_reduce(const double* __restrict__ partial, int nblk, i
     if (threadIdx.x == 0) out[s] = red[0];
 }
 
+
+// dsigma . q as the reference forms it from its float records
+// (src/FSSW.cpp:627-629): a float expression, each product and sum rounded
+// to float (a fused multiply-add would round differently, and the yield
+// amplifies that where its terms cancel)
+__device__ __forceinline__ float dot3_unfused(float a0, float b0, float a1, float b1, float a2, float b2) {
+#pragma clang fp contract(off)
+    return a0 * b0 + a1 * b1 + a2 * b2;
+}
+
+// The diffusion correction of every (cell, species) yield, FP64
+// (src/FSSW.cpp:622-637, 696-702, 783-829):
+//   dN_qmu = g / (2 pi^2 hbar c^3) (dsigma . q) / D_qmu
+//            (-n_B/(e+P) m^2 T^2 S_1 - B T^3 / 3 S_2),
+//   S_1 = sum_n theta_n / n lambda^n K_2(n m / T),
+//   S_2 = sum_n n theta_n lambda^n I_1(n m / T) (-(m / T)^3),
+// with the equilibrium series' truncation (10 terms below 0.7 GeV).
+__global__ void k_diffusion_dn(YieldArgs a, const double* __restrict__ etab, double* __restrict__ dnq) {
+    const int64_t Nc = a.Nc;
+    const int Ns = a.Ns;
+    const int64_t items = Nc * Ns;
+    const KLookup64<true> kl{nullptr, a.ktab, a.oob};
+    const double uf = 1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi);
+    for (int64_t it = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; it < items;
+         it += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t c = it / Ns;
+        const int s = static_cast<int>(it % Ns);
+        const double T = a.cells[F_T * Nc + c], beta = 1. / T;
+        const float dqf = dot3_unfused(a.cells[F_QX * Nc + c], a.cells[F_DA1 * Nc + c], a.cells[F_QY * Nc + c],
+                                       a.cells[F_DA2 * Nc + c], a.cells[F_QZ * Nc + c], a.cells[F_DA3 * Nc + c]);
+        const double dq = static_cast<double>(dqf) * a.coef[6 * Nc + c];
+        const double pq = a.coef[7 * Nc + c];
+        const double m = a.sp.mass[s];
+        const double B = a.sp.B[s], S = a.sp.S[s], Q = a.sp.Q[s], sg = a.sp.sign[s];
+        const double mu = B * a.cells[F_MUB * Nc + c] + S * a.cells[F_MUS * Nc + c] + Q * a.cells[F_MUQ * Nc + c];
+        const double lambda = exp(beta * mu);
+        const int nt = (m < 0.7 && T > 0.05) ? 10 : 1;
+        const double mb = m * beta;
+        double s1 = 0., s2 = 0., th = 1., fug = lambda;
+        for (int n = 1; n <= nt; n++) {
+            const double arg = n * m * beta;
+            s1 += th / n * fug * kl(2, arg);
+            s2 += n * th * fug * (-(mb * mb * mb) * diffusion_i1(etab, arg));
+            th *= -sg;
+            fug *= lambda;
+        }
+        const double pref = static_cast<double>(a.sp.g[s]) * uf;
+        dnq[it] = pref * dq * (-pq * (m * m * T * T * s1) - B * (T * T * T / 3.) * s2);
+    }
+}
 
 }  // namespace issdev

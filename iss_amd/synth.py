@@ -101,7 +101,8 @@ def _lrf_shear(rng, n, epp, frac_max=0.3):
     return comp * scale[:, None]
 
 
-def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=True, shear=True, grid=None):
+def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=True, shear=True, grid=None,
+                diffusion=0.0):
     """C2-style rest-frame surface (28 floats per cell) and the area scale."""
     rng = np.random.default_rng(seed)
     n = int(n_cells)
@@ -134,6 +135,11 @@ def viscous_lrf(n_cells, sp, seed=2, muB_max=0.3, target_hadrons=1.0e4, bulk=Tru
         c[:, 19] = -rng.uniform(0, 0.2, n) * epp
     if shear:
         c[:, 20:25] = _lrf_shear(rng, n, epp)
+    if diffusion > 0:
+        # rest-frame baryon diffusion current, |q| / n_B <= diffusion
+        q = rng.normal(0, 1, (n, 3))
+        q *= (diffusion * rng.uniform(0, 1, n) * np.maximum(nB, 1e-3) / np.linalg.norm(q, axis=1))[:, None]
+        c[:, 25:28] = q
     # area: Poisson mean per cell ~ n_eq * A; choose A for the target
     dens = P / T  # Boltzmann number density [1/fm^3]
     w = rng.uniform(0.5, 1.5, n)
@@ -184,7 +190,7 @@ def _boost_to_lab(u):
 
 
 def viscous_lab(n_cells, sp, seed=7, muB_max=0.2, shear=True, bulk=True, frac_spacelike=0.3,
-                eta_max=1.5, area=1.0):
+                eta_max=1.5, area=1.0, diffusion=0.0):
     """Lab-frame MUSIC records (34 floats, FO_surf order) of a viscous
     surface in Milne coordinates.  Components in the local orthonormal frame
     (tau, x, y, eta): u3 = tau u^eta, da3 the eta_s component (p.dsigma =
@@ -233,6 +239,15 @@ def viscous_lab(n_cells, sp, seed=7, muB_max=0.2, shear=True, bulk=True, frac_sp
             c[:, 19 + k] = Pm[:, a, b]
     if bulk:
         c[:, 29] = -rng.uniform(0, 0.15, n) * epp
+    if diffusion > 0:
+        # baryon diffusion current orthogonal to u: (0, q) in the rest frame,
+        # |q| / n_B <= diffusion, boosted to the lab
+        ql = np.zeros((n, 4))
+        ql[:, 1:] = rng.normal(0, 1, (n, 3))
+        ql[:, 1:] *= (diffusion * rng.uniform(0, 1, n) * np.maximum(nB, 1e-3) /
+                      np.linalg.norm(ql[:, 1:], axis=1))[:, None]
+        qm = np.einsum("nab,nb->na", _boost_to_lab(u), ql)
+        c[:, 30:34] = qm
     return c.astype(np.float32)
 
 

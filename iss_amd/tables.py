@@ -165,6 +165,14 @@ def read_eos(iEOS: int, afterburner: int, table_dir: str = TABLE_DIR) -> dict:
     return {"iEOS": iEOS, "rows": rows}
 
 
+def read_qmu_table(table_dir: str = TABLE_DIR) -> np.ndarray:
+    """Coefficients_RTA_diffusion.dat as FSSW::load_deltaf_qmu_coeff_table
+    reads it (src/FSSW.cpp:1546-1566): rows (T, mu_B, D), mu_B outer and T
+    inner; returned as [150 T][100 mu_B]."""
+    v = np.loadtxt(os.path.join(table_dir, "deltaf_tables", "Coefficients_RTA_diffusion.dat"))
+    return np.ascontiguousarray(v[:, 2].reshape(100, 150).T)
+
+
 def read_kind11(afterburner: int = 1, table_dir: str = TABLE_DIR) -> dict:
     """OSU 14-moment bulk coefficients c0, c1, c2 on a (T, muB) grid, all three
     loaded (the reference's c0 read fails, src/FSSW.cpp:1246-1278)."""

@@ -384,7 +384,7 @@ double bessel_kn(int n, double x) {
 
 // FSSW::initialize_special_function_arrays (src/FSSW.cpp:1609-1643); the K1
 // and K3 columns are zero unless bulk delta f is on (src/FSSW.cpp:1625-1631).
-void SpecialFunctions::init(bool with_bulk) {
+void SpecialFunctions::init(bool with_bulk, bool with_diffusion) {
     length = static_cast<int>((x_max - x_min) / dx) + 1;
     K1.assign(length, 0.);
     K2.assign(length, 0.);
@@ -397,6 +397,38 @@ void SpecialFunctions::init(bool with_bulk) {
         }
         K2[i] = bessel_kn(2, x);
     }
+    En.clear();
+    if (with_diffusion) {
+        En.assign(9 * static_cast<size_t>(length), 0.);
+        for (int i = 0; i < length; i++) {
+            const double x = x_min + i * dx;
+            En[i] = expint_en(2, x);
+            for (int k = 1; k < 9; k++) En[k * length + i] = expint_en(2 * k + 2, x);
+        }
+    }
+}
+
+void SpecialFunctions::E(double arg, double out[9]) const {
+    if (arg < x_min || arg > x_max - dx || En.empty()) {
+        out[0] = expint_en(2, arg);
+        for (int k = 1; k < 9; k++) out[k] = expint_en(2 * k + 2, arg);
+        return;
+    }
+    const int idx = static_cast<int>((arg - x_min) / dx);
+    const double fr = (arg - x_min - idx * dx) / dx;
+    for (int k = 0; k < 9; k++) out[k] = (1. - fr) * En[k * length + idx] + fr * En[k * length + idx + 1];
+}
+
+// E_n(x) = exp(-x) / x int_0^inf exp(-u) (1 + u/x)^-n du, u = e^v, trapezoid
+// in v on [-40, 6] (the integrand decays like e^v and exp(-e^v) at the ends)
+double expint_en(int n, double x) {
+    const double h = 0.05;
+    double sum = 0.;
+    for (double v = -40.; v <= 6.; v += h) {
+        const double u = std::exp(v);
+        sum += std::exp(-u + v - n * std::log1p(u / x));
+    }
+    return std::exp(-x) / x * sum * h;
 }
 
 // get_special_function_K1/K2/K3 (src/FSSW.cpp:1646-1703)
@@ -444,6 +476,31 @@ bool DeltafTables::load_kind11(const std::string& dir) {
 
 // load_CE_deltaf_NEOSBQS_table / load_22mom_deltaf_NEOSBQS_table
 // (src/FSSW.cpp:1304-1376): 200 x 200 rows after one header line.
+// rows (T, mu_B, D) with mu_B outer and T inner
+bool DeltafTables::load_qmu(const std::string& file) {
+    std::ifstream f(file);
+    if (!f.good()) return false;
+    qmu.assign(150 * 100, 0.);
+    for (int j = 0; j < 100; j++)
+        for (int i = 0; i < 150; i++) {
+            double a, b, c;
+            if (!(f >> a >> b >> c)) return false;
+            qmu[i * 100 + j] = c;
+        }
+    return true;
+}
+
+double DeltafTables::qmu_coeff(double T, double muB) const {
+    const double T0 = 0.05, mu0 = 0.0, dT = 0.001, dmu = 0.007892;
+    const int iT = static_cast<int>((T - T0) / dT);
+    const int imu = static_cast<int>((muB - mu0) / dmu);
+    const double xf = (T - T0) / dT - iT, yf = (muB - mu0) / dmu - imu;
+    if (imu > 100 - 2 || iT > 150 - 2 || imu < 0 || iT < 0) return 1e30;
+    const double f1 = qmu[iT * 100 + imu], f2 = qmu[iT * 100 + imu + 1];
+    const double f3 = qmu[(iT + 1) * 100 + imu + 1], f4 = qmu[(iT + 1) * 100 + imu];
+    return f1 * (1. - xf) * (1. - yf) + f2 * (1. - xf) * yf + f3 * xf * yf + f4 * xf * (1. - yf);
+}
+
 bool DeltafTables::load_neos(const std::string& file, int ncol) {
     std::ifstream f(file);
     if (!f.good()) return false;
@@ -695,7 +752,11 @@ bool FSSWOracle::setup(const std::string& dir, const Config& c, std::string* err
     particles = read_resonances(dir + pdg, cfg.quantum_statistics == 1);
     if (particles.empty()) { if (err) *err = "pdg table missing"; return false; }
     order = sampling_order(particles, read_chosen(dir + chosen));
-    if (cfg.include_diffusion) { if (err) *err = "diffusion delta f not restated"; return false; }
+    if (cfg.include_diffusion &&
+        !dft.load_qmu(dir + "/deltaf_tables/Coefficients_RTA_diffusion.dat")) {
+        if (err) *err = "diffusion table missing";
+        return false;
+    }
     if (cfg.include_bulk && cfg.bulk_kind == 11) {
         // kind 11 reads /smash_box for SMASH (src/FSSW.cpp:1216-1219)
         const std::string d = dir + "/deltaf_tables" + (cfg.afterburner == 2 ? "/smash_box" : "/urqmd");
@@ -710,7 +771,7 @@ bool FSSWOracle::setup(const std::string& dir, const Config& c, std::string* err
             if (err) *err = "22-moment table missing"; return false;
         }
     }
-    sf.init(cfg.include_bulk == 1);
+    sf.init(cfg.include_bulk == 1, cfg.include_diffusion == 1);
     psampler.build(cfg.fermion_cdf2_quirk == 1);
     return true;
 }
@@ -785,6 +846,38 @@ void FSSWOracle::yields_one_species(int ri, std::vector<double>& out) const {
         const double pref = p.gspin / (2. * M_PI * M_PI);
         double r[4];
         dN_analytic(p, mu, T, r);
+        // diffusion (src/FSSW.cpp:622-637, 696-702, 783-829)
+        double dNq = 0.;
+        if (cfg.include_diffusion == 1) {
+            // a float dot product of the float records, stored in double (:627-629)
+            const double dsq = s.qx * s.da[1] + s.qy * s.da[2] + s.qz * s.da[3];
+            const double coeff = dft.qmu_coeff(T, muB);
+            // double locals (:633-635)
+            const double rho_B = s.Bn, Edec = s.Edec, Pdec = s.Pdec;
+            const double pq = rho_B / (Edec + Pdec);
+            const int nt = (p.mass < 0.7 && T > 0.05) ? 10 : 1;
+            const double beta = 1. / T, lambda = std::exp(beta * mu), mb = p.mass * beta;
+            double t1 = 0., t2 = 0.;
+            for (int n = 1; n <= nt; n++) {
+                const double arg = n * p.mass * beta;
+                const double theta = std::pow(-p.sign, n - 1), fug = std::pow(lambda, n);
+                t1 += theta / n * fug * sf.K(2, arg);
+                double e[9];
+                sf.E(arg, e);
+                double I1 = std::exp(-arg) / arg * (2. / (arg * arg) + 2. / arg - 1. / 2.) + 3. / 8. * e[0];
+                double df2 = 1., fact = 2., p2k = 4.;
+                for (int k = 3; k <= 10; k++) {
+                    df2 *= (2 * k - 5);
+                    fact *= k;
+                    p2k *= 2;
+                    I1 += 3. * df2 / p2k / fact * e[k - 2];
+                }
+                t2 += n * theta * fug * (-(mb * mb * mb) * I1);
+            }
+            t1 = p.mass * p.mass * T * T * t1;
+            t2 = T * T * T / 3. * t2;
+            dNq = uf * pref * dsq / coeff * (-pq * t1 - p.baryon * t2);
+        }
         const double Neq = uf * pref * dsu * r[0];
         double dNb = 0.;
         if (cfg.include_bulk == 1) {
@@ -800,7 +893,7 @@ void FSSWOracle::yields_one_species(int ri, std::vector<double>& out) const {
                        r[3] * (c[1] - c[2]));
             }
         }
-        out[l] = std::max(0., Neq + dNb);
+        out[l] = std::max(0., Neq + dNb + dNq);
     }
 }
 
@@ -819,6 +912,9 @@ int FSSWOracle::sample_from_cell(double mass, int sign, int baryon, int strange,
     const double dsfac = std::abs(s.da[0]) +
                          std::sqrt(s.da[1] * s.da[1] + s.da[2] * s.da[2] + s.da[3] * s.da[3]);
     const int kind = cfg.bulk_kind, nk = neos_kind();
+    // diffusion (src/FSSW.cpp:1006-1008, 1865, 1931-1937)
+    const double qcoeff = cfg.include_diffusion == 1 ? dft.qmu_coeff(s.Tdec, s.muB) : 1.;
+    const double prefactor_qmu = s.Bn / (s.Edec + s.Pdec);
     for (int tries = 1; tries < 5000; tries++) {
         const double pmag = psampler.sample(mass, T, mu, sign, g, U);
         if (pmag < 0.) return -1;
@@ -856,8 +952,13 @@ int FSSWOracle::sample_from_cell(double mass, int sign, int baryon, int strange,
                        p0 * p0 * (c[1] - c[2]));
             }
         }
+        double dfq = 0.;
+        if (cfg.include_diffusion == 1) {
+            const double qf = -px * s.qx - py * s.qy - pz * s.qz;
+            dfq = (1. - sign * f0) * (prefactor_qmu - baryon / p0) * qf / qcoeff;
+        }
         double f1 = pds / p0 / dsfac;
-        double f2 = (1. + dfs + dfb) / 2.;
+        double f2 = (1. + dfs + dfb + dfq) / 2.;
         f1 = std::max(0., std::min(1., f1));
         f2 = std::max(0., std::min(1., f2));
         if (U(g) < f1 * f2) {
@@ -1042,6 +1143,12 @@ void smooth_spectra(const FSSWOracle& o, const std::vector<Surf>& lab, int ri, c
                     }
                     const double Edec = s.Edec, Pdec = s.Pdec;  // double locals (:687-689)
                     const double dfpref = 1.0 / (2.0 * T * T * (Edec + Pdec));
+                    // diffusion (src/emissionfunction.cpp:737-754, 777-786)
+                    double qcoeff = 1., prefactor_qmu = 0.;
+                    if (o.cfg.include_diffusion == 1) {
+                        qcoeff = o.dft.qmu_coeff(T, s.muB);
+                        prefactor_qmu = s.Bn / (Edec + Pdec);
+                    }
                     const size_t nkk = so.mode == 1 ? 1 : ny;
                     double cell = 0.;
                     for (size_t k = 0; k < nkk; k++) {
@@ -1076,10 +1183,16 @@ void smooth_spectra(const FSSWOracle& o, const std::vector<Surf>& lab, int ri, c
                                        pdotu * pdotu * (c[1] - c[2]));
                             }
                         }
+                        double dfq = 0.;
+                        if (o.cfg.include_diffusion == 1) {
+                            const double qf = pt * s.qmu0 - px * s.qmu1 - py * s.qmu2 - pz * s.qmu3;
+                            dfq = (1. - sign * f0) * (prefactor_qmu - p.baryon / pdotu) * qf / qcoeff;
+                        }
                         double resize = 1.0;
                         if (so.restrict_deltaf == 1)
-                            resize = std::min(1., so.deltaf_max_ratio / (std::abs(dfs + dfb) + 1e-10));
-                        const double result = prefactor * degen * f0 * pdsigma * tau * (1. + (dfs + dfb) * resize);
+                            resize = std::min(1., so.deltaf_max_ratio / (std::abs(dfs + dfb + dfq) + 1e-10));
+                        const double result =
+                            prefactor * degen * f0 * pdsigma * tau * (1. + (dfs + dfb + dfq) * resize);
                         if (so.use_pos_dN_only && result < 0.) continue;
                         cell += result * wk;
                     }

@@ -122,13 +122,16 @@ void surface_tmunu(const std::vector<Surf>& cells, double T[16], double Q[3]);
 
 // ------------------------------------------------------ special functions
 double bessel_kn(int n, double x);  // own quadrature; pinned vs scipy
+double expint_en(int n, double x);  // own quadrature; pinned vs scipy
 
 struct SpecialFunctions {
     static constexpr double x_min = 0.5, x_max = 400., dx = 0.05;
     int length = 0;
     std::vector<double> K1, K2, K3;
-    void init(bool with_bulk);
+    std::vector<double> En;   // [9][length]: E_2, E_4, ..., E_18 (diffusion only)
+    void init(bool with_bulk, bool with_diffusion = false);
     double K(int order, double arg) const;
+    void E(double arg, double out[9]) const;   // get_special_function_En (src/FSSW.cpp:1706-1726)
 };
 
 // ------------------------------------------------------ delta-f tables
@@ -143,6 +146,10 @@ struct DeltafTables {
     int neos_ncol = 0;
     std::vector<double> neos;                       // [row*ncol + col]
     double neos_de = 0.;
+    // diffusion: Coefficients_RTA_diffusion.dat, [iT * 100 + imu] (src/FSSW.cpp:1546-1566)
+    std::vector<double> qmu;
+    bool load_qmu(const std::string& file);
+    double qmu_coeff(double T, double muB) const;   // get_deltaf_qmu_coeff (:1569-1606)
     bool load_kind0(const std::string& file);
     bool load_kind11(const std::string& dir);
     bool load_neos(const std::string& file, int ncol);

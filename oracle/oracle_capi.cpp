@@ -20,7 +20,7 @@ extern "C" {
 
 void* orc_create(const char* table_dir, int afterburner, int qs, int shear, int bulk,
                  int bulk_kind, int hydro_mode, int model, double para1, double yLB,
-                 double yRB, int fd_quirk) {
+                 double yRB, int fd_quirk, int diffusion) {
     Handle* h = new Handle;
     Config c;
     c.afterburner = afterburner;
@@ -34,6 +34,7 @@ void* orc_create(const char* table_dir, int afterburner, int qs, int shear, int 
     c.y_LB = yLB;
     c.y_RB = yRB;
     c.fermion_cdf2_quirk = fd_quirk;
+    c.include_diffusion = diffusion;
     if (!h->o.setup(table_dir, c, &h->err)) {
         delete h;
         return nullptr;
@@ -189,6 +190,7 @@ void orc_tmunu_samples(void* p, double volume, double* T16, double* Q3) {
 }
 
 double orc_bessel_kn(int n, double x) { return bessel_kn(n, x); }
+double orc_expint(int n, double x) { return expint_en(n, x); }
 
 double orc_kn_table(void* p, int order, double x) { return static_cast<Handle*>(p)->o.sf.K(order, x); }
 

@@ -22,7 +22,7 @@ def lib():
         o = C.CDLL(ORACLE_LIB)
         v, i, d, s, L = C.c_void_p, C.c_int, C.c_double, C.c_char_p, C.c_long
         sig = {
-            "orc_create": (v, [s, i, i, i, i, i, i, i, d, d, d, i]),
+            "orc_create": (v, [s, i, i, i, i, i, i, i, d, d, d, i, i]),
             "orc_destroy": (None, [v]),
             "orc_num_species": (i, [v]),
             "orc_num_particles": (i, [v]),
@@ -44,6 +44,7 @@ def lib():
             "orc_hadrons": (None, [v, v]),
             "orc_tmunu_samples": (None, [v, d, v, v]),
             "orc_bessel_kn": (d, [i, d]),
+            "orc_expint": (d, [i, d]),
             "orc_kn_table": (d, [v, i, d]),
             "orc_psample": (i, [v, C.c_uint, d, d, d, i, i, v]),
             "orc_cdf_pick": (None, [v, L, C.c_uint, L, v]),
@@ -76,10 +77,10 @@ def _p(a):
 
 class Oracle:
     def __init__(self, afterburner=1, qs=1, shear=0, bulk=0, bulk_kind=1, hydro_mode=2,
-                 model=30, para1=0.16, y_LB=-5.0, y_RB=5.0, fd_quirk=1, table_dir=TABLE_DIR):
+                 model=30, para1=0.16, y_LB=-5.0, y_RB=5.0, fd_quirk=1, table_dir=TABLE_DIR, diffusion=0):
         self.o = lib()
         self.h = self.o.orc_create(table_dir.encode(), afterburner, qs, shear, bulk, bulk_kind,
-                                   hydro_mode, model, para1, y_LB, y_RB, fd_quirk)
+                                   hydro_mode, model, para1, y_LB, y_RB, fd_quirk, diffusion)
         if not self.h:
             raise RuntimeError("oracle setup failed")
 

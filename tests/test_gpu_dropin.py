@@ -33,12 +33,18 @@ RUNS = {  # name: (music_input, parameter file, surface)
 }
 
 
-def _run(tmp_path, name, extra=()):
+def _run(tmp_path, name, extra=(), surface_text=None, music_text=None):
     mi, params, surf = RUNS[name]
     work = tmp_path / "tests"
     work.mkdir()
-    shutil.copy(os.path.join(FIX, mi), work / "music_input")
-    shutil.copy(os.path.join(FIX, surf), work / surf)
+    if music_text is None:
+        shutil.copy(os.path.join(FIX, mi), work / "music_input")
+    else:
+        (work / "music_input").write_text(music_text)
+    if surface_text is None:
+        shutil.copy(os.path.join(FIX, surf), work / surf)
+    else:
+        (work / surf).write_text(surface_text)
     shutil.copy(os.path.join(FIX, params), tmp_path / params)
     os.symlink(os.path.join(ROOT, "iss_amd", "tables"), tmp_path / "iSS_tables")
     args = [EXE, params, "tests", surf, "randomSeed=7", "number_of_repeated_sampling=1000"]
@@ -222,3 +228,47 @@ def test_dropin_perform_decays(tmp_path):
     qp = np.mean([sum(q.get(h[0], 0) for h in e) for e in prim])
     print("hadrons primary", npr, "decayed", nd, "mean charge", qp, qd)
     assert abs(qd - qp) < 5 * np.sqrt(npr / 20) / np.sqrt(20) + 1
+
+
+def test_dropin_diffusion_spectra_match_oracle(tmp_path):
+    """include_deltaf_diffusion = 1 with a surface that carries n_B and q^mu
+    (turn_on_baryon_diffusion 1 in music_input): the drop-in reads
+    Coefficients_RTA_diffusion.dat (src/FSSW.cpp:191-195) and its smooth
+    spectra match the oracle's on the same file.  The cell is at rest with a
+    transverse q, so the diffusion term shows as a directed (cos phi)
+    modulation of p and pbar that the diffusion-free spectrum lacks."""
+    name = "Viscous1"
+    mi, _, surf = RUNS[name]
+    base = open(os.path.join(FIX, surf)).read().split()
+    line = " ".join(base[:28] + [base[28], "0.05", "0.0", "0.0006", "-0.0003", "0.0"])
+    music = open(os.path.join(FIX, mi)).read()
+    music = music.replace("Include_Rhob_Yes_1_No_0  0", "Include_Rhob_Yes_1_No_0  1")
+    music = music.replace("turn_on_baryon_diffusion  0", "turn_on_baryon_diffusion  1")
+    _run(tmp_path, name, ["MC_sampling=0", "calculate_vn=1", "perform_checks=0", "calculate_vn_to_order=2",
+                          "include_deltaf_diffusion=1", "turn_on_rhob=1", "turn_on_diff=1"],
+         surface_text=line + "\n", music_text=music)
+    cfg = ob.case_config(name)
+    orc = {}
+    for d in (0, 1):
+        orc[d] = ob.Oracle(afterburner=cfg["afterburner"], shear=cfg["shear"], bulk=cfg["bulk"],
+                           bulk_kind=cfg["bulk_kind"], diffusion=d)
+        orc[d].read_surface(str(tmp_path / "tests" / surf), bulk=1, rhob=1, diff=1, eos_file=cfg["eos_file"],
+                            iEOS=cfg["iEOS"])
+    from iss_amd import tables
+    b = tables.bin_tables()
+    sp = orc[1].species()
+    parts = orc[1].particles()
+    dn = _table_blocks(tmp_path / "tests" / "dN_pTdpTdphidy.dat", b["phi"].size)
+    for pid in (211, 2212, -2212):
+        s = int(np.nonzero(sp["pid"] == pid)[0][0])
+        ref = orc[1].spectra(s, b["pT"], b["phi"], b["eta"], b["eta_weight"])[0]
+        ref0 = orc[0].spectra(s, b["pT"], b["phi"], b["eta"], b["eta_weight"])[0]
+        got = dn[int(np.nonzero(parts["pid"] == pid)[0][0])].T
+        rowmax = np.abs(ref).max(axis=1, keepdims=True)
+        err = np.max(np.abs(got - ref) / rowmax)
+        eff = np.max(np.abs(ref0 - ref) / rowmax)
+        v1 = np.loadtxt(tmp_path / "tests" / f"thermal_{pid}_integrated__vndata.dat")[1]
+        print(pid, "row error", err, "diffusion effect", eff, "integrated v1 row", v1)
+        assert err < 5e-5, pid
+        if pid != 211:
+            assert eff > 1e-2, pid
