@@ -100,3 +100,41 @@ def test_coefficient_outside_table_switches_the_correction_off():
         orc.set_cells_lrf(c[None])
         ys.append(orc.yields()[0])
     np.testing.assert_allclose(ys[1], ys[0], rtol=1e-20, atol=1e-30)
+
+
+@pytest.mark.parametrize("pid", [211, 2212, -2212, 3122])
+def test_static_cell_spectrum_with_diffusion_closed_form(pid):
+    """a static cell with n_B and a lab current q = (0, q_x, 0, 0): with
+    f0 (1 - s f0) = sum_n n th_n e^{-n (E - mu)/T} the eta_s integral gives
+    tau A sum_n e^{n mu/T} [th_n 2 mT K1(a_n)
+        - (pT cos(phi) q_x / D) n th_n (n_B/(e+P) 2 mT K1(a_n) - 2 B K0(a_n))],
+    a_n = n mT / T, th_n = (-s)^(n-1)"""
+    from tests.test_spectra_oracle import ETA_FINE, PREF, W_FINE, _cell
+    T, muB, e, P, nB, qx = f32(0.15), f32(0.2), f32(0.4), f32(0.12), f32(0.05), f32(0.002)
+    c = _cell(tau=3.0, area=2.0, T=T, e=e, P=P, muB=muB)
+    c[15] = nB
+    c[31] = qx
+    orc = ob.Oracle(afterburner=1, qs=1, shear=0, bulk=0, diffusion=1)
+    orc.set_cells_lab(c[None])
+    sp = orc.species()
+    s = int(np.nonzero(sp["pid"] == pid)[0][0])
+    m, g, sg, B = sp["mass"][s], sp["g"][s], sp["sign"][s], sp["B"][s]
+    pT = np.array([0.05, 0.3, 1.0, 2.0])
+    phi = np.array([0.0, 1.0, 2.5, 4.0])
+    out = orc.spectra(s, pT, phi, ETA_FINE, W_FINE, restrict_deltaf=0)[0]
+    mT = np.sqrt(m * m + pT * pT)
+    pq = nB / (e + P)
+    D = _qcoeff(T, muB)
+    eq = np.zeros_like(mT)
+    dq = np.zeros_like(mT)
+    for n in range(1, 41):
+        th = (-sg) ** (n - 1)
+        a = n * mT / T
+        w = np.exp(n * B * muB / T)
+        eq += th * w * 2 * mT * special.kn(1, a)
+        dq += n * th * w * (pq * 2 * mT * special.kn(1, a) - 2 * B * special.kn(0, a))
+    ref = PREF * g * 3.0 * 2.0 * (eq[:, None] - (pT[:, None] * np.cos(phi)[None] * qx / D) * dq[:, None])
+    rel_effect = np.max(np.abs(ref - PREF * g * 6.0 * eq[:, None]) / np.abs(ref))
+    print(pid, "diffusion part up to", rel_effect, "of the spectrum")
+    assert rel_effect > 1e-2
+    np.testing.assert_allclose(out, ref, rtol=1e-9)

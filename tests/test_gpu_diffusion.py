@@ -116,14 +116,29 @@ def test_fp64_diffusion_spectra_match_oracle(species, name, df):
     assert eff > 100 * FP64_ROW_TOL
 
 
+def _coeff_D(T, muB):
+    """the diffusion coefficient table's bilinear value (src/FSSW.cpp:1571-1606)"""
+    qt = tables.read_qmu_table()
+    x, y = (T - 0.05) / 0.001, muB / 0.007892
+    iT, im = x.astype(int), y.astype(int)
+    xf, yf = x - iT, y - im
+    return (qt[iT, im] * (1 - xf) * (1 - yf) + qt[iT, im + 1] * (1 - xf) * yf + qt[iT + 1, im + 1] * xf * yf
+            + qt[iT + 1, im] * xf * (1 - yf))
+
+
 def _directed(species, amp, seed):
+    """rest-frame current along x with q_x / D in [amp / 2, amp]: the
+    diffusion delta f stays well inside (-1, 1) where hadrons are, so the
+    sampler's Poisson mean (the unclamped yield) and the smooth spectrum
+    describe the same law"""
     cells, _ = synth.viscous_lrf(20000, species, seed=seed, muB_max=0.3, target_hadrons=4000.0)
     cells = cells.copy()
     epp = cells[:, 12] + cells[:, 14]
     cells[:, 20:25] *= 0.05
     cells[:, 19] = np.maximum(cells[:, 19], -0.01 * epp)
     rng = np.random.default_rng(seed)
-    cells[:, 25] = amp * rng.uniform(0, 1, cells.shape[0]) * cells[:, 15]
+    D = _coeff_D(cells[:, 13].astype(np.float64), cells[:, 16].astype(np.float64))
+    cells[:, 25] = amp * rng.uniform(0.5, 1, cells.shape[0]) * D
     cells[:, 26:28] = 0
     return cells
 
@@ -131,7 +146,7 @@ def _directed(species, amp, seed):
 def test_sampled_baryons_match_smooth_spectrum_with_diffusion(species):
     """chi^2 of sampled p and pbar (pT, phi) counts against the smooth
     spectrum with diffusion; the diffusion-free spectrum must fail"""
-    cells = _directed(species, 0.1, 23)
+    cells = _directed(species, 0.5, 23)
     df = dict(shear=1, bulk=1, bulk_kind=1)
     ctx = _ctx(species, df)
     ctx.set_surface_lrf(cells)
