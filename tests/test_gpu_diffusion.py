@@ -152,7 +152,7 @@ def test_sampled_baryons_match_smooth_spectrum_with_diffusion(species):
     ctx.set_surface_lrf(cells)
     ctx0 = _ctx(species, df, 0)
     ctx0.set_surface_lrf(cells)
-    n_ev = 1000
+    n_ev = 2000
     ctx.sample(n_ev, seed=78)
     h = ctx.hadrons()
     pt_edges = np.linspace(0.0, 2.5, 11)
