@@ -89,6 +89,7 @@ struct iss_ctx {
     int Ns = 0;
     std::vector<int> h_pid;
     DBuf<double> d_mass;
+    DBuf<SpeciesK> d_spk;
     DBuf<int> d_ints;  // g, B, S, Q, sign, pid
     // delta f
     iss_deltaf_desc df{};
@@ -318,19 +319,18 @@ SpeciesDev species_dev(iss_ctx* ctx) {
 
 template <typename R, int MODE, bool RAW, bool EXACT>
 int launch_yield_t(iss_ctx* ctx, const YieldArgs& a) {
-    constexpr int TC = 16;
-    const size_t lds = yield_lds_bytes(sizeof(R), ktables_for(MODE), TC, pad_species(a.Ns));
-    auto kern = k_yield<R, MODE, RAW, TC, EXACT>;
+    const size_t lds = yield_lds_bytes(sizeof(R), ktables_for(MODE), pad_species(a.Ns));
+    auto kern = k_yield<R, MODE, RAW, EXACT>;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(lds)) != hipSuccess)
         return ISS_ERR_HIP;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kYieldThreads, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    const int64_t ntiles = (a.Nc + TC - 1) / TC;
+    const int64_t ntiles = (a.Nc + kYieldThreads - 1) / kYieldThreads;
     const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ntiles, int64_t(ctx->num_cu) * per_cu)));
     ctx->yield_grid = nblk;
-    kern<<<nblk, kYieldThreads, lds, ctx->stream>>>(a);
+    kern<<<nblk, kYieldThreads, lds, ctx->stream>>>(a, a.spk);
     return hipGetLastError() == hipSuccess ? ISS_OK : ISS_ERR_HIP;
 }
 
@@ -381,6 +381,7 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     a.raw = raw ? static_cast<void*>(ctx->d_raw.p) : nullptr;
     a.partial = ctx->d_partial.p;
     a.dnq = nullptr;
+    a.spk = ctx->d_spk.p;
     HIPCHK(ctx->d_oob.ensure(1));
     a.oob = ctx->d_oob.p;
     HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
@@ -640,7 +641,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     if (!ctx) return ISS_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->d_mass.release(); ctx->d_ints.release(); ctx->d_k11.release(); ctx->d_neos.release();
+    ctx->d_mass.release(); ctx->d_spk.release(); ctx->d_ints.release(); ctx->d_k11.release(); ctx->d_neos.release();
     ctx->d_ktab.release(); ctx->d_ktabf.release(); ctx->d_ptab.release();
     ctx->d_cells.release(); ctx->d_tmp.release(); ctx->d_coef.release(); ctx->d_u32.release();
     ctx->d_i64.release(); ctx->d_scan.release(); ctx->d_eos.release(); ctx->d_cdf.release();
@@ -682,10 +683,27 @@ int iss_set_species(iss_ctx* ctx, const iss_species_desc* sp) {
         ints[4 * Ns + i] = sp->sign[i];
         ints[5 * Ns + i] = sp->pid[i];
     }
+    // yield-pass constants: pref = g / (2 pi^2 (hbar c)^3), formed in each precision as the FP64 fallback
+    // and the FP32 series use it
+    std::vector<SpeciesK> spk(Ns);
+    const double uf = 1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi);
+    for (int i = 0; i < Ns; i++) {
+        SpeciesK& k = spk[i];
+        k.m = static_cast<float>(sp->mass[i]);
+        k.pref = static_cast<float>(sp->degeneracy[i]) * static_cast<float>(uf);
+        k.B = static_cast<float>(sp->baryon[i]);
+        k.S = static_cast<float>(sp->strange[i]);
+        k.Q = static_cast<float>(sp->charge[i]);
+        k.sgn = static_cast<float>(sp->sign[i]);
+        k.md = sp->mass[i];
+        k.prefd = static_cast<double>(sp->degeneracy[i]) * uf;
+    }
     HIPCHK(ctx->d_mass.ensure(Ns));
     HIPCHK(ctx->d_ints.ensure(6 * Ns));
+    HIPCHK(ctx->d_spk.ensure(Ns));
     HIPCHK(hipMemcpyAsync(ctx->d_mass.p, sp->mass, sizeof(double) * Ns, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_ints.p, ints.data(), sizeof(int) * 6 * Ns, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_spk.p, spk.data(), sizeof(SpeciesK) * Ns, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->Ns = Ns;
     ctx->h_pid.assign(sp->pid, sp->pid + Ns);

@@ -31,7 +31,7 @@ enum CellField {
 // special-function table (src/FSSW.cpp:1609-1615)
 constexpr double kKx0 = 0.5, kKdx = 0.05, kKxmax = 400.;
 constexpr int kKLen = 7991;
-constexpr int kKLds = 1024;      // rows staged in LDS (args up to 51.65)
+constexpr int kKLds = 1536;      // rows staged in LDS (args up to 77.25)
 
 constexpr int kEnTerms = 9;     // E_2, E_4, ..., E_18 (src/FSSW.cpp:1611, 1637-1642)
 constexpr int kCoefRows = 8;    // per-cell coefficients: 6 bulk / NEoS, 1 / D_qmu, n_B / (e + P)
@@ -50,6 +50,13 @@ struct SpeciesDev {
     const int* Q;
     const int* sign;
     const int* pid;
+};
+
+// species constants of the yield pass, read wave-uniformly (scalar loads):
+// mass and g / (2 pi^2 (hbar c)^3) in FP32 and FP64, charges and statistics
+struct SpeciesK {
+    float m, pref, B, S, Q, sgn;
+    double md, prefd;
 };
 
 struct PTabDev {
@@ -75,7 +82,8 @@ struct YieldArgs {
     void* raw;       // optional per-(cell, species) yields in working precision
     double* partial; // [gridDim.x][Ns]
     int* oob;        // set when a Bessel argument leaves the table
-    const double* dnq;   // [Nc][Ns] diffusion corrections (k_diffusion_dn); null: none
+    const double* dnq;   // [Ns][Nc] diffusion corrections (k_diffusion_dn); null: none
+    const SpeciesK* spk;  // [Ns] species constants of the yield pass
 };
 
 struct SampleArgs {

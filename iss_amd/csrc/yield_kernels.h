@@ -9,28 +9,27 @@ namespace issdev {
 // ============================================================== yield pass
 
 template <typename R>
-struct KLds;
-template <>
-struct KLds<double> {
-    using T = double;
-};
-template <>
-struct KLds<float> {
-    using T = float2;
+struct KLds {
+    using T = R;
 };
 
 // LDS slot of K_order: K_2 first (every mode), then K_1, then K_3
 __device__ __forceinline__ int kslot(int order) { return order == 2 ? 0 : (order == 1 ? 1 : 2); }
 
-// K_n lookup, FP64: the reference's interpolation verbatim
-// (src/FSSW.cpp:1662-1666); rows below kKLds from LDS when staged.  Outside
-// [0.5, 399.95] the reference calls the exact function; the fast variant
-// (EXACT = false) raises *oob instead and the pass is rerun with EXACT = true.
+// K_n lookup: the reference's table and linear interpolation
+// (src/FSSW.cpp:1646-1703), x = 0.5 + 0.05 i.  The fast variant
+// (EXACT = false) reads only the first kKLds rows, staged in LDS (the yield
+// pass), or only the global table (staged = false); an argument outside those
+// rows raises *oob and the pass is rerun with EXACT = true, which reads the
+// global table and calls the exact K_n outside [0.5, 399.95] as the reference
+// calls GSL.  Keeping the two paths in different kernels keeps the fast path
+// a plain LDS access.
 template <bool EXACT>
 struct KLookup64 {
-    const double* lds;   // may be null
-    const double* glob;
+    const double* lds;   // [3][kKLds] staged rows (used when `staged`)
+    const double* glob;  // [3][kKLen]
     int* oob;
+    bool staged;
     __device__ __forceinline__ double operator()(int order, double arg) const {
         if (arg < kKx0 || arg > kKxmax - kKdx) {
             if constexpr (EXACT) return kn_exact(order, arg);
@@ -39,17 +38,36 @@ struct KLookup64 {
         }
         const int idx = static_cast<int>((arg - kKx0) / kKdx);
         const double fr = (arg - kKx0 - idx * kKdx) / kKdx;
-        const double* t = (lds && idx + 1 < kKLds) ? lds + kslot(order) * kKLds
-                                                   : glob + (order - 1) * kKLen;
-        return (1. - fr) * t[idx] + fr * t[idx + 1];
+        double k0, k1;
+        if (!EXACT && staged) {
+            if (idx + 1 >= kKLds) {
+                *oob = 1;
+                return 0.;
+            }
+            k0 = lds[kslot(order) * kKLds + idx];
+            k1 = lds[kslot(order) * kKLds + idx + 1];
+        } else {
+            k0 = glob[(order - 1) * kKLen + idx];
+            k1 = glob[(order - 1) * kKLen + idx + 1];
+        }
+        return (1. - fr) * k0 + fr * k1;
+    }
+    // K_2 and, for NK = 3 or 2, K_1 (and K_3) at one argument
+    template <int NK>
+    __device__ __forceinline__ void k123(double arg, double& K2, double& K1, double& K3) const {
+        K2 = (*this)(2, arg);
+        if (NK >= 2) K1 = (*this)(1, arg);
+        if (NK >= 3) K3 = (*this)(3, arg);
     }
 };
 
-// K_n lookup, FP32: same grid, K_i + f (K_{i+1} - K_i)
+// K_n lookup, FP32: same grid, K_i + f (K_{i+1} - K_i); the fast variant
+// reads FP32 rows from LDS, the exact one the global (K_i, K_{i+1} - K_i)
+// table
 template <bool EXACT>
 struct KLookup32 {
-    const float2* lds;
-    const float2* glob;
+    const float* lds;    // [3][kKLds]
+    const float2* glob;  // [3][kKLen]
     int* oob;
     __device__ __forceinline__ float operator()(int order, float arg) const {
         if (arg < 0.5f || arg > 399.95f) {
@@ -60,8 +78,42 @@ struct KLookup32 {
         const float t = (arg - 0.5f) * 20.f;
         const int idx = static_cast<int>(t);
         const float fr = t - static_cast<float>(idx);
-        const float2 v = (idx < kKLds) ? lds[kslot(order) * kKLds + idx] : glob[(order - 1) * kKLen + idx];
-        return fmaf(fr, v.y, v.x);
+        if constexpr (EXACT) {
+            const float2 v = glob[(order - 1) * kKLen + idx];
+            return fmaf(fr, v.y, v.x);
+        } else {
+            if (idx + 1 >= kKLds) {
+                *oob = 1;
+                return 0.f;
+            }
+            const float k0 = lds[kslot(order) * kKLds + idx], k1 = lds[kslot(order) * kKLds + idx + 1];
+            return fmaf(fr, k1 - k0, k0);
+        }
+    }
+    // K_2 and, for NK = 3 or 2, K_1 (and K_3) at one argument: one index,
+    // the LDS reads issued together, out-of-range arguments flagged without
+    // a branch around the reads
+    template <int NK>
+    __device__ __forceinline__ void k123(float arg, float& K2, float& K1, float& K3) const {
+        if constexpr (EXACT) {
+            K2 = (*this)(2, arg);
+            if (NK >= 2) K1 = (*this)(1, arg);
+            if (NK >= 3) K3 = (*this)(3, arg);
+        } else {
+            const float t = (arg - 0.5f) * 20.f;
+            int idx = static_cast<int>(t);
+            const bool ok = (arg >= 0.5f) & (arg <= 399.95f) & (idx + 1 < kKLds);
+            if (!ok) *oob = 1;
+            idx = ok ? idx : 0;
+            const float fr = t - static_cast<float>(idx);
+            const float a2 = lds[idx], b2 = lds[idx + 1];
+            float a1 = 0.f, b1 = 0.f, a3 = 0.f, b3 = 0.f;
+            if (NK >= 2) { a1 = lds[kKLds + idx]; b1 = lds[kKLds + idx + 1]; }
+            if (NK >= 3) { a3 = lds[2 * kKLds + idx]; b3 = lds[2 * kKLds + idx + 1]; }
+            K2 = fmaf(fr, b2 - a2, a2);
+            if (NK >= 2) K1 = fmaf(fr, b1 - a1, a1);
+            if (NK >= 3) K3 = fmaf(fr, b3 - a3, a3);
+        }
     }
 };
 
@@ -119,23 +171,18 @@ __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, 
     const R mb = mass * beta;
     R r0 = 0, r1 = 0, r2 = 0, r3 = 0;
     R th = 1, fug = lambda;
-#pragma unroll 1
-    for (int n = 1; n <= nt; n++) {
-        const R rn = static_cast<R>(n);
-        const R inv = (sizeof(R) == sizeof(double)) ? R(1) : frcp(rn);
+    auto term = [&](R rn, R inv) {
         const R arg = rn * mass * beta;
-        const R K2 = kl(2, arg);
+        R K2 = 0, K1 = 0, K3 = 0;
+        kl.template k123<(MODE == BULK_NONE) ? 1 : ((MODE == BULK_RTA || MODE == BULK_CE) ? 2 : 3)>(arg, K2, K1, K3);
         // FP64: theta / n as the reference divides; FP32: times 1/n
         const R thn = (sizeof(R) == sizeof(double)) ? th / rn : th * inv;
         const R k2n = (sizeof(R) == sizeof(double)) ? 3 * K2 / rn : 3 * K2 * inv;
         r0 += thn * fug * K2;
         if (MODE == BULK_RTA || MODE == BULK_CE) {
-            const R K1 = kl(1, arg);
             r1 += th * fug * (mb * K1 + k2n);
             r2 += th * fug * K1;
         } else if (MODE == BULK_14MOM || MODE == BULK_22MOM) {
-            const R K1 = kl(1, arg);
-            const R K3 = kl(3, arg);
             const R k3n = (sizeof(R) == sizeof(double)) ? 3 * K3 / rn : 3 * K3 * inv;
             r1 += th * fug * K2;
             r2 += th * fug * (mb * K1 + k2n);
@@ -143,6 +190,16 @@ __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, 
         }
         th *= -sgn;
         fug *= lambda;
+    };
+    if (nt == 1) {
+        // one term (every species above 0.7 GeV): n = 1 folds away
+        term(R(1), R(1));
+    } else {
+#pragma unroll 1
+        for (int n = 1; n <= nt; n++) {
+            const R rn = static_cast<R>(n);
+            term(rn, (sizeof(R) == sizeof(double)) ? R(1) : frcp(rn));
+        }
     }
     const R m2 = mass * mass;
     const R pd = pref * dsu;
@@ -167,16 +224,13 @@ __device__ __forceinline__ R yield_one(R mass, R pref, R Bq, R Sq, R Qq, R sgn, 
     return N > R(0) ? N : R(0);
 }
 
-// FP32 yields whose bulk correction nearly cancels N_eq ((|N_eq| + |dN|) >
-// 4 |N|) are recomputed in FP64 so that every FP32 yield stays within 1e-5
-// of FP64.
-
 // FP64 yield of one (cell, species) from global memory only: the fallback
-// for ill-conditioned FP32 entries (kept out of line: it is rare).
+// for FP32 yields whose bulk correction nearly cancels N_eq ((|N_eq| + |dN|)
+// > 4 |N|), so that every FP32 yield stays within 1e-5 of FP64 (rare).
 template <int MODE, bool EXACT>
 __device__ double yield_fp64_global(const YieldArgs& a, int64_t c, int s) {
     const int64_t Nc = a.Nc;
-    const KLookup64<EXACT> k64{nullptr, a.ktab, a.oob};
+    const KLookup64<EXACT> k64{nullptr, a.ktab, a.oob, false};
     const double T = a.cells[F_T * Nc + c];
     double cd[6];
     for (int k = 0; k < 6; k++) cd[k] = a.coef[k * Nc + c];
@@ -186,183 +240,149 @@ __device__ double yield_fp64_global(const YieldArgs& a, int64_t c, int s) {
     return yield_one<double, MODE>(a.sp.mass[s], pref, a.sp.B[s], a.sp.S[s], a.sp.Q[s], a.sp.sign[s], T,
                                    1. / T, (double)a.cells[F_MUB * Nc + c], (double)a.cells[F_MUS * Nc + c],
                                    (double)a.cells[F_MUQ * Nc + c], (double)a.cells[F_DA0 * Nc + c], Pi, cd,
-                                   k64, (bool*)nullptr, a.dnq ? a.dnq[c * a.Ns + s] : 0.);
+                                   k64, (bool*)nullptr, a.dnq ? a.dnq[static_cast<int64_t>(s) * Nc + c] : 0.);
 }
 
 constexpr int kYieldThreads = 256;
-// ill-conditioned FP32 items recomputed in FP64 after each tile; a tile with
-// more than this many falls back to recomputing inline
-constexpr int kFixupCap = 1024;
+
+// an empty use of a register value: the compiler completes its load here
+template <typename T>
+__device__ __forceinline__ void settle(const T& x) {
+    asm volatile("" ::"v"(x));
+}
+// species per CDF staging chunk
+constexpr int kYieldChunk = 8;
 
 __host__ __device__ constexpr int ktables_for(int mode) {
     return mode == BULK_NONE ? 1 : ((mode == BULK_RTA || mode == BULK_CE) ? 2 : 3);
 }
 
-// LDS bytes of one k_yield workgroup for nsp (padded) species
-__host__ __device__ constexpr size_t yield_lds_bytes(int esz, int ntab, int tc, int nsp) {
-    return static_cast<size_t>(nsp) * (2 * esz + 4)                 // mass, pref, charges
-           + static_cast<size_t>(ntab) * kKLds * 8                  // K rows
-           + static_cast<size_t>(tc) * nsp * esz                    // yield tile
-           + static_cast<size_t>(kYieldThreads / kWave) * nsp * 8   // FP64 row buffers
-           + static_cast<size_t>(tc) * 16 * esz                     // cell parameters
-           + static_cast<size_t>(kFixupCap + 1) * 4;                // deferred FP64 items
-}
-
 __host__ __device__ constexpr int pad_species(int ns) { return (ns + 3) & ~3; }
 
-// Workgroup-tiled yield pass.  A workgroup owns a tile of TC cells at a time
-// (persistent over tiles):
-//   1. items (species s, cell c) in species-major order: 16 consecutive lanes
-//      share a species (LDS broadcast) and the 10-term light species fill
-//      whole waves; raw yields go to an LDS tile;
-//   2. one wave per cell row: lanes take contiguous species blocks, an FP64
-//      local prefix plus one wave scan gives the species CDF, which is staged
-//      in an LDS row buffer and stored coalesced;
-//   3. FP64 column sums of the tile feed per-thread species accumulators,
-//      written once per workgroup as partials.
-template <typename R, int MODE, bool RAW, int TC, bool EXACT>
-__global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a) {
+// LDS bytes of one k_yield workgroup for nsp (padded) species
+__host__ __device__ constexpr size_t yield_lds_bytes(int esz, int ntab, int nsp) {
+    return static_cast<size_t>(ntab) * kKLds * esz                                // K rows
+           + static_cast<size_t>(nsp) * 8                                         // species sums
+           + static_cast<size_t>(kYieldThreads) * (kYieldChunk + 1) * (8 + esz);  // prefixes, yields
+}
+
+// Yield pass, one thread per cell (persistent over tiles of 256 cells).  The
+// species loop is uniform across the wave: species constants come from scalar
+// loads, the light species' 10-term series keeps whole waves together, and the
+// cell's parameters stay in registers.  Per species:
+//   the yield (FP32, or FP64 for precision 64), with ill-conditioned FP32
+//   items ((|N_eq| + |dN|) > 4 |N|) recomputed in FP64;
+//   the cell's running FP64 prefix, staged in LDS and stored coalesced as the
+//   species CDF row, kYieldChunk species at a time;
+//   the yield, staged beside it: per chunk, 32 threads per species sum the
+//   tile's column in FP64 (fixed order) into the workgroup's species sums.
+template <typename R, int MODE, bool RAW, bool EXACT>
+__global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const SpeciesK* __restrict__ spk) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NT = ktables_for(MODE);
+    constexpr int SC = kYieldChunk;
     using KT = typename KLds<R>::T;
     const int Ns = a.Ns;
     const int nsp = pad_species(Ns);
-    R* s_mass = reinterpret_cast<R*>(smem);
-    R* s_pref = s_mass + nsp;
-    char4* s_q = reinterpret_cast<char4*>(s_pref + nsp);
-    KT* s_k = reinterpret_cast<KT*>(s_q + nsp);
-    R* s_tile = reinterpret_cast<R*>(s_k + NT * kKLds);
-    double* s_row = reinterpret_cast<double*>(s_tile + TC * nsp);
-    R* s_cell = reinterpret_cast<R*>(s_row + (kYieldThreads / kWave) * nsp);
-    int* s_fix = reinterpret_cast<int*>(s_cell + TC * 16);  // [0] count, then (cl << 16 | s)
+    static_assert(kYieldThreads == 32 * SC, "one 32-lane group per species of a chunk");
+    KT* s_k = reinterpret_cast<KT*>(smem);
+    double* s_acc = reinterpret_cast<double*>(s_k + NT * kKLds);
+    double* s_stage = s_acc + nsp;
+    R* s_y = reinterpret_cast<R*>(s_stage + kYieldThreads * (SC + 1));
 
-    const int64_t Nc = a.Nc;
-    const R uf = static_cast<R>(1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi));
-    for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
-        s_mass[s] = static_cast<R>(a.sp.mass[s]);
-        s_pref[s] = static_cast<R>(a.sp.g[s]) * uf;
-        s_q[s] = make_char4(a.sp.B[s], a.sp.S[s], a.sp.Q[s], a.sp.sign[s]);
-    }
     // K_2 first, then K_1, K_3 as the mode needs
     for (int i = threadIdx.x; i < NT * kKLds; i += blockDim.x) {
         const int t = i / kKLds, r = i % kKLds;
         const int o = (t == 0) ? 2 : (t == 1 ? 1 : 3);
-        if constexpr (sizeof(R) == sizeof(double))
-            s_k[i] = a.ktab[(o - 1) * kKLen + r];
-        else
-            s_k[i] = a.ktabf[(o - 1) * kKLen + r];
+        s_k[i] = static_cast<KT>(a.ktab[(o - 1) * kKLen + r]);
     }
-    // this thread's two species accumulators (Ns <= 2 * blockDim)
-    double acc0 = 0., acc1 = 0.;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wib = threadIdx.x / kWave;
-    const int64_t ntiles = (Nc + TC - 1) / TC;
-    const int items = TC * Ns;
-    // lane-contiguous species block for the row scan
-    const int per = (Ns + kWave - 1) / kWave;
-    const int s_lo = min(lane * per, Ns), s_hi = min(s_lo + per, Ns);
+    for (int i = threadIdx.x; i < nsp; i += blockDim.x) s_acc[i] = 0.;
+    __syncthreads();
 
+    const int64_t Nc = a.Nc;
+    double* stage = s_stage + threadIdx.x * (SC + 1);
+    R* ystage = s_y + threadIdx.x * (SC + 1);
+    // column-sum role: species j of the chunk, cells sub, sub + 32, ...
+    const int cj = threadIdx.x >> 5, csub = threadIdx.x & 31;
+    // store role: entry cj_store of rows cl_store, cl_store + 256 / SC, ...
+    const int cj_store = threadIdx.x % SC, cl_store = threadIdx.x / SC;
+    const int64_t ntiles = (Nc + kYieldThreads - 1) / kYieldThreads;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t c0 = tile * TC;
-        const int nc = static_cast<int>(min<int64_t>(TC, Nc - c0));
-        __syncthreads();  // previous tile's LDS reads are done
-        if (threadIdx.x == 0) s_fix[0] = 0;
-        if (threadIdx.x < nc) {
-            // parameter-major [k][cell]: the 16 lanes of a species read 16
-            // consecutive banks
-            const int64_t c = c0 + threadIdx.x;
-            R* p = s_cell + threadIdx.x;
-            const R T = a.cells[F_T * Nc + c];
-            p[0 * TC] = T;
-            p[1 * TC] = R(1) / T;
-            p[2 * TC] = a.cells[F_MUB * Nc + c];
-            p[3 * TC] = a.cells[F_MUS * Nc + c];
-            p[4 * TC] = a.cells[F_MUQ * Nc + c];
-            p[5 * TC] = a.cells[F_DA0 * Nc + c];
-            R Pi = 0;
+        const int64_t c0 = tile * kYieldThreads;
+        const int64_t c = c0 + threadIdx.x;
+        const int nc = static_cast<int>(min<int64_t>(kYieldThreads, Nc - c0));
+        const bool valid = c < Nc;
+        R T = R(0.15), muB = 0, muS = 0, muQ = 0, dsu = 0, Pi = 0;
+        R cf[6] = {0, 0, 0, 0, 0, 0};
+        if (valid) {
+            T = a.cells[F_T * Nc + c];
+            muB = a.cells[F_MUB * Nc + c];
+            muS = a.cells[F_MUS * Nc + c];
+            muQ = a.cells[F_MUQ * Nc + c];
+            dsu = a.cells[F_DA0 * Nc + c];
             if (MODE != BULK_NONE) {
                 Pi = a.cells[F_PI * Nc + c];
                 if (MODE == BULK_RTA) Pi = Pi / static_cast<R>(kHbarC);
-                for (int k = 0; k < 6; k++) p[(7 + k) * TC] = static_cast<R>(a.coef[k * Nc + c]);
+#pragma unroll
+                for (int k = 0; k < 6; k++) cf[k] = static_cast<R>(a.coef[k * Nc + c]);
             }
-            p[6 * TC] = Pi;
         }
-        __syncthreads();
-        // 1. yields into the LDS tile
-        for (int it = threadIdx.x; it < items; it += blockDim.x) {
-            const int s = it / TC, cl = it % TC;
-            R y = 0;
-            if (cl < nc) {
-                const R* q0 = s_cell + cl;
-                const R p[13] = {q0[0], q0[TC], q0[2 * TC], q0[3 * TC], q0[4 * TC], q0[5 * TC], q0[6 * TC],
-                                 q0[7 * TC], q0[8 * TC], q0[9 * TC], q0[10 * TC], q0[11 * TC], q0[12 * TC]};
-                const R dnq = a.dnq ? static_cast<R>(a.dnq[(c0 + cl) * Ns + s]) : R(0);
-                const char4 q = s_q[s];
-                if constexpr (sizeof(R) == sizeof(double)) {
-                    const KLookup64<EXACT> kl{reinterpret_cast<const double*>(s_k), a.ktab, a.oob};
-                    y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, p[0], p[1], p[2],
-                                           p[3], p[4], p[5], p[6], p + 7, kl, (bool*)nullptr, dnq);
-                } else {
-                    const KLookup32<EXACT> kl{reinterpret_cast<const float2*>(s_k), a.ktabf, a.oob};
-                    bool ill = false;
-                    y = yield_one<R, MODE>(s_mass[s], s_pref[s], q.x, q.y, q.z, q.w, p[0], p[1], p[2],
-                                           p[3], p[4], p[5], p[6], p + 7, kl,
-                                           MODE == BULK_NONE ? (bool*)nullptr : &ill, dnq);
-                    if (MODE != BULK_NONE && ill) {
-                        const int k = atomicAdd(&s_fix[0], 1);
-                        if (k < kFixupCap) s_fix[1 + k] = (cl << 16) | s;
-                        else y = static_cast<R>(yield_fp64_global<MODE, EXACT>(a, c0 + cl, s));
+        // the cell's loads complete here, once per tile: a wait placed inside
+        // the species loop would also wait for the CDF stores issued since
+        // (loads and stores share one counter)
+        settle(T); settle(muB); settle(muS); settle(muQ); settle(dsu); settle(Pi);
+#pragma unroll
+        for (int k = 0; k < 6; k++) settle(cf[k]);
+        const R beta = R(1) / T;
+        double run = 0.;
+        for (int s0 = 0; s0 < Ns; s0 += SC) {
+            const int sn = min(SC, Ns - s0);
+            SpeciesK qn = spk[s0];
+#pragma unroll 2
+            for (int j = 0; j < sn; j++) {
+                const int s = s0 + j;
+                const SpeciesK q = qn;
+                if (j + 1 < sn) qn = spk[s + 1];  // the next species' constants in flight
+                R y = 0;
+                if (valid) {
+                    const R dnq = a.dnq ? static_cast<R>(a.dnq[static_cast<int64_t>(s) * Nc + c]) : R(0);
+                    if constexpr (sizeof(R) == sizeof(double)) {
+                        const KLookup64<EXACT> kl{reinterpret_cast<const double*>(s_k), a.ktab, a.oob, true};
+                        y = yield_one<R, MODE>(q.md, q.prefd, q.B, q.S, q.Q, q.sgn, T, beta, muB, muS, muQ, dsu,
+                                               Pi, cf, kl, (bool*)nullptr, dnq);
+                    } else {
+                        const KLookup32<EXACT> kl{reinterpret_cast<const float*>(s_k), a.ktabf, a.oob};
+                        bool ill = false;
+                        y = yield_one<R, MODE>(q.m, q.pref, q.B, q.S, q.Q, q.sgn, T, beta, muB, muS, muQ, dsu, Pi,
+                                               cf, kl, MODE == BULK_NONE ? (bool*)nullptr : &ill, dnq);
+                        if (MODE != BULK_NONE && ill) y = static_cast<R>(yield_fp64_global<MODE, EXACT>(a, c, s));
                     }
+                    if (RAW) static_cast<R*>(a.raw)[c * Ns + s] = y;
                 }
+                run += static_cast<double>(y);
+                stage[j] = run;
+                ystage[j] = y;
             }
-            s_tile[cl * nsp + s] = y;
-        }
-        __syncthreads();
-        if constexpr (sizeof(R) == sizeof(float) && MODE != BULK_NONE) {
-            // 1b. deferred FP64 recomputation of the ill-conditioned items
-            const int nfix = min(s_fix[0], kFixupCap);
-            for (int k = threadIdx.x; k < nfix; k += blockDim.x) {
-                const int e = s_fix[1 + k];
-                const int cl = e >> 16, sp = e & 0xFFFF;
-                s_tile[cl * nsp + sp] = static_cast<R>(yield_fp64_global<MODE, EXACT>(a, c0 + cl, sp));
+            __syncthreads();
+            // the chunk's CDF entries: sn consecutive doubles per cell row
+            // (thread t stores entry t % SC of rows t / SC, t / SC + 256 / SC, ...)
+            if (cj_store < sn)
+                for (int cl = cl_store; cl < nc; cl += kYieldThreads / SC)
+                    a.cdf[(c0 + cl) * Ns + s0 + cj_store] = s_stage[cl * (SC + 1) + cj_store];
+            // species sums of the chunk (the group of species j is uniform)
+            if (cj < sn) {
+                double v = 0.;
+                for (int cl = csub; cl < nc; cl += 32) v += static_cast<double>(s_y[cl * (SC + 1) + cj]);
+#pragma unroll
+                for (int d = 16; d > 0; d >>= 1) v += __shfl_xor(v, d, 32);
+                if (csub == 0) s_acc[s0 + cj] += v;
             }
-            if (nfix > 0) __syncthreads();
+            __syncthreads();
         }
-        // 2. species CDF rows: one wave per row
-        double* rowbuf = s_row + wib * nsp;
-        for (int cl = wib; cl < nc; cl += blockDim.x / kWave) {
-            const R* yrow = s_tile + cl * nsp;
-            double run = 0.;
-            for (int s = s_lo; s < s_hi; s++) run += static_cast<double>(yrow[s]);
-            const double excl = wave_inclusive_scan(run) - run;
-            double pre = excl;
-            for (int s = s_lo; s < s_hi; s++) {
-                pre += static_cast<double>(yrow[s]);
-                rowbuf[s] = pre;
-            }
-            const double total = __shfl(excl + run, kWave - 1, kWave);
-            // rowbuf is written and read by this wave only
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int64_t c = c0 + cl;
-            double* row = a.cdf + c * Ns;
-            for (int s = lane; s < Ns; s += kWave) {
-                row[s] = rowbuf[s];
-                if (RAW) static_cast<R*>(a.raw)[c * Ns + s] = yrow[s];
-            }
-            if (lane == 0) a.ctot[c] = total;
-            __builtin_amdgcn_wave_barrier();
-        }
-        // 3. FP64 column sums into the species accumulators
-        for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
-            double v = 0.;
-            for (int cl = 0; cl < nc; cl++) v += static_cast<double>(s_tile[cl * nsp + s]);
-            if (s == static_cast<int>(threadIdx.x)) acc0 += v; else acc1 += v;
-        }
+        if (valid) a.ctot[c] = run;
     }
-    for (int s = threadIdx.x; s < Ns; s += blockDim.x)
-        a.partial[static_cast<int64_t>(blockIdx.x) * Ns + s] = (s == static_cast<int>(threadIdx.x)) ? acc0 : acc1;
+    __syncthreads();
+    for (int s = threadIdx.x; s < Ns; s += blockDim.x) a.partial[static_cast<int64_t>(blockIdx.x) * Ns + s] = s_acc[s];
 }
 
 // FP64 per-species totals: one workgroup per species, fixed summation order
@@ -402,12 +422,12 @@ __global__ void k_diffusion_dn(YieldArgs a, const double* __restrict__ etab, dou
     const int64_t Nc = a.Nc;
     const int Ns = a.Ns;
     const int64_t items = Nc * Ns;
-    const KLookup64<true> kl{nullptr, a.ktab, a.oob};
+    const KLookup64<true> kl{nullptr, a.ktab, a.oob, false};
     const double uf = 1.0 / (kHbarC * kHbarC * kHbarC) / (2. * kPi * kPi);
     for (int64_t it = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; it < items;
          it += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int64_t c = it / Ns;
-        const int s = static_cast<int>(it % Ns);
+        const int s = static_cast<int>(it / Nc);
+        const int64_t c = it - static_cast<int64_t>(s) * Nc;
         const double T = a.cells[F_T * Nc + c], beta = 1. / T;
         const float dqf = dot3_unfused(a.cells[F_QX * Nc + c], a.cells[F_DA1 * Nc + c], a.cells[F_QY * Nc + c],
                                        a.cells[F_DA2 * Nc + c], a.cells[F_QZ * Nc + c], a.cells[F_DA3 * Nc + c]);
