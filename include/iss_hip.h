@@ -106,6 +106,11 @@ typedef struct {
     double y_LB, y_RB;
     int32_t max_trials;      /* maximum_impatience (src/FSSW.cpp:1872) */
     int32_t precision;       /* yield precision used if yields are stale */
+    /* local_charge_conservation (src/FSSW.cpp:931-938, 1035-1048): species of
+     * negative charge are not sampled; every positive hadron is followed, in
+     * its event, by its antiparticle from the same cell (same eta_s).  Needs
+     * every positive species' antiparticle in the species table. */
+    int32_t local_charge_conservation;
 } iss_sample_opts;
 
 typedef struct {

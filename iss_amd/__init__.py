@@ -66,7 +66,8 @@ class _SampleOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("event0", C.c_int64), ("n_events", C.c_int32),
                 ("proposal", C.c_int32), ("sampling_model", C.c_int32),
                 ("hydro_mode", C.c_int32), ("y_LB", C.c_double), ("y_RB", C.c_double),
-                ("max_trials", C.c_int32), ("precision", C.c_int32)]
+                ("max_trials", C.c_int32), ("precision", C.c_int32),
+                ("local_charge_conservation", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -298,9 +299,10 @@ class Context:
 
     # -------------------------------------------------------------- sampling
     def sample(self, n_events, seed=1, event0=0, hydro_mode=2, y_LB=-5.0, y_RB=5.0,
-               max_trials=5000, precision=32, proposal=0, sampling_model=30) -> int:
+               max_trials=5000, precision=32, proposal=0, sampling_model=30,
+               local_charge_conservation=0) -> int:
         o = _SampleOpts(seed, event0, n_events, proposal, sampling_model, hydro_mode, y_LB, y_RB,
-                        max_trials, precision)
+                        max_trials, precision, int(local_charge_conservation))
         n = C.c_int64()
         self._check(self.lib.iss_sample(self.h, C.byref(o), C.byref(n)), "iss_sample")
         self.n_events = n_events

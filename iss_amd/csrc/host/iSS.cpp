@@ -202,11 +202,6 @@ int iSS::upload_species_and_tables(std::string* err) {
     std::vector<int> missing;
     const auto order = iss_io::sampling_order(particle_, iss_io::read_chosen(chosen), &missing);
     if (!missing.empty()) warn("not all chosen particles are in the pdg particle list!");
-    const int lcc = static_cast<int>(paraRdr_ptr->getVal("local_charge_conservation", 0));
-    if (lcc) {
-        *err = "local_charge_conservation = 1 is not supported";
-        return ISS_ERR_UNSUPPORTED;
-    }
     const int n = static_cast<int>(order.size());
     std::vector<int> pid(n), g(n), B(n), S(n), Q(n), sg(n);
     std::vector<double> m(n);
@@ -322,6 +317,8 @@ int iSS::generate_samples() {
     so.y_RB = yRB;
     so.max_trials = 5000;
     so.precision = precision;
+    // FSSW's local_charge_conservation (src/FSSW.cpp:85, 931-938, 1035-1048)
+    so.local_charge_conservation = static_cast<int>(paraRdr_ptr->getVal("local_charge_conservation", 0)) == 1;
     release_hadrons();
     const bool decays = static_cast<int>(paraRdr_ptr->getVal("perform_decays", 0)) == 1;
     if (decays) {
@@ -393,7 +390,11 @@ int iSS::generate_samples() {
         }
         return ISS_OK;
     };
-    const int proposal = static_cast<int>(paraRdr_ptr->getVal("sampling_proposal", 0));
+    int proposal = static_cast<int>(paraRdr_ptr->getVal("sampling_proposal", 0));
+    if (proposal == 1 && so.local_charge_conservation) {
+        warn("sampling_proposal = 1 with local_charge_conservation = 1: the plain proposal is used");
+        proposal = 0;
+    }
     if (proposal == 1) {
         // learned proposal: the first events (about 2e6 hadrons) come from the
         // plain sampler; the network is trained on their accepted hadrons and

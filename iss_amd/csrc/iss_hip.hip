@@ -87,7 +87,15 @@ struct iss_ctx {
     int num_cu = 256;
     // species
     int Ns = 0;
-    std::vector<int> h_pid;
+    std::vector<int> h_pid, h_charge, h_anti;
+    DBuf<int> d_anti;
+    // local charge conservation: output positions' flags, partner offsets,
+    // sources, partner set-up records, the second output list, new event offsets
+    DBuf<uint32_t> d_lcc_flag;
+    DBuf<int64_t> d_lcc_g, d_lcc_src, d_lcc_evoff;
+    DBuf<HSetup> d_lcc_rec;
+    DBuf<Hadron> d_out2;
+    int yields_lcc = 0;
     DBuf<double> d_mass;
     DBuf<SpeciesK> d_spk;
     DBuf<int> d_ints;  // g, B, S, Q, sign, pid
@@ -351,7 +359,7 @@ int launch_yield(iss_ctx* ctx, const YieldArgs& a, bool raw, bool exact) {
     }
 }
 
-int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_host) {
+int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_host, int lcc = 0) {
     if (ctx->Ns == 0) return set_err(ctx, ISS_ERR_STATE, "species not set");
     if (precision != 32 && precision != 64) return set_err(ctx, ISS_ERR_ARG, "precision must be 32 or 64");
     int rc = ensure_ktab(ctx);
@@ -382,6 +390,7 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     a.partial = ctx->d_partial.p;
     a.dnq = nullptr;
     a.spk = ctx->d_spk.p;
+    a.lcc = lcc;
     HIPCHK(ctx->d_oob.ensure(1));
     a.oob = ctx->d_oob.p;
     HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
@@ -426,6 +435,7 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     ctx->stats.ms_yield = ms2;
     ctx->yields_valid = true;
     ctx->yield_precision = precision;
+    ctx->yields_lcc = lcc;
     return ISS_OK;
 }
 
@@ -648,6 +658,8 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_ctot.release(); ctx->d_partial.release(); ctx->d_oob.release(); ctx->d_stot.release(); ctx->d_raw.release();
     ctx->d_counts.release(); ctx->d_evcount.release(); ctx->d_off.release(); ctx->d_evoff.release();
     ctx->d_tree0.release(); ctx->d_tree1.release(); ctx->d_out.release(); ctx->d_setup.release(); ctx->d_stats.release();
+    ctx->d_anti.release(); ctx->d_lcc_flag.release(); ctx->d_lcc_g.release(); ctx->d_lcc_src.release();
+    ctx->d_lcc_evoff.release(); ctx->d_lcc_rec.release(); ctx->d_out2.release();
     ctx->d_aux.release(); ctx->d_mlp_par.release(); ctx->d_adam_m.release(); ctx->d_adam_v.release();
     ctx->d_grad.release(); ctx->d_mlp_frag.release(); ctx->d_fb_list.release(); ctx->d_fb_count.release(); ctx->d_lrec.release();
     ctx->d_tpart.release(); ctx->d_tloss.release(); ctx->d_tout.release(); ctx->d_tr_cell.release();
@@ -707,6 +719,17 @@ int iss_set_species(iss_ctx* ctx, const iss_species_desc* sp) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->Ns = Ns;
     ctx->h_pid.assign(sp->pid, sp->pid + Ns);
+    ctx->h_charge.assign(sp->charge, sp->charge + Ns);
+    // antiparticle entries (local charge conservation)
+    ctx->h_anti.assign(Ns, -1);
+    for (int i = 0; i < Ns; i++)
+        for (int j = 0; j < Ns; j++)
+            if (sp->pid[j] == -sp->pid[i]) {
+                ctx->h_anti[i] = j;
+                break;
+            }
+    HIPCHK(ctx->d_anti.ensure(Ns));
+    HIPCHK(hipMemcpy(ctx->d_anti.p, ctx->h_anti.data(), sizeof(int) * Ns, hipMemcpyHostToDevice));
     ctx->yields_valid = false;
     if (!ctx->tables_ready) {
         iss_deltaf_desc d{};
@@ -882,8 +905,16 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     if (o->proposal == 1 && !ctx->mlp_ready)
         return set_err(ctx, ISS_ERR_STATE, "learned proposal: no network parameters (init, set or train)");
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
-    if (!ctx->yields_valid) {
-        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr);
+    const int lcc = o->local_charge_conservation ? 1 : 0;
+    if (lcc) {
+        for (int s = 0; s < ctx->Ns; s++)
+            if (ctx->h_charge[s] > 0 && ctx->h_anti[s] < 0)
+                return set_err(ctx, ISS_ERR_ARG,
+                               "local charge conservation: the antiparticle of pid " + std::to_string(ctx->h_pid[s]) +
+                                   " is not in the species table");
+    }
+    if (!ctx->yields_valid || ctx->yields_lcc != lcc) {
+        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc);
         if (rc) return rc;
     }
     const int64_t Nc = ctx->Nc;
@@ -952,6 +983,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
     a.Kdev = nullptr;
+    a.anti = ctx->d_anti.p;
     int per_cu_trials = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, 256, 0) != hipSuccess ||
         per_cu_trials < 1)
@@ -1006,6 +1038,45 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         }
     }
     HIPCHK(hipGetLastError());
+    if (lcc && K > 0) {
+        // partners of the positive hadrons (src/FSSW.cpp:1035-1048)
+        HIPCHK(ctx->d_lcc_flag.ensure(K));
+        HIPCHK(ctx->d_lcc_g.ensure(K + 1));
+        HIPCHK(ctx->d_lcc_src.ensure(K));
+        HIPCHK(ctx->d_lcc_evoff.ensure(E + 1));
+        k_lcc_flag<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_lcc_flag.p, ctx->d_lcc_src.p);
+        HIPCHK(hipGetLastError());
+        rc = launch_scan(ctx, ctx->d_lcc_flag.p, K, ctx->d_lcc_g.p);
+        if (rc) return rc;
+        int64_t P = 0;
+        HIPCHK(hipMemcpyAsync(&P, ctx->d_lcc_g.p + K, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx->d_out2.ensure(K + P));
+        HIPCHK(ctx->d_lcc_rec.ensure(std::max<int64_t>(P, 1)));
+        k_lcc_place<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_lcc_flag.p, ctx->d_lcc_g.p,
+                                                              ctx->d_lcc_src.p, ctx->d_out.p, ctx->d_out2.p,
+                                                              ctx->d_lcc_rec.p);
+        HIPCHK(hipGetLastError());
+        if (P > 0) {
+            SampleArgs ap = a;
+            ap.out = ctx->d_out2.p;
+            ap.K = P;
+            ap.Kdev = nullptr;
+            const int nblk = static_cast<int>(
+                std::max<int64_t>(1, std::min<int64_t>((P + 255) / 256, int64_t(ctx->num_cu) * per_cu_plain)));
+            k_trials<<<nblk, 256, 0, ctx->stream>>>(ap, ctx->d_lcc_rec.p);
+            HIPCHK(hipGetLastError());
+        }
+        k_lcc_events<<<(E + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_evoff.p, E, ctx->d_lcc_g.p,
+                                                                    ctx->d_lcc_evoff.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(ctx->d_evoff.p, ctx->d_lcc_evoff.p, sizeof(int64_t) * (E + 1),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        std::swap(ctx->d_out.p, ctx->d_out2.p);
+        std::swap(ctx->d_out.cap, ctx->d_out2.cap);
+        K += P;
+        ctx->last_plain = false;  // output positions no longer match the set-up records
+    }
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     unsigned long long st[6];
     HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
@@ -1411,7 +1482,8 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     }
     const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned;
     if (R > 4096) return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most 4096");
-    if (!ctx->yields_valid || (o->precision != ctx->yield_precision && (o->precision == 32 || o->precision == 64))) {
+    if (!ctx->yields_valid || ctx->yields_lcc ||
+        (o->precision != ctx->yield_precision && (o->precision == 32 || o->precision == 64))) {
         int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr);
         if (rc) return rc;
     }

@@ -84,6 +84,7 @@ struct YieldArgs {
     int* oob;        // set when a Bessel argument leaves the table
     const double* dnq;   // [Ns][Nc] diffusion corrections (k_diffusion_dn); null: none
     const SpeciesK* spk;  // [Ns] species constants of the yield pass
+    int lcc;              // local charge conservation: negative species get no yield
 };
 
 struct SampleArgs {
@@ -121,6 +122,11 @@ struct SampleArgs {
     const unsigned long long* out_base;  // hadrons of earlier rounds
     uint64_t out_cap;                  // hadron slots in out (0: unchecked)
     int64_t setup_cap;                 // set-up records (0: unchecked)
+    // local charge conservation: the antiparticle's species index ([Ns], -1
+    // none); partner records carry kPartnerBit in j
+    const int* anti;
 };
+
+constexpr uint32_t kPartnerBit = 0x80000000u;
 
 }  // namespace issdev

@@ -144,7 +144,19 @@ __device__ __forceinline__ void emit_lab(const SampleArgs& a, uint32_t chunk, in
     const float mT = sqrtf(lx * lx + ly * ly + fm * fm);
     const float tau = a.cells[F_TAU * Nc + c], eta = a.cells[F_ETA * Nc + c];
     float eta_s, E, pzl;
-    if (a.boost_invariant) {
+    float t_par = 0.f, z_par = 0.f;
+    const bool partner = (jj & kPartnerBit) != 0u;
+    if (a.boost_invariant && partner) {
+        // a local-charge-conservation partner keeps its parent's eta_s
+        // (src/FSSW.cpp:1044-1046); the parent is the previous hadron
+        const Hadron hp = static_cast<const Hadron*>(a.out)[outp - 1];
+        t_par = hp.t;
+        z_par = hp.z;
+        eta_s = 0.5f * logf((hp.t + hp.z) / (hp.t - hp.z));
+        const float rap = eta_s + (asinhf(lz / mT) - eta);
+        E = mT * coshf(rap);
+        pzl = mT * sinhf(rap);
+    } else if (a.boost_invariant) {
         Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, chunk) ^ 0x00800000u);
         const float rap = a.yLB + (a.yRB - a.yLB) * static_cast<float>(st.uniform());
         eta_s = rap - (asinhf(lz / mT) - eta);
@@ -167,6 +179,10 @@ __device__ __forceinline__ void emit_lab(const SampleArgs& a, uint32_t chunk, in
     h.x = a.cells[F_X * Nc + c];
     h.y = a.cells[F_Y * Nc + c];
     h.z = tau * sinhf(eta_s);
+    if (a.boost_invariant && partner) {
+        h.t = t_par;
+        h.z = z_par;
+    }
     static_cast<Hadron*>(a.out)[outp] = h;
 }
 
@@ -236,6 +252,27 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
     q.fsg = static_cast<float>(a.sp.sign[s]);
 }
 
+// the species-dependent part of a set-up record: the |p| table, its first
+// row and the proposal's normalisation at E~ = m~ - mu~
+__device__ __forceinline__ HSetup setup_record(const SampleArgs& a, int64_t c, uint32_t jj, int s) {
+    PState q;
+    pstate_of(a, c, s, q);
+    const int sign = a.sp.sign[s];
+    const int tb = ((sign == -1) ? 0 : 3) + ((q.x < 30.) ? 0 : (q.x < 50. ? 1 : 2));
+    double cm[3];
+    pcdf_all(a.pt, tb, q.x, cm);
+    HSetup r;
+    r.cell = static_cast<uint32_t>(c);
+    r.j = jj;
+    r.species = static_cast<uint16_t>(s);
+    r.tb = static_cast<uint8_t>(tb);
+    r.i0 = static_cast<int32_t>((q.x - a.pt.Et0[tb]) / a.pt.dEt[tb]);
+    r.flags = (r.i0 < 0 || r.i0 >= a.pt.len[tb] - 1) ? 1 : 0;
+    r.mterm = cm[2] + 2. * q.mut * cm[1] + q.aa * cm[0];
+    r.out = 0;
+    return r;
+}
+
 // Set-up of every hadron slot, one thread per slot (no divergence): the cell
 // (offsets), the species (the cell's FP64 species CDF), the |p| table and its
 // normalisation at E~ = m~ - mu~, and the output slot.
@@ -269,20 +306,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
         s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
         if (s >= a.Ns) s = a.Ns - 1;
     }
-    PState q;
-    pstate_of(a, c, s, q);
-    const int sign = a.sp.sign[s];
-    const int tb = ((sign == -1) ? 0 : 3) + ((q.x < 30.) ? 0 : (q.x < 50. ? 1 : 2));
-    double cm[3];
-    pcdf_all(a.pt, tb, q.x, cm);
-    HSetup r;
-    r.cell = static_cast<uint32_t>(c);
-    r.j = jj;
-    r.species = static_cast<uint16_t>(s);
-    r.tb = static_cast<uint8_t>(tb);
-    r.i0 = static_cast<int32_t>((q.x - a.pt.Et0[tb]) / a.pt.dEt[tb]);
-    r.flags = (r.i0 < 0 || r.i0 >= a.pt.len[tb] - 1) ? 1 : 0;
-    r.mterm = cm[2] + 2. * q.mut * cm[1] + q.aa * cm[0];
+    HSetup r = setup_record(a, c, jj, s);
     if (BATCH) {
         // batched surfaces: slot of the hadron within its surface's block of
         // this round, after the hadrons of the earlier rounds
@@ -516,6 +540,56 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     }
     __syncthreads();
     if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+// ============================================= local charge conservation
+// FSSW's local_charge_conservation (src/FSSW.cpp:931-938, 1035-1048): the
+// yields of negative species are zero, and every positive hadron is followed
+// in its event by its antiparticle, sampled from the same cell with the
+// negated charges (the antiparticle's species entry).  After the main pass:
+//   k_lcc_flag    per slot, its species (the same draw as k_setup) and
+//                 whether it is positive, by output position
+//   scan          partner offsets g (exclusive) over output positions
+//   k_lcc_place   hadron o moves to o + g[o]; a positive one leaves the slot
+//                 after it to its partner, whose set-up record is built here
+//   k_trials      samples the partners (j carries kPartnerBit: own streams)
+//   k_lcc_events  event offsets e -> evoff[e] + g[evoff[e]]
+__global__ void k_lcc_flag(SampleArgs a, uint32_t* __restrict__ pflag, int64_t* __restrict__ src) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.K) return;
+    const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
+    const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
+    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+    const double* row = a.cdf + c * a.Ns;
+    int s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
+    if (s >= a.Ns) s = a.Ns - 1;
+    const uint64_t o = a.perm(static_cast<uint64_t>(i));
+    pflag[o] = (a.sp.Q[s] > 0) ? 1u : 0u;
+    src[o] = (i << 10) | s;
+}
+
+__global__ void k_lcc_place(SampleArgs a, const uint32_t* __restrict__ pflag, const int64_t* __restrict__ g,
+                            const int64_t* __restrict__ src, const Hadron* __restrict__ old_out,
+                            Hadron* __restrict__ new_out, HSetup* __restrict__ prec) {
+    const int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o >= a.K) return;
+    const int64_t n = o + g[o];
+    new_out[n] = old_out[o];
+    if (!pflag[o]) return;
+    const int64_t i = src[o] >> 10;
+    const int s = static_cast<int>(src[o] & 1023);
+    const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
+    const uint32_t jj = static_cast<uint32_t>(i - a.off[c]) | kPartnerBit;
+    HSetup r = setup_record(a, c, jj, a.anti[s]);
+    r.out = static_cast<uint64_t>(n + 1);
+    prec[g[o]] = r;
+}
+
+__global__ void k_lcc_events(const int64_t* __restrict__ evoff, int E, const int64_t* __restrict__ g,
+                             int64_t* __restrict__ evoff_new) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > E) return;
+    evoff_new[e] = evoff[e] + g[evoff[e]];
 }
 
 }  // namespace issdev

@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const Spec
                 const SpeciesK q = qn;
                 if (j + 1 < sn) qn = spk[s + 1];  // the next species' constants in flight
                 R y = 0;
-                if (valid) {
+                if (valid && !(a.lcc && q.Q < 0.f)) {
                     const R dnq = a.dnq ? static_cast<R>(a.dnq[static_cast<int64_t>(s) * Nc + c]) : R(0);
                     if constexpr (sizeof(R) == sizeof(double)) {
                         const KLookup64<EXACT> kl{reinterpret_cast<const double*>(s_k), a.ktab, a.oob, true};

@@ -272,3 +272,29 @@ def test_dropin_diffusion_spectra_match_oracle(tmp_path):
         assert err < 5e-5, pid
         if pid != 211:
             assert eff > 1e-2, pid
+
+
+def test_dropin_local_charge_conservation(tmp_path):
+    """local_charge_conservation = 1 (src/FSSW.cpp:931-938, 1035-1048): no
+    negative species is sampled on its own; each positive hadron is followed
+    by its antiparticle from the same cell, so every event is neutral."""
+    _run(tmp_path, "Ideal2", ["use_binary_format=1", "number_of_repeated_sampling=20", "perform_checks=0",
+                              "local_charge_conservation=1"])
+    evs = _read_bin(tmp_path / "particle_samples.bin")
+    assert len(evs) == 20
+    t = ob.decay_table(1)
+    q = {int(m): int(t["Q"][i]) for i, m in reversed(list(enumerate(t["monval"])))}
+    npos = 0
+    for e in evs:
+        assert sum(q[h[0]] for h in e) == 0
+        k = 0
+        while k < len(e):
+            if q[e[k][0]] > 0:
+                npos += 1
+                assert e[k + 1][0] == -e[k][0] and e[k + 1][2:5] == e[k][2:5]  # t, x, y
+                k += 2
+            else:
+                assert q[e[k][0]] == 0
+                k += 1
+    print("positive parents", npos)
+    assert npos > 100
