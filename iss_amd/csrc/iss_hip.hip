@@ -145,6 +145,13 @@ struct iss_ctx {
     int adam_t = 0;
     DBuf<int64_t> d_fb_list;
     DBuf<LRec> d_lrec;
+    // learned sampler's (cell, species) pairs: compact list, species
+    // bitmaps, per-cell counts and first pairs, slot species and records
+    DBuf<uint2> d_pkey;
+    DBuf<uint32_t> d_lbits, d_lcnt;
+    DBuf<int64_t> d_pairoff;
+    DBuf<uint16_t> d_sp16;
+    DBuf<LSlot> d_lslot;
     DBuf<unsigned long long> d_fb_count;
     DBuf<float> d_tpart, d_tloss, d_tout;
     DBuf<uint32_t> d_tr_cell, d_ev_cell;
@@ -662,6 +669,8 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_lcc_evoff.release(); ctx->d_lcc_rec.release(); ctx->d_out2.release();
     ctx->d_aux.release(); ctx->d_mlp_par.release(); ctx->d_adam_m.release(); ctx->d_adam_v.release();
     ctx->d_grad.release(); ctx->d_mlp_frag.release(); ctx->d_fb_list.release(); ctx->d_fb_count.release(); ctx->d_lrec.release();
+    ctx->d_pkey.release(); ctx->d_lbits.release(); ctx->d_lcnt.release(); ctx->d_pairoff.release();
+    ctx->d_sp16.release(); ctx->d_lslot.release();
     ctx->d_tpart.release(); ctx->d_tloss.release(); ctx->d_tout.release(); ctx->d_tr_cell.release();
     ctx->d_ev_cell.release(); ctx->d_tr_spec.release(); ctx->d_ev_spec.release(); ctx->d_i32a.release();
     ctx->d_i32b.release(); ctx->d_f32a.release(); ctx->d_f32b.release(); ctx->d_f32c.release();
@@ -1008,16 +1017,41 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
         HIPCHK(hipMemsetAsync(ctx->d_fb_count.p, 0, sizeof(unsigned long long), ctx->stream));
         if (K > 0) {
-            LearnArgs L;
+            LearnArgs L{};
             L.a = a;
             L.aux = ctx->d_aux.p;
             L.W = MlpDev{ctx->d_mlp_frag.p, ctx->d_mlp_par.p};
             L.fb_list = ctx->d_fb_list.p;
             L.fb_count = ctx->d_fb_count.p;
             L.stats2 = ctx->d_stats.p + 4;
-            HIPCHK(ctx->d_lrec.ensure(std::max<int64_t>(K, 1)));
+            // the pass's (cell, species) pairs: one network evaluation and
+            // band envelope per pair
+            const int Wb = (ctx->Ns + 31) / 32;
+            HIPCHK(ctx->d_pkey.ensure(K));
+            HIPCHK(ctx->d_lslot.ensure(K));
+            HIPCHK(ctx->d_sp16.ensure(K));
+            HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(Nc, 1) * Wb));
+            HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(Nc, 1)));
+            HIPCHK(ctx->d_pairoff.ensure(Nc + 1));
+            const int pblk = static_cast<int>((Nc + kPairCellsPerBlock - 1) / kPairCellsPerBlock);
+            k_lpair_pick<<<pblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, ctx->d_lbits.p, ctx->d_lcnt.p,
+                                                                            ctx->d_sp16.p);
+            HIPCHK(hipGetLastError());
+            rc = launch_scan(ctx, ctx->d_lcnt.p, Nc, ctx->d_pairoff.p);
+            if (rc) return rc;
+            k_lpair_fill<<<pblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, Wb, ctx->d_lbits.p, ctx->d_pairoff.p,
+                                                                            ctx->d_sp16.p, ctx->d_lslot.p,
+                                                                            ctx->d_pkey.p);
+            HIPCHK(hipGetLastError());
+            int64_t P = 0;
+            HIPCHK(hipMemcpyAsync(&P, ctx->d_pairoff.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            L.pkey = ctx->d_pkey.p;
+            L.P = P;
+            L.slot = ctx->d_lslot.p;
+            HIPCHK(ctx->d_lrec.ensure(std::max<int64_t>(P, 1)));
             const int lds = kMlpBytes * (kSetupThreads / kWave);
-            const int64_t nwave = (K + 63) / 64;
+            const int64_t nwave = (P + 63) / 64;
             k_learn_setup<<<(nwave + kSetupThreads / 64 - 1) / (kSetupThreads / 64), kSetupThreads, lds, ctx->stream>>>(
                 L, ctx->d_lrec.p);
             HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));

@@ -726,6 +726,7 @@ __device__ bool band_setup(const LParams& P, const float rho[kOut], BandTable& B
 }
 
 // ======================================================== learned sampler
+struct LSlot;
 struct LearnArgs {
     SampleArgs a;
     const float* aux;
@@ -733,6 +734,11 @@ struct LearnArgs {
     int64_t* fb_list;                 // slots handed to the plain sampler
     unsigned long long* fb_count;
     unsigned long long* stats2;       // [0] fallback hadrons, [1] envelope violations
+    // (cell, species) pairs of the pass ([P] compact list) and each slot's
+    // cell, counter and pair
+    const uint2* pkey;
+    int64_t P;
+    const LSlot* slot;
 };
 
 // Per-hadron record of the learned sampler (96 bytes): the band table with
@@ -752,12 +758,111 @@ struct alignas(16) LRec {
 };
 static_assert(sizeof(LRec) == 96, "LRec layout");
 
+// Per-slot record of the learned sampler: the hadron's cell, its index in
+// the cell (its random streams) and its (cell, species) pair, whose band
+// table every hadron of that pair shares.
+struct alignas(16) LSlot {
+    uint32_t cell;
+    uint32_t j;
+    uint32_t pair;
+    uint16_t species;
+    uint16_t pad;
+};
+static_assert(sizeof(LSlot) == 16, "LSlot layout");
+
+// The pairs of a pass.  One hadron in 1.5 shares its (cell, species) with
+// another in the 200k-cell workload, so the network and the band envelope are
+// evaluated once per pair.  One wave per cell, twice:
+//   k_lpair_pick   the cell's FP64 species CDF row goes to LDS, each slot
+//                  draws its species there (the same draw and search as
+//                  k_setup); the cell's species bitmap and its number of
+//                  pairs are stored
+//   scan           the cells' first pair indices
+//   k_lpair_fill   the compact pair list (cell, species) in bitmap order and
+//                  each slot's record with its pair
+constexpr int kPairCellsPerBlock = 4;
+constexpr int kPairWords = kMaxSpecies / 32;
+
+__global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_pick(SampleArgs a, uint32_t* __restrict__ gbits,
+                                                                          uint32_t* __restrict__ cnt,
+                                                                          uint16_t* __restrict__ sp16) {
+    __shared__ double s_row[kPairCellsPerBlock][kMaxSpecies];
+    __shared__ uint32_t s_bits[kPairCellsPerBlock][kPairWords];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kPairCellsPerBlock + w;
+    if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
+    const int Ns = a.Ns, Wb = (Ns + 31) / 32;
+    const int64_t o0 = a.off[c], n = a.off[c + 1] - o0;
+    uint32_t* bits = s_bits[w];
+    if (n == 0) {
+        if (lane < Wb) gbits[c * Wb + lane] = 0u;
+        if (lane == 0) cnt[c] = 0u;
+        return;
+    }
+    double* row = s_row[w];
+    for (int s = lane; s < Ns; s += 64) row[s] = a.cdf[c * Ns + s];
+    if (lane < Wb) bits[lane] = 0u;
+    const double tot = a.ctot[c];
+    wave_lds_sync();
+    for (int64_t r = lane; r < n; r += 64) {
+        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, a.chunk));
+        int s = upper_bound(row, 0, Ns, st.uniform() * tot);
+        if (s >= Ns) s = Ns - 1;
+        sp16[o0 + r] = static_cast<uint16_t>(s);
+        atomicOr(&bits[s >> 5], 1u << (s & 31));
+    }
+    wave_lds_sync();
+    const uint32_t wd = lane < Wb ? bits[lane] : 0u;
+    if (lane < Wb) gbits[c * Wb + lane] = wd;
+    const uint32_t np = wave_sum(static_cast<uint32_t>(__popc(wd)));
+    if (lane == 0) cnt[c] = np;
+}
+
+__global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_fill(SampleArgs a, int Wb,
+                                                                          const uint32_t* __restrict__ gbits,
+                                                                          const int64_t* __restrict__ pairoff,
+                                                                          const uint16_t* __restrict__ sp16,
+                                                                          LSlot* __restrict__ slot,
+                                                                          uint2* __restrict__ pkey) {
+    __shared__ uint32_t s_bits[kPairCellsPerBlock][kPairWords];
+    __shared__ uint32_t s_pre[kPairCellsPerBlock][kPairWords];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kPairCellsPerBlock + w;
+    if (c >= a.Nc) return;
+    const int64_t o0 = a.off[c], n = a.off[c + 1] - o0;
+    if (n == 0) return;
+    uint32_t* bits = s_bits[w];
+    uint32_t* pre = s_pre[w];
+    const uint32_t wd = lane < Wb ? gbits[c * Wb + lane] : 0u;
+    const uint32_t pc = __popc(wd);
+    const uint32_t incl = wave_inclusive_scan(pc);
+    if (lane < Wb) {
+        bits[lane] = wd;
+        pre[lane] = incl - pc;
+    }
+    wave_lds_sync();
+    const int64_t base = pairoff[c];
+    auto rank = [&](int s) { return pre[s >> 5] + __popc(bits[s >> 5] & ((1u << (s & 31)) - 1u)); };
+    for (int s = lane; s < a.Ns; s += 64)
+        if (bits[s >> 5] & (1u << (s & 31))) pkey[base + rank(s)] = make_uint2(static_cast<uint32_t>(c), s);
+    for (int64_t r = lane; r < n; r += 64) {
+        const int s = sp16[o0 + r];
+        LSlot rec;
+        rec.cell = static_cast<uint32_t>(c);
+        rec.j = static_cast<uint32_t>(r);
+        rec.pair = static_cast<uint32_t>(base + rank(s));
+        rec.species = static_cast<uint16_t>(s);
+        rec.pad = 0;
+        slot[o0 + r] = rec;
+    }
+}
+
+
 constexpr int kSetupThreads = 256;
 constexpr int kMlpBytes = (64 * kXld + 64 * kHld) * 2;
 
-// Set-up of the learned sampler, one wave per 64 consecutive slots: cell,
-// species exactly as k_setup, then the network inputs, the
-// network on MFMA and the band envelope.
+// Set-up of the learned sampler, one wave per 64 consecutive (cell, species)
+// pairs: the network inputs, the network on MFMA and the band envelope.
 __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec* __restrict__ rec) {
     extern __shared__ __align__(16) char lds[];
     const SampleArgs& a = L.a;
@@ -767,22 +872,19 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
     uint16_t* Hs = Xs + 64 * kXld;
     float* Os = reinterpret_cast<float*>(wlds);
     const int64_t s0 = (static_cast<int64_t>(blockIdx.x) * (kSetupThreads / 64) + (threadIdx.x >> 6)) * 64;
-    if (s0 >= a.K) return;  // whole waves only: the MFMA below needs every lane
+    if (s0 >= L.P) return;  // whole waves only: the MFMA below needs every lane
     const int64_t i = s0 + lane;
-    const bool live = i < a.K;
+    const bool live = i < L.P;
     int64_t c = 0;
     int s = 0;
-    uint32_t jj = 0;
+    const uint32_t jj = 0;
     LParams P{};
     float f[kIn];
     for (int k = 0; k < kIn; k++) f[k] = 0.f;
     if (live) {
-        c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
-        jj = static_cast<uint32_t>(i - a.off[c]);
-        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
-        const double* row = a.cdf + c * a.Ns;
-        s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
-        if (s >= a.Ns) s = a.Ns - 1;
+        const uint2 pk = L.pkey[i];
+        c = pk.x;
+        s = static_cast<int>(pk.y);
         lparams_of(a, L.aux, c, s, P);
         features_of(a, c, s, P, f);
     }
@@ -856,10 +958,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const int rank = __popcll(need & ((1ull << lane) - 1));
             if (!active && rank < navail) {
                 cur = next + rank;
-                const uint4* r = reinterpret_cast<const uint4*>(rec + cur);
+                const LSlot sl = L.slot[cur];
+                const uint4* r = reinterpret_cast<const uint4*>(rec + sl.pair);
                 const uint4 h0 = r[0], h1 = r[1], h2 = r[2], h3 = r[3], h4 = r[4], h5 = r[5];
-                c = h0.x;
-                jj = h0.y;
+                c = sl.cell;
+                jj = sl.j;
                 s = static_cast<int>(h0.z & 0xFFFFu);
                 const int flags = static_cast<int>((h0.z >> 16) & 0xFFu);
                 klast = static_cast<int>(h0.z >> 24);
