@@ -52,7 +52,8 @@ OPS_PLAIN_PROPOSAL = 115.0     # |p| proposal: table search step terms, ratio te
 OPS_PLAIN_TRIAL = 85.0         # direction, f0, delta f weights, w1 w2 test
 OPS_ACCEPTED = 60.0            # boost to the lab frame and the record
 OPS_PLAIN_SETUP = 190.0        # species pick, |p| normalisation (FP64 counted as FP32)
-OPS_LEARN_SETUP = 2900.0       # 14 inputs, 128 GELU, 9-band envelope, thresholds
+OPS_LEARN_SETUP = 2900.0       # per (cell, species) pair: 14 inputs, 128 GELU, 9-band envelope, thresholds
+OPS_PAIR_PICK = 60.0           # per hadron: species draw and search, bitmap, slot record
 OPS_LEARN_PROPOSAL = 250.0     # band pick, t, f0, direction, weights, envelope ratio
 # one smooth-spectrum integrand point as the reference writes it
 # (src/emissionfunction.cpp:756-806): pt, pz, p.u, exponent, exp, f0,
@@ -140,7 +141,7 @@ def run_steps(ctx, args, proposal, first_event):
     for i in range(args.warmup):
         step(i)
     acc = {"hadrons": 0, "trials": 0, "inner": 0, "ms_yield": [], "ms_sample": [], "ms_setup": [],
-           "ms_trials": [], "violations": 0, "fallback": 0}
+           "ms_trials": [], "violations": 0, "fallback": 0, "pairs": 0}
     t0 = time.perf_counter()
     for i in range(args.steps):
         n, st = step(args.warmup + i)
@@ -153,6 +154,7 @@ def run_steps(ctx, args, proposal, first_event):
         acc["ms_trials"].append(st.ms_trials_kernel)
         acc["violations"] += st.n_violations
         acc["fallback"] += st.n_fallback
+        acc["pairs"] += st.n_pairs if proposal == 1 else 0
     acc["elapsed"] = time.perf_counter() - t0
     for k in ("ms_yield", "ms_sample", "ms_setup", "ms_trials"):
         acc[k] = float(np.mean(acc[k]))
@@ -321,7 +323,8 @@ def main():
     roof_y = {"bound": "hbm", "achieved": y_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
               "frac": y_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles("k_yield"), "kernel": "k_yield",
               "ms": learned["ms_yield"], "bytes": ybytes}
-    l_setup = roofline_valu("k_learn_setup", OPS_LEARN_SETUP * hl, learned["ms_setup"])
+    pairs = learned["pairs"] / args.steps
+    l_setup = roofline_valu("k_learn_setup", OPS_LEARN_SETUP * pairs + OPS_PAIR_PICK * hl, learned["ms_setup"])
     l_trials = roofline_valu("k_learn_trials", OPS_LEARN_PROPOSAL * learned["inner"] / args.steps +
                              OPS_ACCEPTED * hl, learned["ms_trials"])
     p_setup = roofline_valu("k_setup", OPS_PLAIN_SETUP * hp, plain["ms_setup"])

@@ -129,6 +129,7 @@ typedef struct {
     double ms_trials_kernel; /* device time of the last trial kernel(s), fallback included */
     double ms_decay;         /* device time of the last iss_decay */
     int64_t n_decay_generations;
+    int64_t n_pairs;         /* learned proposal: (cell, species) pairs of the last pass */
 } iss_stats;
 
 /* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */

@@ -77,7 +77,8 @@ class Stats(C.Structure):
                 ("ms_sample_kernel", C.c_double), ("ms_yield_kernel", C.c_double),
                 ("n_fallback", C.c_int64), ("n_violations", C.c_int64),
                 ("ms_setup_kernel", C.c_double), ("ms_trials_kernel", C.c_double),
-                ("ms_decay", C.c_double), ("n_decay_generations", C.c_int64)]
+                ("ms_decay", C.c_double), ("n_decay_generations", C.c_int64),
+                ("n_pairs", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -373,7 +373,7 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     if (rc) return rc;
     const int64_t Nc = ctx->Nc;
     const int Ns = ctx->Ns;
-    HIPCHK(ctx->d_cdf.ensure(std::max<int64_t>(1, Nc) * Ns));
+    HIPCHK(ctx->d_cdf.ensure(std::max<int64_t>(1, Nc) * cdf_stride(Ns)));
     HIPCHK(ctx->d_ctot.ensure(std::max<int64_t>(1, Nc)));
     HIPCHK(ctx->d_stot.ensure(Ns));
     // partials sized for the largest grid the occupancy can give
@@ -1048,6 +1048,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             HIPCHK(hipStreamSynchronize(ctx->stream));
             L.pkey = ctx->d_pkey.p;
             L.P = P;
+            ctx->stats.n_pairs = P;
             L.slot = ctx->d_lslot.p;
             HIPCHK(ctx->d_lrec.ensure(std::max<int64_t>(P, 1)));
             const int lds = kMlpBytes * (kSetupThreads / kWave);

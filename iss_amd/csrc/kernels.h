@@ -4,7 +4,7 @@
 //   cells   float  [kCellFields][Nc]  rest-frame surface, field-major (SoA)
 //   coef    double [8][Nc]            per-cell delta-f coefficients (6 bulk /
 //                                     NEoS, 1 / D_qmu, n_B / (e + P))
-//   cdf     double [Nc][Ns]           per-cell inclusive species prefix of the
+//   cdf     double [Nc][cdf_stride]   per-cell inclusive species prefix of the
 //                                     clamped yields (the species-selection CDF)
 //   ctot    double [Nc]               per-cell total yield (last cdf entry)
 //   ktab    double [3][kKLen]         K1, K2, K3 at x = 0.5 + 0.05 i
@@ -37,6 +37,9 @@ constexpr int kEnTerms = 9;     // E_2, E_4, ..., E_18 (src/FSSW.cpp:1611, 1637-
 constexpr int kCoefRows = 8;    // per-cell coefficients: 6 bulk / NEoS, 1 / D_qmu, n_B / (e + P)
 
 constexpr int kMaxSpecies = 512;
+// the species CDF's row stride: rows start on 128-byte boundaries, so that
+// the yield pass stores whole 128-byte lines (16 species per staging chunk)
+__host__ __device__ constexpr int cdf_stride(int ns) { return (ns + 15) & ~15; }
 constexpr int kMaxChunks = kMaxSpecies / kWave;
 constexpr int kPTabLen = 2001;
 

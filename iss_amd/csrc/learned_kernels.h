@@ -800,7 +800,7 @@ __global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_pick(SampleAr
         return;
     }
     double* row = s_row[w];
-    for (int s = lane; s < Ns; s += 64) row[s] = a.cdf[c * Ns + s];
+    for (int s = lane; s < Ns; s += 64) row[s] = a.cdf[c * cdf_stride(Ns) + s];
     if (lane < Wb) bits[lane] = 0u;
     const double tot = a.ctot[c];
     wave_lds_sync();

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     int s;
     {
         Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, chunk));
-        const double* row = a.cdf + c * a.Ns;
+        const double* row = a.cdf + c * cdf_stride(a.Ns);
         s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
         if (s >= a.Ns) s = a.Ns - 1;
     }
@@ -560,7 +560,7 @@ __global__ void k_lcc_flag(SampleArgs a, uint32_t* __restrict__ pflag, int64_t* 
     const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
     Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
-    const double* row = a.cdf + c * a.Ns;
+    const double* row = a.cdf + c * cdf_stride(a.Ns);
     int s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
     if (s >= a.Ns) s = a.Ns - 1;
     const uint64_t o = a.perm(static_cast<uint64_t>(i));

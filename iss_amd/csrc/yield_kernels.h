@@ -251,7 +251,9 @@ __device__ __forceinline__ void settle(const T& x) {
     asm volatile("" ::"v"(x));
 }
 // species per CDF staging chunk
-constexpr int kYieldChunk = 8;
+// species per CDF staging chunk: one 128-byte line of a CDF row in FP32 mode;
+// half a line in FP64 mode, whose wider staging would halve the occupancy
+__host__ __device__ constexpr int yield_chunk(int esz) { return esz == 8 ? 8 : 16; }
 
 __host__ __device__ constexpr int ktables_for(int mode) {
     return mode == BULK_NONE ? 1 : ((mode == BULK_RTA || mode == BULK_CE) ? 2 : 3);
@@ -263,7 +265,7 @@ __host__ __device__ constexpr int pad_species(int ns) { return (ns + 3) & ~3; }
 __host__ __device__ constexpr size_t yield_lds_bytes(int esz, int ntab, int nsp) {
     return static_cast<size_t>(ntab) * kKLds * esz                                // K rows
            + static_cast<size_t>(nsp) * 8                                         // species sums
-           + static_cast<size_t>(kYieldThreads) * (kYieldChunk + 1) * (8 + esz);  // prefixes, yields
+           + static_cast<size_t>(kYieldThreads) * (yield_chunk(esz) + 1) * (8 + esz);  // prefixes, yields
 }
 
 // Yield pass, one thread per cell (persistent over tiles of 256 cells).  The
@@ -273,18 +275,19 @@ __host__ __device__ constexpr size_t yield_lds_bytes(int esz, int ntab, int nsp)
 //   the yield (FP32, or FP64 for precision 64), with ill-conditioned FP32
 //   items ((|N_eq| + |dN|) > 4 |N|) recomputed in FP64;
 //   the cell's running FP64 prefix, staged in LDS and stored coalesced as the
-//   species CDF row, kYieldChunk species at a time;
-//   the yield, staged beside it: per chunk, 32 threads per species sum the
+//   species CDF row, yield_chunk species at a time;
+//   the yield, staged beside it: per chunk, a group of threads per species sum the
 //   tile's column in FP64 (fixed order) into the workgroup's species sums.
 template <typename R, int MODE, bool RAW, bool EXACT>
 __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const SpeciesK* __restrict__ spk) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NT = ktables_for(MODE);
-    constexpr int SC = kYieldChunk;
+    constexpr int SC = yield_chunk(sizeof(R));
+    constexpr int G = kYieldThreads / SC;  // threads per species in the column sums
     using KT = typename KLds<R>::T;
     const int Ns = a.Ns;
     const int nsp = pad_species(Ns);
-    static_assert(kYieldThreads == 32 * SC, "one 32-lane group per species of a chunk");
+    static_assert(G * SC == kYieldThreads && G <= kWave, "one lane group per species of a chunk");
     KT* s_k = reinterpret_cast<KT*>(smem);
     double* s_acc = reinterpret_cast<double*>(s_k + NT * kKLds);
     double* s_stage = s_acc + nsp;
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const Spec
     double* stage = s_stage + threadIdx.x * (SC + 1);
     R* ystage = s_y + threadIdx.x * (SC + 1);
     // column-sum role: species j of the chunk, cells sub, sub + 32, ...
-    const int cj = threadIdx.x >> 5, csub = threadIdx.x & 31;
+    const int cj = threadIdx.x / G, csub = threadIdx.x % G;
     // store role: entry cj_store of rows cl_store, cl_store + 256 / SC, ...
     const int cj_store = threadIdx.x % SC, cl_store = threadIdx.x / SC;
     const int64_t ntiles = (Nc + kYieldThreads - 1) / kYieldThreads;
@@ -368,13 +371,13 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const Spec
             // (thread t stores entry t % SC of rows t / SC, t / SC + 256 / SC, ...)
             if (cj_store < sn)
                 for (int cl = cl_store; cl < nc; cl += kYieldThreads / SC)
-                    a.cdf[(c0 + cl) * Ns + s0 + cj_store] = s_stage[cl * (SC + 1) + cj_store];
+                    a.cdf[(c0 + cl) * cdf_stride(Ns) + s0 + cj_store] = s_stage[cl * (SC + 1) + cj_store];
             // species sums of the chunk (the group of species j is uniform)
             if (cj < sn) {
                 double v = 0.;
-                for (int cl = csub; cl < nc; cl += 32) v += static_cast<double>(s_y[cl * (SC + 1) + cj]);
+                for (int cl = csub; cl < nc; cl += G) v += static_cast<double>(s_y[cl * (SC + 1) + cj]);
 #pragma unroll
-                for (int d = 16; d > 0; d >>= 1) v += __shfl_xor(v, d, 32);
+                for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
                 if (csub == 0) s_acc[s0 + cj] += v;
             }
             __syncthreads();
