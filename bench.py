@@ -53,7 +53,6 @@ OPS_PLAIN_TRIAL = 85.0         # direction, f0, delta f weights, w1 w2 test
 OPS_ACCEPTED = 60.0            # boost to the lab frame and the record
 OPS_PLAIN_SETUP = 190.0        # species pick, |p| normalisation (FP64 counted as FP32)
 OPS_LEARN_SETUP = 2900.0       # per (cell, species) pair: 14 inputs, 128 GELU, 9-band envelope, thresholds
-OPS_PAIR_PICK = 60.0           # per hadron: species draw and search, bitmap, slot record
 OPS_LEARN_PROPOSAL = 250.0     # band pick, t, f0, direction, weights, envelope ratio
 # one smooth-spectrum integrand point as the reference writes it
 # (src/emissionfunction.cpp:756-806): pt, pz, p.u, exponent, exp, f0,
@@ -324,7 +323,7 @@ def main():
               "frac": y_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles("k_yield"), "kernel": "k_yield",
               "ms": learned["ms_yield"], "bytes": ybytes}
     pairs = learned["pairs"] / args.steps
-    l_setup = roofline_valu("k_learn_setup", OPS_LEARN_SETUP * pairs + OPS_PAIR_PICK * hl, learned["ms_setup"])
+    l_setup = roofline_valu("k_learn_setup", OPS_LEARN_SETUP * pairs, learned["ms_setup"])
     l_trials = roofline_valu("k_learn_trials", OPS_LEARN_PROPOSAL * learned["inner"] / args.steps +
                              OPS_ACCEPTED * hl, learned["ms_trials"])
     p_setup = roofline_valu("k_setup", OPS_PLAIN_SETUP * hp, plain["ms_setup"])
@@ -358,6 +357,8 @@ def main():
         "proposals_per_hadron": {"learned": learned["inner"] / learned["hadrons"],
                                  "plain": plain["inner"] / plain["hadrons"]},
         "envelope_violations": learned["violations"],
+        "learned_pairs_per_step": pairs,
+        "hadrons_per_pair": hl / max(pairs, 1.0),
         "fallback_hadrons": learned["fallback"],
         "training": rep,
         "autotune": tune,

@@ -125,7 +125,8 @@ typedef struct {
     double ms_yield_kernel;  /* device time of the last yield kernel alone */
     int64_t n_fallback;      /* learned proposal: hadrons handed to the plain sampler */
     int64_t n_violations;    /* learned proposal: draws above the envelope (0 when exact) */
-    double ms_setup_kernel;  /* device time of the last per-hadron set-up kernel */
+    double ms_setup_kernel;  /* device time of the last set-up kernel (plain: per hadron;
+                                learned: the network and envelopes, per (cell, species) pair) */
     double ms_trials_kernel; /* device time of the last trial kernel(s), fallback included */
     double ms_decay;         /* device time of the last iss_decay */
     int64_t n_decay_generations;

@@ -3,10 +3,12 @@
     python -m iss_amd.prof stats <kernel_stats.csv>
         kernel-trace summary: calls, total, average and share per kernel
     python -m iss_amd.prof traffic <fetch_dir> <write_dir> [out.json]
-        HBM bytes per dispatch and kernel from two `rocprofv3 --pmc` runs
-        (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  Both counters are
-        scaled so that the largest __amd_rocclr_copyBuffer dispatch reads and
-        writes the size bench.py --calibration-copy-mib copied.
+        HBM bytes of each kernel's largest dispatch (the full-size launch: the
+        autotuner's shorter passes dispatch the same kernels) from two
+        `rocprofv3 --pmc` runs (FETCH_SIZE, then WRITE_SIZE, KiB per
+        dispatch).  Both counters are scaled so that the largest
+        __amd_rocclr_copyBuffer dispatch reads and writes the size
+        bench.py --calibration-copy-mib copied.
     python -m iss_amd.prof counters <pmc_dir> [<pmc_dir> ...]
         mean of every counter per kernel
 """
@@ -56,8 +58,8 @@ def traffic(fetch_dir, write_dir, out=None, copy_mib=256):
             continue
         f = fe.get(k, {}).get("FETCH_SIZE", [])
         w = wr.get(k, {}).get("WRITE_SIZE", [])
-        fb = sum(f) / len(f) * ff * 1024 if f else 0.0
-        wb = sum(w) / len(w) * fw * 1024 if w else 0.0
+        fb = max(f) * ff * 1024 if f else 0.0
+        wb = max(w) * fw * 1024 if w else 0.0
         entry = {"fetch_bytes": fb, "write_bytes": wb, "bytes": fb + wb, "dispatches": len(f)}
         res[k] = entry
         base = k.split("<")[0]
