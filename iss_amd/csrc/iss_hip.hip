@@ -61,6 +61,10 @@ template <typename T>
 struct DBuf {
     T* p = nullptr;
     size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
     hipError_t ensure(size_t n) {
         if (n <= cap && p) return hipSuccess;
         if (p) (void)hipFree(p);
@@ -74,6 +78,22 @@ struct DBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+    }
+};
+
+// one generation of the resonance feed-down: its particles and per-particle
+// bookkeeping (kept by the context and reused across calls)
+struct DecayGen {
+    const Hadron* P = nullptr;
+    int64_t n = 0;
+    DBuf<Hadron> own;
+    DBuf<uint8_t> kind;
+    DBuf<int32_t> chan;
+    DBuf<uint32_t> nchild, size;
+    DBuf<int64_t> off, pos, parent;
+    void release() {
+        own.release(); kind.release(); chan.release(); nchild.release(); size.release();
+        off.release(); pos.release(); parent.release();
     }
 };
 
@@ -95,6 +115,8 @@ struct iss_ctx {
     DBuf<int64_t> d_lcc_g, d_lcc_src, d_lcc_evoff;
     DBuf<HSetup> d_lcc_rec;
     DBuf<Hadron> d_out2;
+    // resonance feed-down: generation buffers and the second output list
+    std::vector<std::unique_ptr<DecayGen>> dgen;
     int yields_lcc = 0;
     DBuf<double> d_mass;
     DBuf<SpeciesK> d_spk;
@@ -678,6 +700,8 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_spart.release(); ctx->d_sout.release(); ctx->d_sspec.release();
     ctx->d_surf_off.release(); ctx->d_bevents.release(); ctx->d_bround.release(); ctx->d_bevcount.release();
     ctx->d_bbase.release(); ctx->d_perms.release(); ctx->d_seg.release(); ctx->d_lam.release();
+    for (auto& g : ctx->dgen) g->release();
+    ctx->dgen.clear();
     ctx->d_dec_int.release(); ctx->d_dec_dbl.release(); ctx->d_qmu.release(); ctx->d_etab.release(); ctx->d_dnq.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -1814,17 +1838,6 @@ int iss_set_decay_table(iss_ctx* ctx, const iss_decay_table* t) {
     return ISS_OK;
 }
 
-namespace {
-struct DecayGen {
-    const Hadron* P = nullptr;
-    int64_t n = 0;
-    DBuf<Hadron> own;
-    DBuf<uint8_t> kind;
-    DBuf<int32_t> chan;
-    DBuf<uint32_t> nchild, size;
-    DBuf<int64_t> off, pos, parent;
-};
-}  // namespace
 
 int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     if (!ctx) return ISS_ERR_ARG;
@@ -1833,8 +1846,10 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
         return set_err(ctx, ISS_ERR_UNSUPPORTED, "decays apply to the hadron list of iss_sample (not to batches)");
     (void)hipSetDevice(ctx->device);
     const int E = ctx->E;
-    std::vector<std::unique_ptr<DecayGen>> gens;
-    gens.emplace_back(new DecayGen);
+    // generation buffers live in the context and only grow
+    auto& gens = ctx->dgen;
+    if (gens.empty()) gens.emplace_back(new DecayGen);
+    int ngen = 1;
     gens[0]->P = ctx->d_out.p;
     gens[0]->n = ctx->K;
     HIPCHK(ctx->d_stats.ensure(8));
@@ -1860,7 +1875,8 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (nn == 0) break;
         if (g + 1 >= kDecayMaxGen) return set_err(ctx, ISS_ERR_RANGE, "decay chains deeper than 12 generations");
-        gens.emplace_back(new DecayGen);
+        if (static_cast<int>(gens.size()) < g + 2) gens.emplace_back(new DecayGen);
+        ngen = g + 2;
         DecayGen& H = *gens[g + 1];
         HIPCHK(H.own.ensure(nn));
         HIPCHK(H.parent.ensure(nn));
@@ -1871,7 +1887,7 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
         HIPCHK(hipGetLastError());
     }
     // final descendants, bottom-up
-    const int last = static_cast<int>(gens.size()) - 1;
+    const int last = ngen - 1;
     for (int g = last; g >= 0; g--) {
         DecayGen& G = *gens[g];
         if (G.n == 0) continue;
@@ -1891,7 +1907,7 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
         k_decay_place<<<(G.n + 255) / 256, 256, 0, ctx->stream>>>(G.kind.p, G.off.p, G.nchild.p, G.pos.p,
                                                                   gens[g + 1]->size.p, G.n, gens[g + 1]->pos.p);
     }
-    DBuf<Hadron> out;
+    DBuf<Hadron>& out = ctx->d_out2;
     HIPCHK(out.ensure(std::max<int64_t>(F, 1)));
     for (int g = 0; g <= last; g++) {
         DecayGen& G = *gens[g];
@@ -1913,10 +1929,8 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    gens.clear();
     std::swap(ctx->d_out.p, out.p);
     std::swap(ctx->d_out.cap, out.cap);
-    out.release();
     ctx->K = F;
     ctx->stats.n_hadrons = F;
     ctx->stats.ms_decay = ms;
