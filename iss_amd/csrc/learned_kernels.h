@@ -349,6 +349,10 @@ __device__ __forceinline__ float gelu_sig(float x) {
     return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-2.8853900817779268f * u));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
+// the logistic function with the hardware exp2 and reciprocal (1 ulp each)
+__device__ __forceinline__ float sigmoid_fast(float z) {
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
 __device__ __forceinline__ float gelu_d(float x) {
     const float sg = gelu_sig(x);
     return sg + 2.f * x * sg * (1.f - sg) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * x * x);
@@ -894,7 +898,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
     mlp_forward_wave(L.W, Xs, Hs, Os);
     if (!live) return;
     float rho[kOut];
-    for (int k = 0; k < kOut; k++) rho[k] = 1.f / (1.f + expf(-Os[lane * kOut + k]));
+    for (int k = 0; k < kOut; k++) rho[k] = sigmoid_fast(Os[lane * kOut + k]);
     BandTable B;
     int flags = P.flags & 3;
     if (!flags && !band_setup<false>(P, rho, B, nullptr, nullptr)) flags = 2;
@@ -1222,7 +1226,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs t) {
     for (int k = 0; k < kOut; k++) dz[k] = 0.f;
     if (valid) {
         float rho[kOut];
-        for (int k = 0; k < kOut; k++) rho[k] = 1.f / (1.f + expf(-Os[l * kOut + k]));
+        for (int k = 0; k < kOut; k++) rho[k] = sigmoid_fast(Os[l * kOut + k]);
         BandTable B;
         if (!band_setup<true>(P, rho, B, &lm, dz)) { valid = false; lm = 0.f; }
     }
