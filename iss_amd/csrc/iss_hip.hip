@@ -1067,16 +1067,14 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
                                                                             ctx->d_sp16.p, ctx->d_lslot.p,
                                                                             ctx->d_pkey.p);
             HIPCHK(hipGetLastError());
-            int64_t P = 0;
-            HIPCHK(hipMemcpyAsync(&P, ctx->d_pairoff.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
+            // the pair count stays on the device: the set-up grid covers the
+            // bound P <= K and the waves beyond P return at once
             L.pkey = ctx->d_pkey.p;
-            L.P = P;
-            ctx->stats.n_pairs = P;
+            L.P = ctx->d_pairoff.p + Nc;
             L.slot = ctx->d_lslot.p;
-            HIPCHK(ctx->d_lrec.ensure(std::max<int64_t>(P, 1)));
+            HIPCHK(ctx->d_lrec.ensure(std::max<int64_t>(K, 1)));
             const int lds = kMlpBytes * (kSetupThreads / kWave);
-            const int64_t nwave = (P + 63) / 64;
+            const int64_t nwave = (K + 63) / 64;
             HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
             k_learn_setup<<<(nwave + kSetupThreads / 64 - 1) / (kSetupThreads / 64), kSetupThreads, lds, ctx->stream>>>(
                 L, ctx->d_lrec.p);
@@ -1140,6 +1138,9 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     unsigned long long st[6];
     HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    int64_t npairs = 0;
+    if (o->proposal == 1 && K > 0)
+        HIPCHK(hipMemcpyAsync(&npairs, ctx->d_pairoff.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms_all = 0.f, ms_k = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
@@ -1161,6 +1162,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     ctx->stats.ms_sample_kernel = ms_k;
     ctx->stats.n_fallback = static_cast<int64_t>(st[4]);
     ctx->stats.n_violations = static_cast<int64_t>(st[5]);
+    if (o->proposal == 1) ctx->stats.n_pairs = npairs;
     if (n_hadrons) *n_hadrons = K;
     if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
     return ISS_OK;

@@ -738,10 +738,10 @@ struct LearnArgs {
     int64_t* fb_list;                 // slots handed to the plain sampler
     unsigned long long* fb_count;
     unsigned long long* stats2;       // [0] fallback hadrons, [1] envelope violations
-    // (cell, species) pairs of the pass ([P] compact list) and each slot's
-    // cell, counter and pair
+    // (cell, species) pairs of the pass ([*P] compact list; the count lives on
+    // the device) and each slot's cell, counter and pair
     const uint2* pkey;
-    int64_t P;
+    const int64_t* P;
     const LSlot* slot;
 };
 
@@ -876,9 +876,10 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
     uint16_t* Hs = Xs + 64 * kXld;
     float* Os = reinterpret_cast<float*>(wlds);
     const int64_t s0 = (static_cast<int64_t>(blockIdx.x) * (kSetupThreads / 64) + (threadIdx.x >> 6)) * 64;
-    if (s0 >= L.P) return;  // whole waves only: the MFMA below needs every lane
+    const int64_t npairs = *L.P;
+    if (s0 >= npairs) return;  // whole waves only: the MFMA below needs every lane
     const int64_t i = s0 + lane;
-    const bool live = i < L.P;
+    const bool live = i < npairs;
     int64_t c = 0;
     int s = 0;
     const uint32_t jj = 0;
