@@ -154,6 +154,7 @@ struct iss_ctx {
     DBuf<int64_t> d_off, d_evoff, d_tree0, d_tree1;
     DBuf<Hadron> d_out;
     DBuf<HSetup> d_setup;
+    DBuf<uint32_t> d_cellidx;  // the cell of every slot of the pass
     DBuf<unsigned long long> d_stats;
     int64_t K = 0;
     int E = 0;
@@ -687,6 +688,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_ctot.release(); ctx->d_partial.release(); ctx->d_oob.release(); ctx->d_stot.release(); ctx->d_raw.release();
     ctx->d_counts.release(); ctx->d_evcount.release(); ctx->d_off.release(); ctx->d_evoff.release();
     ctx->d_tree0.release(); ctx->d_tree1.release(); ctx->d_out.release(); ctx->d_setup.release(); ctx->d_stats.release();
+    ctx->d_cellidx.release();
     ctx->d_anti.release(); ctx->d_lcc_flag.release(); ctx->d_lcc_g.release(); ctx->d_lcc_src.release();
     ctx->d_lcc_evoff.release(); ctx->d_lcc_rec.release(); ctx->d_out2.release();
     ctx->d_aux.release(); ctx->d_mlp_par.release(); ctx->d_adam_m.release(); ctx->d_adam_v.release();
@@ -1017,6 +1019,12 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.stats = ctx->d_stats.p;
     a.Kdev = nullptr;
     a.anti = ctx->d_anti.p;
+    HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(K, 1)));
+    if (K > 0) {
+        k_cell_index<<<(Nc + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, Nc, 0, ctx->d_cellidx.p);
+        HIPCHK(hipGetLastError());
+        a.cellidx = ctx->d_cellidx.p;
+    }
     int per_cu_trials = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, 256, 0) != hipSuccess ||
         per_cu_trials < 1)
@@ -1609,6 +1617,8 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     a.out_base = ctx->d_bbase.p;
     a.out_cap = static_cast<uint64_t>(out_cap);
     a.setup_cap = setup_cap;
+    HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(setup_cap, 1)));
+    a.cellidx = ctx->d_cellidx.p;
     a.boost_invariant = binv ? 1 : 0;
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
@@ -1639,6 +1649,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         }
         k_batch_multi_leaves<<<((static_cast<int64_t>(S) << D) + 255) / 256, 256, 0, ctx->stream>>>(
             b, cur, D, ctx->d_bevcount.p);
+        if (N > 0) k_cell_index<<<(N + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, N, setup_cap, ctx->d_cellidx.p);
         k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
         k_trials<<<trial_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
         k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);

@@ -128,6 +128,8 @@ struct SampleArgs {
     // local charge conservation: the antiparticle's species index ([Ns], -1
     // none); partner records carry kPartnerBit in j
     const int* anti;
+    // the cell of every slot of the pass (k_cell_index); null: search the offsets
+    const uint32_t* cellidx;
 };
 
 constexpr uint32_t kPartnerBit = 0x80000000u;

@@ -252,6 +252,17 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
     q.fsg = static_cast<float>(a.sp.sign[s]);
 }
 
+// The cell of every hadron slot of a pass: a thread per cell writes its
+// slots' cell index (the set-up then reads it instead of searching the cell
+// offsets per slot).  Slots at or beyond `cap` (> 0) are not written.
+__global__ void k_cell_index(const int64_t* __restrict__ off, int64_t Nc, int64_t cap, uint32_t* __restrict__ cellidx) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    int64_t o1 = off[c + 1];
+    if (cap > 0 && o1 > cap) o1 = cap;
+    for (int64_t i = off[c]; i < o1; i++) cellidx[i] = static_cast<uint32_t>(c);
+}
+
 // the species-dependent part of a set-up record: the |p| table, its first
 // row and the proposal's normalisation at E~ = m~ - mu~
 __device__ __forceinline__ HSetup setup_record(const SampleArgs& a, int64_t c, uint32_t jj, int s) {
@@ -297,7 +308,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     for (int64_t ri = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; ri < n;
          ri += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t i = list ? list[ri] : ri;
-    const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
+    const int64_t c = a.cellidx ? static_cast<int64_t>(a.cellidx[i]) : upper_bound(a.off + 1, int64_t(0), a.Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
     int s;
     {
@@ -557,7 +568,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
 __global__ void k_lcc_flag(SampleArgs a, uint32_t* __restrict__ pflag, int64_t* __restrict__ src) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.K) return;
-    const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
+    const int64_t c = a.cellidx ? static_cast<int64_t>(a.cellidx[i]) : upper_bound(a.off + 1, int64_t(0), a.Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
     Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
     const double* row = a.cdf + c * cdf_stride(a.Ns);
