@@ -139,3 +139,21 @@ def test_closed_channel_is_an_error(species):
         ctx.decay(seed=1)
     assert ctx.event_counts().sum() == 0
     ctx.close()
+
+
+def test_empty_list_and_stable_only(species):
+    """no hadrons, and a list of stable hadrons only: nothing to decay"""
+    ctx = _ctx(species)
+    ctx.set_hadrons(np.array([0, 0, 0]), np.zeros(0, dtype=iss_amd.HADRON_DTYPE))
+    assert ctx.decay(seed=1) == 0
+    assert np.array_equal(ctx.event_counts(), [0, 0, 0])
+    t = ob.decay_table(1)
+    st = np.nonzero(t["stable"] == 1)[0]
+    rng = np.random.default_rng(2)
+    h = _hadrons(t, rng.choice(st, 30), rng)
+    ctx.set_hadrons(np.array([10, 0, 20]), h)
+    assert ctx.decay(seed=1) == 30
+    assert np.array_equal(ctx.event_counts(), [10, 0, 20])
+    assert np.array_equal(ctx.hadrons(), h)
+    assert ctx.stats().n_decay_generations == 1
+    ctx.close()

@@ -188,3 +188,21 @@ def test_sampled_baryons_match_smooth_spectrum_with_diffusion(species):
     assert p0 < 1e-6
     ctx.close()
     ctx0.close()
+
+
+def test_zero_current_equals_no_diffusion(species):
+    """q = 0 everywhere: the diffusion terms vanish identically (yields in
+    both precisions, the sampled list of the same seed)"""
+    cells, _ = synth.viscous_lrf(2000, species, seed=19, muB_max=0.3)
+    cells = cells.copy()
+    cells[:, 25:28] = 0.0
+    out = {}
+    for d in (0, 1):
+        ctx = _ctx(species, MODES[0][1], d)
+        ctx.set_surface_lrf(cells)
+        out[d] = [ctx.compute_yields(precision=p, per_cell=True)[1] for p in (64, 32)]
+        ctx.sample(5, seed=3)
+        out[d].append(ctx.hadrons())
+        ctx.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)

@@ -150,3 +150,22 @@ def test_missing_antiparticle_is_an_error(species):
         ctx.sample(2, seed=1, local_charge_conservation=1)
     assert ctx.sample(2, seed=1) > 0
     ctx.close()
+
+
+def test_empty_surface_and_single_cell_with_lcc(species):
+    ctx = _ctx(species)
+    ctx.set_surface_lrf(np.zeros((0, 28), dtype=np.float32))
+    assert ctx.sample(4, seed=1, local_charge_conservation=1) == 0
+    assert np.all(ctx.event_counts() == 0)
+    cells, _ = synth.viscous_lrf(1, species, seed=9, target_hadrons=80.0)
+    ctx.set_surface_lrf(cells)
+    n = ctx.sample(50, seed=2, local_charge_conservation=1)
+    h = ctx.hadrons()
+    counts = ctx.event_counts()
+    assert n == h.size == counts.sum() > 0
+    _check_structure(h, counts, species)
+    # the switch also turns back off: negative species are sampled again
+    ctx.sample(50, seed=2)
+    q = dict(zip(species["pid"].tolist(), species["Q"].tolist()))
+    assert any(q[int(p)] < 0 for p in ctx.hadrons()["pid"])
+    ctx.close()
