@@ -101,7 +101,8 @@ typedef struct {
     int64_t event0;          /* index of the first event (counter-based RNG) */
     int32_t n_events;
     int32_t proposal;        /* 0 plain rejection, 1 learned mixture */
-    int32_t sampling_model;  /* 30 Poisson (src/FSSW.cpp:293-298) */
+    int32_t sampling_model;  /* dN_dy_sampling_model (src/FSSW.cpp:250-309): 30 Poisson, 1 integer part
+                                plus one with the fraction's probability, 10 and 20 negative binomial */
     int32_t hydro_mode;
     double y_LB, y_RB;
     int32_t max_trials;      /* maximum_impatience (src/FSSW.cpp:1872) */
@@ -111,6 +112,8 @@ typedef struct {
      * its event, by its antiparticle from the same cell (same eta_s).  Needs
      * every positive species' antiparticle in the species table. */
     int32_t local_charge_conservation;
+    /* dN_dy_sampling_para1 of the multiplicity models 10 and 20 (src/FSSW.cpp:275-292) */
+    double sampling_para1;
 } iss_sample_opts;
 
 typedef struct {

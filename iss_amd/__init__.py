@@ -67,7 +67,7 @@ class _SampleOpts(C.Structure):
                 ("proposal", C.c_int32), ("sampling_model", C.c_int32),
                 ("hydro_mode", C.c_int32), ("y_LB", C.c_double), ("y_RB", C.c_double),
                 ("max_trials", C.c_int32), ("precision", C.c_int32),
-                ("local_charge_conservation", C.c_int32)]
+                ("local_charge_conservation", C.c_int32), ("sampling_para1", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -301,9 +301,14 @@ class Context:
     # -------------------------------------------------------------- sampling
     def sample(self, n_events, seed=1, event0=0, hydro_mode=2, y_LB=-5.0, y_RB=5.0,
                max_trials=5000, precision=32, proposal=0, sampling_model=30,
-               local_charge_conservation=0) -> int:
+               local_charge_conservation=0, sampling_para1=0.16) -> int:
+        """One sampling pass of n_events events (FSSW's per-event loop,
+        src/FSSW.cpp:884-1070).  sampling_model is dN_dy_sampling_model: 30
+        Poisson per cell, 1 / 10 / 20 per (event, species) from the species
+        yield (src/FSSW.cpp:250-309), with sampling_para1 the negative
+        binomial parameter of 10 and 20."""
         o = _SampleOpts(seed, event0, n_events, proposal, sampling_model, hydro_mode, y_LB, y_RB,
-                        max_trials, precision, int(local_charge_conservation))
+                        max_trials, precision, int(local_charge_conservation), float(sampling_para1))
         n = C.c_int64()
         self._check(self.lib.iss_sample(self.h, C.byref(o), C.byref(n)), "iss_sample")
         self.n_events = n_events

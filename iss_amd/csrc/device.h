@@ -108,6 +108,46 @@ __device__ inline int64_t poisson(double lam, Stream& s) {
     return static_cast<int64_t>(floor(lam + 0.5));  // unreachable in practice
 }
 
+// standard normal by the polar method (Marsaglia and Bray 1964)
+__device__ inline double std_normal(Stream& s) {
+    for (int it = 0; it < 1000; it++) {
+        const double u = 2. * s.uniform() - 1., v = 2. * s.uniform() - 1.;
+        const double q = u * u + v * v;
+        if (q > 0. && q < 1.) return u * sqrt(-2. * log(q) / q);
+    }
+    return 0.;
+}
+
+// Gamma(shape, 1): G. Marsaglia and W. W. Tsang, "A simple method for
+// generating gamma variables", ACM TOMS 26 (2000) 363-372; shape < 1 through
+// Gamma(shape + 1) U^(1/shape)
+__device__ inline double gamma_variate(double shape, Stream& s) {
+    if (!(shape > 0.)) return 0.;
+    double boost = 1.;
+    if (shape < 1.) {
+        boost = pow(s.uniform(), 1. / shape);
+        shape += 1.;
+    }
+    const double d = shape - 1. / 3., c = 1. / sqrt(9. * d);
+    for (int it = 0; it < 10000; it++) {
+        double x, v;
+        do {
+            x = std_normal(s);
+            v = 1. + c * x;
+        } while (v <= 0.);
+        v = v * v * v;
+        const double u = s.uniform();
+        if (u < 1. - 0.0331 * x * x * x * x || log(u) < 0.5 * x * x + d * (1. - v + log(v))) return d * v * boost;
+    }
+    return d * boost;
+}
+
+// negative binomial with success probability p and n successes (mean
+// n (1 - p) / p), as the Gamma-Poisson mixture Poisson(Gamma(n) (1 - p) / p)
+__device__ inline int64_t negative_binomial(double p, double n, Stream& s) {
+    return poisson(gamma_variate(n, s) * (1. - p) / p, s);
+}
+
 // Binomial(n, p), exact.  n p < 10 (after p -> min(p, 1-p)): inversion;
 // otherwise BTRS of W. Hoermann, "The generation of binomial random variates",
 // J. Statist. Comput. Simul. 46 (1993) 101-110.

@@ -312,6 +312,7 @@ int iSS::generate_samples() {
     so.n_events = nev;
     so.proposal = 0;
     so.sampling_model = static_cast<int>(paraRdr_ptr->getVal("dN_dy_sampling_model"));
+    so.sampling_para1 = paraRdr_ptr->getVal("dN_dy_sampling_para1", 0.16);
     so.hydro_mode = hydro_mode;
     so.y_LB = yLB;
     so.y_RB = yRB;
@@ -393,6 +394,10 @@ int iSS::generate_samples() {
     int proposal = static_cast<int>(paraRdr_ptr->getVal("sampling_proposal", 0));
     if (proposal == 1 && so.local_charge_conservation) {
         warn("sampling_proposal = 1 with local_charge_conservation = 1: the plain proposal is used");
+        proposal = 0;
+    }
+    if (proposal == 1 && so.sampling_model != 30) {
+        warn("sampling_proposal = 1 with dN_dy_sampling_model != 30: the plain proposal is used");
         proposal = 0;
     }
     if (proposal == 1) {

@@ -155,6 +155,12 @@ struct iss_ctx {
     DBuf<Hadron> d_out;
     DBuf<HSetup> d_setup;
     DBuf<uint32_t> d_cellidx;  // the cell of every slot of the pass
+    // multiplicity models 1, 10, 20: species-major cell prefixes, per (event,
+    // species) counts and offsets, slot species
+    DBuf<double> d_ycol;
+    DBuf<uint32_t> d_mcnt;
+    DBuf<int64_t> d_msegoff;
+    DBuf<uint16_t> d_msp;
     DBuf<unsigned long long> d_stats;
     int64_t K = 0;
     int E = 0;
@@ -389,7 +395,8 @@ int launch_yield(iss_ctx* ctx, const YieldArgs& a, bool raw, bool exact) {
     }
 }
 
-int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_host, int lcc = 0) {
+int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_host, int lcc = 0,
+               double* ycol = nullptr) {
     if (ctx->Ns == 0) return set_err(ctx, ISS_ERR_STATE, "species not set");
     if (precision != 32 && precision != 64) return set_err(ctx, ISS_ERR_ARG, "precision must be 32 or 64");
     int rc = ensure_ktab(ctx);
@@ -421,6 +428,7 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     a.dnq = nullptr;
     a.spk = ctx->d_spk.p;
     a.lcc = lcc;
+    a.ycol = ycol;
     HIPCHK(ctx->d_oob.ensure(1));
     a.oob = ctx->d_oob.p;
     HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
@@ -689,6 +697,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_counts.release(); ctx->d_evcount.release(); ctx->d_off.release(); ctx->d_evoff.release();
     ctx->d_tree0.release(); ctx->d_tree1.release(); ctx->d_out.release(); ctx->d_setup.release(); ctx->d_stats.release();
     ctx->d_cellidx.release();
+    ctx->d_ycol.release(); ctx->d_mcnt.release(); ctx->d_msegoff.release(); ctx->d_msp.release();
     ctx->d_anti.release(); ctx->d_lcc_flag.release(); ctx->d_lcc_g.release(); ctx->d_lcc_src.release();
     ctx->d_lcc_evoff.release(); ctx->d_lcc_rec.release(); ctx->d_out2.release();
     ctx->d_aux.release(); ctx->d_mlp_par.release(); ctx->d_adam_m.release(); ctx->d_adam_v.release();
@@ -930,23 +939,189 @@ int iss_compute_yields(iss_ctx* ctx, const iss_yield_opts* opts, double* species
     return run_yields(ctx, opts->precision, per_cell, species_totals);
 }
 
+// local charge conservation needs every positive species' antiparticle
+static int lcc_check(iss_ctx* ctx) {
+    for (int s = 0; s < ctx->Ns; s++)
+        if (ctx->h_charge[s] > 0 && ctx->h_anti[s] < 0)
+            return set_err(ctx, ISS_ERR_ARG,
+                           "local charge conservation: the antiparticle of pid " + std::to_string(ctx->h_pid[s]) +
+                               " is not in the species table");
+    return ISS_OK;
+}
+
+// partners of the positive hadrons (src/FSSW.cpp:1035-1048): each positive
+// hadron of the pass gets its antiparticle, from the same cell, in the slot
+// after it.  a: the pass's arguments (a.out = d_out, a.K hadrons); mrec: the
+// multiplicity models' set-up records, or null for the Poisson pass.
+static int lcc_partners(iss_ctx* ctx, const SampleArgs& a, int E, const HSetup* mrec, int per_cu_plain,
+                        int64_t* Kio) {
+    const int64_t K = *Kio;
+    HIPCHK(ctx->d_lcc_flag.ensure(K));
+    HIPCHK(ctx->d_lcc_g.ensure(K + 1));
+    HIPCHK(ctx->d_lcc_src.ensure(K));
+    HIPCHK(ctx->d_lcc_evoff.ensure(E + 1));
+    if (mrec)
+        k_lcc_flag_rec<<<(K + 255) / 256, 256, 0, ctx->stream>>>(mrec, K, a.sp.Q, ctx->d_lcc_flag.p,
+                                                                 ctx->d_lcc_src.p);
+    else
+        k_lcc_flag<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_lcc_flag.p, ctx->d_lcc_src.p);
+    HIPCHK(hipGetLastError());
+    int rc = launch_scan(ctx, ctx->d_lcc_flag.p, K, ctx->d_lcc_g.p);
+    if (rc) return rc;
+    int64_t P = 0;
+    HIPCHK(hipMemcpyAsync(&P, ctx->d_lcc_g.p + K, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (K + P >= (1ll << 31)) return set_err(ctx, ISS_ERR_RANGE, "more than 2^31 hadrons in one pass");
+    HIPCHK(ctx->d_out2.ensure(K + P));
+    HIPCHK(ctx->d_lcc_rec.ensure(std::max<int64_t>(P, 1)));
+    k_lcc_place<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_lcc_flag.p, ctx->d_lcc_g.p, ctx->d_lcc_src.p,
+                                                          ctx->d_out.p, ctx->d_out2.p, ctx->d_lcc_rec.p, mrec);
+    HIPCHK(hipGetLastError());
+    if (P > 0) {
+        SampleArgs ap = a;
+        ap.out = ctx->d_out2.p;
+        ap.K = P;
+        ap.Kdev = nullptr;
+        const int nblk = static_cast<int>(
+            std::max<int64_t>(1, std::min<int64_t>((P + 255) / 256, int64_t(ctx->num_cu) * per_cu_plain)));
+        k_trials<<<nblk, 256, 0, ctx->stream>>>(ap, ctx->d_lcc_rec.p);
+        HIPCHK(hipGetLastError());
+    }
+    k_lcc_events<<<(E + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_evoff.p, E, ctx->d_lcc_g.p, ctx->d_lcc_evoff.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->d_evoff.p, ctx->d_lcc_evoff.p, sizeof(int64_t) * (E + 1), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    std::swap(ctx->d_out.p, ctx->d_out2.p);
+    std::swap(ctx->d_out.cap, ctx->d_out2.cap);
+    *Kio = K + P;
+    ctx->last_plain = false;  // output positions no longer match the set-up records
+    return ISS_OK;
+}
+
+// dN_dy_sampling_model 1, 10, 20 (src/FSSW.cpp:250-309): per (event,
+// species) multiplicities, then each hadron's cell from its species' cell
+// prefix (src/FSSW.cpp:920-979), then the plain momentum sampler
+static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
+    const int model = o->sampling_model;
+    if (model != 1 && model != 10 && model != 20)
+        return set_err(ctx, ISS_ERR_UNSUPPORTED, "dN_dy_sampling_model must be 1, 10, 20 or 30");
+    if (o->proposal != 0)
+        return set_err(ctx, ISS_ERR_UNSUPPORTED, "the learned proposal samples with dN_dy_sampling_model = 30");
+    const int lcc = o->local_charge_conservation ? 1 : 0;
+    if (lcc) {
+        int rc = lcc_check(ctx);
+        if (rc) return rc;
+    }
+    if ((model == 10 || model == 20) && !(o->sampling_para1 > 0.))
+        return set_err(ctx, ISS_ERR_ARG, "dN_dy_sampling_para1 must be positive for the negative binomial models");
+    if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
+    const int64_t Nc = ctx->Nc;
+    const int Ns = ctx->Ns;
+    const int E = o->n_events;
+    // yields with their species-major copy, scanned along the cells
+    HIPCHK(ctx->d_ycol.ensure(std::max<int64_t>(Nc, 1) * Ns));
+    int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc, ctx->d_ycol.p);
+    if (rc) return rc;
+    const bool binv = o->hydro_mode != 2;
+    const double mult = binv ? (o->y_RB - o->y_LB) : 1.0;
+    const uint32_t chunk = static_cast<uint32_t>(o->event0);
+    const int64_t nseg = static_cast<int64_t>(E) * Ns;
+    HIPCHK(ctx->d_mcnt.ensure(nseg));
+    HIPCHK(ctx->d_msegoff.ensure(nseg + 1));
+    HIPCHK(ctx->d_evoff.ensure(E + 1));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 6 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    if (Nc > 0) k_row_scan<<<Ns, 256, 0, ctx->stream>>>(ctx->d_ycol.p, Nc);
+    k_model_counts<<<(nseg + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_stot.p, Ns, E, mult, model, o->sampling_para1,
+                                                                o->seed, chunk, ctx->d_mcnt.p);
+    HIPCHK(hipGetLastError());
+    rc = launch_scan(ctx, ctx->d_mcnt.p, nseg, ctx->d_msegoff.p);
+    if (rc) return rc;
+    k_model_evoff<<<(E + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_msegoff.p, E, Ns, ctx->d_evoff.p);
+    int64_t K = 0;
+    HIPCHK(hipMemcpyAsync(&K, ctx->d_msegoff.p + nseg, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (K >= (1ll << 31)) return set_err(ctx, ISS_ERR_RANGE, "more than 2^31 hadrons in one pass");
+    if (K > 0 && Nc == 0) return set_err(ctx, ISS_ERR_STATE, "hadrons without cells");
+    HIPCHK(ctx->d_msp.ensure(std::max<int64_t>(K, 1)));
+    HIPCHK(ctx->d_out.ensure(std::max<int64_t>(K, 1)));
+    HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
+    k_model_fill<<<(nseg + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_msegoff.p, nseg, Ns, ctx->d_msp.p);
+    SampleArgs a = base_args(ctx);
+    a.evoff = ctx->d_evoff.p;
+    a.E = E;
+    a.K = K;
+    a.seed = o->seed;
+    a.chunk = chunk;
+    a.boost_invariant = binv ? 1 : 0;
+    a.yLB = static_cast<float>(o->y_LB);
+    a.yRB = static_cast<float>(o->y_RB);
+    a.max_trials = o->max_trials;
+    a.refill_min = ctx->refill_min[0];
+    a.out = ctx->d_out.p;
+    a.stats = ctx->d_stats.p;
+    a.anti = ctx->d_anti.p;
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    if (K > 0) {
+        k_model_setup<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_msp.p, ctx->d_ycol.p, ctx->d_setup.p);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (ctx->blocks_per_cu[0] > 0) per_cu = std::min(ctx->blocks_per_cu[0], per_cu);
+    if (K > 0) {
+        const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((K + 255) / 256,
+                                                                                  int64_t(ctx->num_cu) * per_cu)));
+        k_trials<<<nblk, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+        HIPCHK(hipGetLastError());
+    }
+    ctx->last_plain = true;  // the records' output slots index d_out
+    if (lcc && K > 0) {
+        rc = lcc_partners(ctx, a, E, ctx->d_setup.p, per_cu, &K);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+    unsigned long long st[6];
+    HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms_all = 0.f, ms_k = 0.f, ms_su = 0.f, ms_tr = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
+    HIPCHK(hipEventElapsedTime(&ms_k, ctx->ev[4], ctx->ev[5]));
+    HIPCHK(hipEventElapsedTime(&ms_su, ctx->ev[4], ctx->ev[6]));
+    HIPCHK(hipEventElapsedTime(&ms_tr, ctx->ev[6], ctx->ev[5]));
+    ctx->stats.ms_setup_kernel = ms_su;
+    ctx->stats.ms_trials_kernel = ms_tr;
+    ctx->stats.ms_sample = ms_all;
+    ctx->stats.ms_sample_kernel = ms_k;
+    ctx->K = K;
+    ctx->E = E;
+    ctx->stats.n_hadrons = K;
+    ctx->stats.n_trials = static_cast<int64_t>(st[0]);
+    ctx->stats.n_inner = static_cast<int64_t>(st[1]);
+    ctx->stats.n_failed = static_cast<int64_t>(st[2]);
+    ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
+    ctx->stats.n_fallback = 0;
+    ctx->stats.n_violations = 0;
+    if (n_hadrons) *n_hadrons = K;
+    if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
+    return ISS_OK;
+}
+
 int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     if (!ctx || !o) return ISS_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     if (o->n_events <= 0) return set_err(ctx, ISS_ERR_ARG, "n_events must be positive");
-    if (o->sampling_model != 30)
-        return set_err(ctx, ISS_ERR_UNSUPPORTED, "only dN_dy_sampling_model = 30 (Poisson) is supported");
+    if (o->sampling_model != 30) return sample_model(ctx, o, n_hadrons);
     if (o->proposal != 0 && o->proposal != 1) return set_err(ctx, ISS_ERR_ARG, "proposal must be 0 or 1");
     if (o->proposal == 1 && !ctx->mlp_ready)
         return set_err(ctx, ISS_ERR_STATE, "learned proposal: no network parameters (init, set or train)");
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
     const int lcc = o->local_charge_conservation ? 1 : 0;
     if (lcc) {
-        for (int s = 0; s < ctx->Ns; s++)
-            if (ctx->h_charge[s] > 0 && ctx->h_anti[s] < 0)
-                return set_err(ctx, ISS_ERR_ARG,
-                               "local charge conservation: the antiparticle of pid " + std::to_string(ctx->h_pid[s]) +
-                                   " is not in the species table");
+        int rc = lcc_check(ctx);
+        if (rc) return rc;
     }
     if (!ctx->yields_valid || ctx->yields_lcc != lcc) {
         int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc);
@@ -1105,43 +1280,8 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     }
     HIPCHK(hipGetLastError());
     if (lcc && K > 0) {
-        // partners of the positive hadrons (src/FSSW.cpp:1035-1048)
-        HIPCHK(ctx->d_lcc_flag.ensure(K));
-        HIPCHK(ctx->d_lcc_g.ensure(K + 1));
-        HIPCHK(ctx->d_lcc_src.ensure(K));
-        HIPCHK(ctx->d_lcc_evoff.ensure(E + 1));
-        k_lcc_flag<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_lcc_flag.p, ctx->d_lcc_src.p);
-        HIPCHK(hipGetLastError());
-        rc = launch_scan(ctx, ctx->d_lcc_flag.p, K, ctx->d_lcc_g.p);
+        rc = lcc_partners(ctx, a, E, nullptr, per_cu_plain, &K);
         if (rc) return rc;
-        int64_t P = 0;
-        HIPCHK(hipMemcpyAsync(&P, ctx->d_lcc_g.p + K, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(ctx->d_out2.ensure(K + P));
-        HIPCHK(ctx->d_lcc_rec.ensure(std::max<int64_t>(P, 1)));
-        k_lcc_place<<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_lcc_flag.p, ctx->d_lcc_g.p,
-                                                              ctx->d_lcc_src.p, ctx->d_out.p, ctx->d_out2.p,
-                                                              ctx->d_lcc_rec.p);
-        HIPCHK(hipGetLastError());
-        if (P > 0) {
-            SampleArgs ap = a;
-            ap.out = ctx->d_out2.p;
-            ap.K = P;
-            ap.Kdev = nullptr;
-            const int nblk = static_cast<int>(
-                std::max<int64_t>(1, std::min<int64_t>((P + 255) / 256, int64_t(ctx->num_cu) * per_cu_plain)));
-            k_trials<<<nblk, 256, 0, ctx->stream>>>(ap, ctx->d_lcc_rec.p);
-            HIPCHK(hipGetLastError());
-        }
-        k_lcc_events<<<(E + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_evoff.p, E, ctx->d_lcc_g.p,
-                                                                    ctx->d_lcc_evoff.p);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(ctx->d_evoff.p, ctx->d_lcc_evoff.p, sizeof(int64_t) * (E + 1),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-        std::swap(ctx->d_out.p, ctx->d_out2.p);
-        std::swap(ctx->d_out.cap, ctx->d_out2.cap);
-        K += P;
-        ctx->last_plain = false;  // output positions no longer match the set-up records
     }
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     unsigned long long st[6];

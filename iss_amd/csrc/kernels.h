@@ -88,6 +88,7 @@ struct YieldArgs {
     const double* dnq;   // [Ns][Nc] diffusion corrections (k_diffusion_dn); null: none
     const SpeciesK* spk;  // [Ns] species constants of the yield pass
     int lcc;              // local charge conservation: negative species get no yield
+    double* ycol;         // optional [Ns][Nc] FP64 yields, species-major (multiplicity models)
 };
 
 struct SampleArgs {

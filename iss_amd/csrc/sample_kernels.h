@@ -148,11 +148,14 @@ __device__ __forceinline__ void emit_lab(const SampleArgs& a, uint32_t chunk, in
     const bool partner = (jj & kPartnerBit) != 0u;
     if (a.boost_invariant && partner) {
         // a local-charge-conservation partner keeps its parent's eta_s
-        // (src/FSSW.cpp:1044-1046); the parent is the previous hadron
+        // (src/FSSW.cpp:1044-1046); the parent is the previous hadron, from
+        // the same cell: eta_s = asinh(z / tau) keeps full precision at any
+        // eta_s, where 0.5 log((t + z) / (t - z)) cancels (t - z rounds to 0
+        // in FP32 beyond |eta_s| ~ 8)
         const Hadron hp = static_cast<const Hadron*>(a.out)[outp - 1];
         t_par = hp.t;
         z_par = hp.z;
-        eta_s = 0.5f * logf((hp.t + hp.z) / (hp.t - hp.z));
+        eta_s = asinhf(hp.z / tau);
         const float rap = eta_s + (asinhf(lz / mT) - eta);
         E = mT * coshf(rap);
         pzl = mT * sinhf(rap);
@@ -579,9 +582,23 @@ __global__ void k_lcc_flag(SampleArgs a, uint32_t* __restrict__ pflag, int64_t* 
     src[o] = (i << 10) | s;
 }
 
+// the same flags from the set-up records of the multiplicity models, whose
+// output slot is the record index
+__global__ void k_lcc_flag_rec(const HSetup* __restrict__ rec, int64_t K, const int* __restrict__ Q,
+                               uint32_t* __restrict__ pflag, int64_t* __restrict__ src) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const int s = rec[i].species;
+    pflag[i] = (Q[s] > 0) ? 1u : 0u;
+    src[i] = (i << 10) | s;
+}
+
+// mrec: the set-up records the slots were sampled from (multiplicity
+// models), or null for the Poisson pass (cell from the offsets)
 __global__ void k_lcc_place(SampleArgs a, const uint32_t* __restrict__ pflag, const int64_t* __restrict__ g,
                             const int64_t* __restrict__ src, const Hadron* __restrict__ old_out,
-                            Hadron* __restrict__ new_out, HSetup* __restrict__ prec) {
+                            Hadron* __restrict__ new_out, HSetup* __restrict__ prec,
+                            const HSetup* __restrict__ mrec) {
     const int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (o >= a.K) return;
     const int64_t n = o + g[o];
@@ -589,8 +606,15 @@ __global__ void k_lcc_place(SampleArgs a, const uint32_t* __restrict__ pflag, co
     if (!pflag[o]) return;
     const int64_t i = src[o] >> 10;
     const int s = static_cast<int>(src[o] & 1023);
-    const int64_t c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
-    const uint32_t jj = static_cast<uint32_t>(i - a.off[c]) | kPartnerBit;
+    int64_t c;
+    uint32_t jj;
+    if (mrec) {
+        c = mrec[i].cell;
+        jj = mrec[i].j | kPartnerBit;
+    } else {
+        c = upper_bound(a.off + 1, int64_t(0), a.Nc, i);
+        jj = static_cast<uint32_t>(i - a.off[c]) | kPartnerBit;
+    }
     HSetup r = setup_record(a, c, jj, a.anti[s]);
     r.out = static_cast<uint64_t>(n + 1);
     prec[g[o]] = r;
@@ -601,6 +625,96 @@ __global__ void k_lcc_events(const int64_t* __restrict__ evoff, int E, const int
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e > E) return;
     evoff_new[e] = evoff[e] + g[evoff[e]];
+}
+
+// ====================================== multiplicity models 1, 10 and 20
+// FSSW's per-(species, event) multiplicity (src/FSSW.cpp:250-309) for the
+// models other than Poisson, whose per-cell splitting the main pass uses:
+// the reference's order species -> events -> hadrons -> cell pick
+// (src/FSSW.cpp:920-979) becomes
+//   k_model_counts   per (event, species): N from the model
+//   scan             slots in (event, species) order: events are contiguous
+//                    and, within an event, species follow the sampling order
+//   k_model_fill     each slot's species
+//   k_model_setup    each slot's cell from its species' FP64 cell prefix
+//                    (the species-major yields, scanned along the cells) and
+//                    its set-up record; the trial kernel follows
+constexpr uint32_t TAG_MODEL = 10, TAG_MODEL_CELL = 11;
+
+__global__ void k_model_counts(const double* __restrict__ stot, int Ns, int E, double mult, int model,
+                               double para1, uint64_t seed, uint32_t chunk, uint32_t* __restrict__ cnt) {
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (it >= static_cast<int64_t>(E) * Ns) return;
+    const int e = static_cast<int>(it / Ns), s = static_cast<int>(it % Ns);
+    Stream st(seed, static_cast<uint32_t>(s), static_cast<uint32_t>(e), tag_word(TAG_MODEL, chunk));
+    const double dN = stot[s] * mult;
+    const double fl = floor(dN), frac = dN - fl;
+    int64_t n = 0;
+    if (model == 1) {
+        n = static_cast<int64_t>(fl) + (st.uniform() < frac ? 1 : 0);
+    } else if (model == 10) {
+        const double k = para1 * frac, p = 1. / (1. + para1);
+        n = static_cast<int64_t>(fl) + (k < 1e-15 ? 0 : negative_binomial(p, k, st));
+    } else {  // 20
+        const double k = para1 * dN, p = 1. / (1. + para1);
+        n = k < 1e-15 ? static_cast<int64_t>(fl) : negative_binomial(p, k, st);
+    }
+    cnt[it] = static_cast<uint32_t>(min<int64_t>(n, 0xFFFFFFFFll));
+}
+
+// per (event, species) segment: its slots' species
+__global__ void k_model_fill(const int64_t* __restrict__ segoff, int64_t nseg, int Ns, uint16_t* __restrict__ sp) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= nseg) return;
+    const uint16_t s = static_cast<uint16_t>(g % Ns);
+    for (int64_t i = segoff[g]; i < segoff[g + 1]; i++) sp[i] = s;
+}
+
+// event offsets: the first slot of each event's first species
+__global__ void k_model_evoff(const int64_t* __restrict__ segoff, int E, int Ns, int64_t* __restrict__ evoff) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > E) return;
+    evoff[e] = segoff[static_cast<int64_t>(e) * Ns];
+}
+
+// FP64 inclusive prefix of each species' row of cell yields ([Ns][Nc], in
+// place): one workgroup per species, tiles in order (deterministic)
+__global__ __launch_bounds__(256) void k_row_scan(double* __restrict__ rows, int64_t Nc) {
+    __shared__ double wsum[256 / kWave];
+    double* row = rows + static_cast<int64_t>(blockIdx.x) * Nc;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    double carry = 0.;
+    for (int64_t t0 = 0; t0 < Nc; t0 += 256) {
+        const int64_t c = t0 + threadIdx.x;
+        const double v = c < Nc ? row[c] : 0.;
+        const double incl = wave_inclusive_scan(v);
+        if (lane == kWave - 1) wsum[w] = incl;
+        __syncthreads();
+        double pre = carry, tot = carry;
+        for (int i = 0; i < 256 / kWave; i++) {
+            if (i < w) pre += wsum[i];
+            tot += wsum[i];
+        }
+        if (c < Nc) row[c] = pre + incl;
+        carry = tot;
+        __syncthreads();
+    }
+}
+
+// per slot: the cell (the slot's species' cell prefix) and the set-up record;
+// the slot index is the hadron's identity for its random streams
+__global__ void k_model_setup(SampleArgs a, const uint16_t* __restrict__ sp, const double* __restrict__ ccum,
+                              HSetup* __restrict__ rec) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.K) return;
+    const int s = sp[i];
+    const double* row = ccum + static_cast<int64_t>(s) * a.Nc;
+    Stream st(a.seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), tag_word(TAG_MODEL_CELL, a.chunk));
+    int64_t c = upper_bound(row, int64_t(0), a.Nc, st.uniform() * row[a.Nc - 1]);
+    if (c >= a.Nc) c = a.Nc - 1;
+    HSetup r = setup_record(a, c, static_cast<uint32_t>(i), s);
+    r.out = static_cast<uint64_t>(i);
+    rec[i] = r;
 }
 
 }  // namespace issdev

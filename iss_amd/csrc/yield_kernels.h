@@ -362,6 +362,7 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const Spec
                     }
                     if (RAW) static_cast<R*>(a.raw)[c * Ns + s] = y;
                 }
+                if (a.ycol && valid) a.ycol[static_cast<int64_t>(s) * Nc + c] = static_cast<double>(y);
                 run += static_cast<double>(y);
                 stage[j] = run;
                 ystage[j] = y;

@@ -298,3 +298,46 @@ def test_dropin_local_charge_conservation(tmp_path):
                 k += 1
     print("positive parents", npos)
     assert npos > 100
+
+
+def _event_sizes(path):
+    """hadrons per event of a binary particle_samples file, without parsing them"""
+    raw = path.read_bytes()
+    off, sizes = 0, []
+    while off < len(raw):
+        (n,) = struct.unpack_from("<i", raw, off)
+        sizes.append(n)
+        off += 4 + 40 * n
+    return np.array(sizes, dtype=np.float64)
+
+
+def test_dropin_sampling_models(tmp_path):
+    """dN_dy_sampling_model 1 and 20 (src/FSSW.cpp:250-309) through the
+    parameter file: with model 1 every species' count varies by at most one
+    between events; with model 20 and dN_dy_sampling_para1 = 1 the event
+    multiplicity has the Poisson pass's mean and twice its variance (a sum of
+    negative binomials with n = dN, p = 1/2)."""
+    # the Ideal2 cell with 1/100 of its d sigma: about 190 hadrons per event
+    cols = open(os.path.join(FIX, RUNS["Ideal2"][2])).read().split()
+    cols[4] = repr(float(cols[4]) / 100.0)
+    line = "  ".join(cols) + "\n"
+
+    def run(sub, n_ev, extra):
+        d = tmp_path / sub
+        d.mkdir()
+        _run(d, "Ideal2", ["use_binary_format=1", f"number_of_repeated_sampling={n_ev}", "perform_checks=0"] + extra,
+             surface_text=line)
+        return d / "particle_samples.bin"
+
+    ev1 = _read_bin(run("m1", 200, ["dN_dy_sampling_model=1"]))
+    assert len(ev1) == 200
+    pids = sorted({h[0] for e in ev1 for h in e})
+    per = np.array([[sum(1 for h in e if h[0] == p) for p in pids] for e in ev1])
+    assert np.all(per.max(axis=0) - per.min(axis=0) <= 1)
+    n30 = _event_sizes(run("m30", 2000, ["dN_dy_sampling_model=30"]))
+    n20 = _event_sizes(run("m20", 2000, ["dN_dy_sampling_model=20", "dN_dy_sampling_para1=1"]))
+    assert n30.size == n20.size == 2000
+    mean = n30.mean()
+    print("model 30 mean", mean, "var", n30.var(), "model 20 mean", n20.mean(), "var", n20.var())
+    assert abs(n20.mean() - mean) < 5 * np.sqrt(2 * mean / n20.size) + 5 * np.sqrt(mean / n30.size)
+    assert 1.7 < n20.var() / mean < 2.3

@@ -75,6 +75,8 @@ def test_partners_follow_parents_and_events_are_neutral(species, hydro_mode):
     npos = _check_structure(h, counts, species)
     print("hadrons", n, "positive parents", npos)
     assert npos > 1000
+    for f in ("E", "px", "py", "pz", "t", "x", "y", "z"):
+        assert np.all(np.isfinite(h[f])), f
     E = h["E"].astype(np.float64)
     m2 = E ** 2 - h["px"].astype(np.float64) ** 2 - h["py"].astype(np.float64) ** 2 - h["pz"].astype(np.float64) ** 2
     bad = np.abs(m2 - h["mass"].astype(np.float64) ** 2) >= 1e-6 * E ** 2 + 1e-4
