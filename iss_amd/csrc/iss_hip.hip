@@ -1148,16 +1148,23 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipGetLastError());
     int rc = launch_scan(ctx, ctx->d_counts.p, Nc, ctx->d_off.p);
     if (rc) return rc;
-    // multinomial over events, root = K
-    HIPCHK(hipMemcpyAsync(ctx->d_tree0.p, ctx->d_off.p + Nc, sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
-    int64_t* cur = ctx->d_tree0.p;
-    int64_t* nxt = ctx->d_tree1.p;
-    for (int d = 0; d < D; d++) {
-        const int64_t nodes = 1ll << d;
-        k_multinomial_level<<<(nodes + 255) / 256, 256, 0, ctx->stream>>>(cur, nxt, d, E, o->seed, chunk);
-        std::swap(cur, nxt);
+    // multinomial over events, root = K: one workgroup for up to 2048
+    // events, else one launch per tree level
+    if ((1ll << D) <= kMultiFusedLeaves) {
+        k_multinomial_fused<<<1, 1024, 0, ctx->stream>>>(ctx->d_off.p + Nc, D, E, o->seed, chunk,
+                                                         ctx->d_evcount.p);
+    } else {
+        HIPCHK(hipMemcpyAsync(ctx->d_tree0.p, ctx->d_off.p + Nc, sizeof(int64_t), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        int64_t* cur = ctx->d_tree0.p;
+        int64_t* nxt = ctx->d_tree1.p;
+        for (int d = 0; d < D; d++) {
+            const int64_t nodes = 1ll << d;
+            k_multinomial_level<<<(nodes + 255) / 256, 256, 0, ctx->stream>>>(cur, nxt, d, E, o->seed, chunk);
+            std::swap(cur, nxt);
+        }
+        k_multinomial_leaves<<<((1ll << D) + 255) / 256, 256, 0, ctx->stream>>>(cur, D, E, ctx->d_evcount.p);
     }
-    k_multinomial_leaves<<<((1ll << D) + 255) / 256, 256, 0, ctx->stream>>>(cur, D, E, ctx->d_evcount.p);
     HIPCHK(hipGetLastError());
     rc = launch_scan(ctx, ctx->d_evcount.p, E, ctx->d_evoff.p);
     if (rc) return rc;

@@ -113,6 +113,40 @@ __global__ void k_multinomial_level(const int64_t* __restrict__ parent, int64_t*
     child[2 * k + 1] = n - nl;
 }
 
+// The whole tree in one workgroup for 2^D <= kMultiFusedLeaves (heap layout
+// in LDS: level d at [2^d, 2^(d+1))), with the same streams and draws as
+// the level-by-level kernels, so both give the same counts.  root: K.
+constexpr int kMultiFusedLeaves = 2048;
+__global__ __launch_bounds__(1024) void k_multinomial_fused(const int64_t* __restrict__ root, int D, int E,
+                                                            uint64_t seed, uint32_t chunk,
+                                                            uint32_t* __restrict__ evcount) {
+    __shared__ int64_t tree[2 * kMultiFusedLeaves];
+    if (threadIdx.x == 0) tree[1] = *root;
+    __syncthreads();
+    for (int d = 0; d < D; d++) {
+        const int64_t nodes = 1ll << d;
+        for (int64_t k = threadIdx.x; k < nodes; k += blockDim.x) {
+            const int64_t n = tree[nodes + k];
+            const int64_t lo = (k * E) >> d, hi = ((k + 1) * E) >> d;
+            const int64_t mid = ((2 * k + 1) * E) >> (d + 1);
+            int64_t nl = 0;
+            if (hi > lo && n > 0) {
+                const double p = static_cast<double>(mid - lo) / static_cast<double>(hi - lo);
+                Stream st(seed, static_cast<uint32_t>(k), static_cast<uint32_t>(d), tag_word(TAG_MULTI, chunk));
+                nl = binomial(n, p, st);
+            }
+            tree[2 * nodes + 2 * k] = nl;
+            tree[2 * nodes + 2 * k + 1] = n - nl;
+        }
+        __syncthreads();
+    }
+    const int64_t leaves = 1ll << D;
+    for (int64_t k = threadIdx.x; k < leaves; k += blockDim.x) {
+        const int64_t lo = (k * E) >> D, hi = ((k + 1) * E) >> D;
+        if (hi - lo == 1) evcount[lo] = static_cast<uint32_t>(tree[leaves + k]);
+    }
+}
+
 // leaves (level D, 2^D >= E): node k covers at most one event
 __global__ void k_multinomial_leaves(const int64_t* __restrict__ leaf, int D, int E,
                                      uint32_t* __restrict__ evcount) {

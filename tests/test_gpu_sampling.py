@@ -199,3 +199,35 @@ def test_same_seed_same_output():
     assert np.array_equal(a, b) and np.array_equal(ca, ctx.event_counts())
     ctx.sample(50, seed=6)
     assert not np.array_equal(a, ctx.hadrons()[: a.size]) or a.size != ctx.stats().n_hadrons
+
+
+@pytest.mark.parametrize("n_events", [1, 3, 2048, 2049, 5000])
+def test_event_counts_tree_sizes(n_events):
+    """The per-event split of the hadron total (a binomial tree over the
+    events: one workgroup up to 2048 events, one launch per level beyond):
+    counts add up to the total, and are Poisson with the mean of the total
+    per event (chi^2 of the count histogram, p > 1e-3)."""
+    sp = tables.species_arrays(1)
+    cells, _ = synth.viscous_lrf(300, sp, seed=8, target_hadrons=30.0)
+    ctx = iss_amd.Context(0)
+    ctx.set_species(sp)
+    ctx.set_deltaf(shear=1, bulk=1, bulk_kind=11, k11=tables.read_kind11(1))
+    ctx.set_surface_lrf(cells)
+    lam = ctx.compute_yields(precision=32).sum()
+    n = ctx.sample(n_events, seed=21)
+    c = ctx.event_counts()
+    assert c.size == n_events and c.sum() == n
+    if n_events >= 2048:
+        kmax = int(c.max()) + 1
+        obs = np.bincount(c, minlength=kmax + 1).astype(np.float64)
+        pmf = sst.poisson.pmf(np.arange(kmax + 1), lam)
+        pmf[-1] += sst.poisson.sf(kmax, lam)
+        exp = n_events * pmf
+        ok = exp > 10
+        chi = float(np.sum((obs[ok] - exp[ok]) ** 2 / exp[ok]))
+        pv = sst.chi2.sf(chi, int(ok.sum()) - 1)
+        print(n_events, "mean", c.mean(), "lambda", lam, "chi2 p", pv)
+        assert pv > 1e-3
+    ctx.sample(n_events, seed=21)
+    assert np.array_equal(ctx.event_counts(), c)
+    ctx.close()
