@@ -89,7 +89,7 @@ __device__ inline int64_t poisson(double lam, Stream& s) {
         }
         return k;
     }
-    const double slam = sqrt(lam), loglam = log(lam);
+    const double slam = sqrt(lam);
     const double b = 0.931 + 2.53 * slam;
     const double a = -0.059 + 0.02483 * b;
     const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
@@ -101,8 +101,9 @@ __device__ inline int64_t poisson(double lam, Stream& s) {
         const double kd = floor((2. * a / us + b) * U + lam + 0.43);
         if (us >= 0.07 && V <= vr) return static_cast<int64_t>(kd);
         if (kd < 0. || (us < 0.013 && V > us)) continue;
+        // past the squeeze (few draws): the exact test
         if (log(V) + log(inv_alpha) - log(a / (us * us) + b) <=
-            -lam + kd * loglam - lgamma(kd + 1.))
+            -lam + kd * log(lam) - lgamma(kd + 1.))
             return static_cast<int64_t>(kd);
     }
     return static_cast<int64_t>(floor(lam + 0.5));  // unreachable in practice
@@ -178,7 +179,10 @@ __device__ inline int64_t binomial(int64_t n, double p, Stream& s) {
         const double vr = 0.92 - 4.2 / b;
         const double lpq = log(pp / q);
         const double m = floor((nd + 1.) * pp);
-        const double h = lgamma(m + 1.) + lgamma(nd - m + 1.);
+        // log(m! (n - m)!), needed only past the squeeze (most draws stop
+        // there), so computed on first use: two FP64 lgamma less per draw
+        double h = 0.;
+        bool have_h = false;
         k = static_cast<int64_t>(floor(nd * pp + 0.5));
         for (int it = 0; it < 10000; it++) {
             const double u = s.uniform() - 0.5;
@@ -187,6 +191,10 @@ __device__ inline int64_t binomial(int64_t n, double p, Stream& s) {
             const double kd = floor((2. * a / us + b) * u + c);
             if (kd < 0. || kd > nd) continue;
             if (us >= 0.07 && v <= vr) { k = static_cast<int64_t>(kd); break; }
+            if (!have_h) {
+                h = lgamma(m + 1.) + lgamma(nd - m + 1.);
+                have_h = true;
+            }
             v = log(v * alpha / (a / (us * us) + b));
             if (v <= h - lgamma(kd + 1.) - lgamma(nd - kd + 1.) + (kd - m) * lpq) {
                 k = static_cast<int64_t>(kd);

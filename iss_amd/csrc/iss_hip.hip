@@ -81,6 +81,34 @@ struct DBuf {
     }
 };
 
+// Pinned host memory for the read-backs of counts, flags and totals: a copy
+// from device memory into pageable memory goes through the runtime's staging
+// buffer and costs tens of microseconds more than one into pinned memory.
+// Slots: PIN_OOB int, PIN_K int64, PIN_STATS 8 x u64, PIN_AUX int64, then
+// PIN_TOT [Ns] doubles.
+struct PinBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    PinBuf() = default;
+    PinBuf(const PinBuf&) = delete;
+    PinBuf& operator=(const PinBuf&) = delete;
+    ~PinBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    template <typename T>
+    T* at(size_t off) { return reinterpret_cast<T*>(static_cast<char*>(p) + off); }
+};
+constexpr size_t PIN_OOB = 0, PIN_K = 8, PIN_STATS = 64, PIN_AUX = 128, PIN_TOT = 256;
+
 // one generation of the resonance feed-down: its particles and per-particle
 // bookkeeping (kept by the context and reused across calls)
 struct DecayGen {
@@ -100,6 +128,7 @@ struct DecayGen {
 }  // namespace
 
 struct iss_ctx {
+    PinBuf h_pin;  // pinned read-back slots (PIN_*)
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
@@ -440,32 +469,35 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
         HIPCHK(hipGetLastError());
         a.dnq = ctx->d_dnq.p;
     }
-    if (Nc > 0) {
-        int rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, false) : launch_yield<float>(ctx, a, raw, false);
-        if (rc) return set_err(ctx, rc, "yield kernel launch failed");
-        HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-        int oob = 0;
-        HIPCHK(hipMemcpyAsync(&oob, ctx->d_oob.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (oob) {
-            // a Bessel argument left the table: rerun with the exact fallback
-            rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, true) : launch_yield<float>(ctx, a, raw, true);
+    // the pass, the species reduction and the read-backs, then one wait; a
+    // Bessel argument outside the table (flag oob) reruns the pass with the
+    // exact fallback
+    HIPCHK(ctx->h_pin.ensure(PIN_TOT + sizeof(double) * Ns));
+    int* h_oob = ctx->h_pin.at<int>(PIN_OOB);
+    double* h_tot = ctx->h_pin.at<double>(PIN_TOT);
+    for (int exact = 0; exact < 2; exact++) {
+        if (Nc > 0) {
+            int rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, exact != 0)
+                                       : launch_yield<float>(ctx, a, raw, exact != 0);
             if (rc) return set_err(ctx, rc, "yield kernel launch failed");
-            HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+            nblk = ctx->yield_grid;
+        } else {
+            HIPCHK(hipMemsetAsync(ctx->d_partial.p, 0, sizeof(double) * Ns, ctx->stream));
         }
-        nblk = ctx->yield_grid;
-    } else {
-        HIPCHK(hipMemsetAsync(ctx->d_partial.p, 0, sizeof(double) * Ns, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+        k_species_reduce<<<Ns, 256, 0, ctx->stream>>>(ctx->d_partial.p, nblk, Ns, ctx->d_stot.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+        const bool check = Nc > 0 && !exact;
+        if (check) HIPCHK(hipMemcpyAsync(h_oob, ctx->d_oob.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        if (totals_host)
+            HIPCHK(hipMemcpyAsync(h_tot, ctx->d_stot.p, sizeof(double) * Ns, hipMemcpyDeviceToHost, ctx->stream));
+        if (raw && Nc > 0)
+            HIPCHK(hipMemcpyAsync(per_cell_host, ctx->d_raw.p, esz * Nc * Ns, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (totals_host) std::memcpy(totals_host, h_tot, sizeof(double) * Ns);
+        if (!(check && *h_oob)) break;
     }
-    k_species_reduce<<<Ns, 256, 0, ctx->stream>>>(ctx->d_partial.p, nblk, Ns, ctx->d_stot.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
-    if (totals_host)
-        HIPCHK(hipMemcpyAsync(totals_host, ctx->d_stot.p, sizeof(double) * Ns, hipMemcpyDeviceToHost, ctx->stream));
-    if (raw && Nc > 0)
-        HIPCHK(hipMemcpyAsync(per_cell_host, ctx->d_raw.p, esz * Nc * Ns, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms1 = 0.f, ms2 = 0.f;
     HIPCHK(hipEventElapsedTime(&ms1, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&ms2, ctx->ev[0], ctx->ev[2]));
@@ -1148,11 +1180,13 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipGetLastError());
     int rc = launch_scan(ctx, ctx->d_counts.p, Nc, ctx->d_off.p);
     if (rc) return rc;
-    // multinomial over events, root = K: one workgroup for up to 2048
-    // events, else one launch per tree level
-    if ((1ll << D) <= kMultiFusedLeaves) {
-        k_multinomial_fused<<<1, 1024, 0, ctx->stream>>>(ctx->d_off.p + Nc, D, E, o->seed, chunk,
-                                                         ctx->d_evcount.p);
+    // multinomial over events, root = K: per-slot uniform events and
+    // histograms up to kEventHistMax events, else the binomial tree, one
+    // launch per level
+    if (E <= kEventHistMax) {
+        HIPCHK(hipMemsetAsync(ctx->d_evcount.p, 0, sizeof(uint32_t) * E, ctx->stream));
+        k_event_hist<<<kEventHistBlocks, 256, 0, ctx->stream>>>(ctx->d_off.p + Nc, E, o->seed, chunk,
+                                                                ctx->d_evcount.p);
     } else {
         HIPCHK(hipMemcpyAsync(ctx->d_tree0.p, ctx->d_off.p + Nc, sizeof(int64_t), hipMemcpyDeviceToDevice,
                               ctx->stream));
@@ -1168,9 +1202,11 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipGetLastError());
     rc = launch_scan(ctx, ctx->d_evcount.p, E, ctx->d_evoff.p);
     if (rc) return rc;
-    int64_t K = 0;
-    HIPCHK(hipMemcpyAsync(&K, ctx->d_off.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx->h_pin.ensure(PIN_TOT + sizeof(double) * ctx->Ns));
+    HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_K), ctx->d_off.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    int64_t K = *ctx->h_pin.at<int64_t>(PIN_K);
     HIPCHK(ctx->d_out.ensure(std::max<int64_t>(K, 1)));
     SampleArgs a{};
     a.cells = ctx->d_cells.p;
@@ -1203,7 +1239,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.anti = ctx->d_anti.p;
     HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(K, 1)));
     if (K > 0) {
-        k_cell_index<<<(Nc + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, Nc, 0, ctx->d_cellidx.p);
+        k_cell_index<<<(Nc + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, Nc, 0, ctx->d_cellidx.p);  // 4 waves of 64 cells
         HIPCHK(hipGetLastError());
         a.cellidx = ctx->d_cellidx.p;
     }
@@ -1291,12 +1327,16 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
-    unsigned long long st[6];
-    HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long* h_st = ctx->h_pin.at<unsigned long long>(PIN_STATS);
+    HIPCHK(hipMemcpyAsync(h_st, ctx->d_stats.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     int64_t npairs = 0;
     if (o->proposal == 1 && K > 0)
-        HIPCHK(hipMemcpyAsync(&npairs, ctx->d_pairoff.p + Nc, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_AUX), ctx->d_pairoff.p + Nc, sizeof(int64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    unsigned long long st[6];
+    std::memcpy(st, h_st, sizeof(st));
+    if (o->proposal == 1 && K > 0) npairs = *ctx->h_pin.at<int64_t>(PIN_AUX);
     float ms_all = 0.f, ms_k = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
     HIPCHK(hipEventElapsedTime(&ms_k, ctx->ev[4], ctx->ev[5]));
@@ -2005,6 +2045,7 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     if (ctx->E <= 0 && ctx->K > 0)
         return set_err(ctx, ISS_ERR_UNSUPPORTED, "decays apply to the hadron list of iss_sample (not to batches)");
     (void)hipSetDevice(ctx->device);
+    HIPCHK(ctx->h_pin.ensure(PIN_TOT + sizeof(double) * std::max(ctx->Ns, 1)));
     const int E = ctx->E;
     // generation buffers live in the context and only grow
     auto& gens = ctx->dgen;
@@ -2030,9 +2071,10 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
         HIPCHK(hipGetLastError());
         int rc = launch_scan(ctx, G.nchild.p, n, G.off.p);
         if (rc) return rc;
-        int64_t nn = 0;
-        HIPCHK(hipMemcpyAsync(&nn, G.off.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_K), G.off.p + n, sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int64_t nn = *ctx->h_pin.at<int64_t>(PIN_K);
         if (nn == 0) break;
         if (g + 1 >= kDecayMaxGen) return set_err(ctx, ISS_ERR_RANGE, "decay chains deeper than 12 generations");
         if (static_cast<int>(gens.size()) < g + 2) gens.emplace_back(new DecayGen);
@@ -2058,9 +2100,10 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     HIPCHK(hipGetLastError());
     int rc = launch_scan(ctx, gens[0]->size.p, gens[0]->n, gens[0]->pos.p);
     if (rc) return rc;
-    int64_t F = 0;
-    HIPCHK(hipMemcpyAsync(&F, gens[0]->pos.p + gens[0]->n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_K), gens[0]->pos.p + gens[0]->n, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int64_t F = *ctx->h_pin.at<int64_t>(PIN_K);
     for (int g = 0; g < last; g++) {
         DecayGen& G = *gens[g];
         if (G.n == 0) continue;
@@ -2083,10 +2126,11 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    unsigned long long nfb[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(nfb, ctx->d_stats.p + 6, sizeof(nfb), hipMemcpyDeviceToHost, ctx->stream));
-    const unsigned long long nbelow = nfb[0], nf = nfb[1];
+    unsigned long long* h_nfb = ctx->h_pin.at<unsigned long long>(PIN_STATS);
+    HIPCHK(hipMemcpyAsync(h_nfb, ctx->d_stats.p + 6, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    const unsigned long long nbelow = h_nfb[0], nf = h_nfb[1];
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     std::swap(ctx->d_out.p, out.p);

@@ -113,38 +113,40 @@ __global__ void k_multinomial_level(const int64_t* __restrict__ parent, int64_t*
     child[2 * k + 1] = n - nl;
 }
 
-// The whole tree in one workgroup for 2^D <= kMultiFusedLeaves (heap layout
-// in LDS: level d at [2^d, 2^(d+1))), with the same streams and draws as
-// the level-by-level kernels, so both give the same counts.  root: K.
-constexpr int kMultiFusedLeaves = 2048;
-__global__ __launch_bounds__(1024) void k_multinomial_fused(const int64_t* __restrict__ root, int D, int E,
-                                                            uint64_t seed, uint32_t chunk,
-                                                            uint32_t* __restrict__ evcount) {
-    __shared__ int64_t tree[2 * kMultiFusedLeaves];
-    if (threadIdx.x == 0) tree[1] = *root;
+// Up to kEventHistMax events the sizes come without the tree: each of the K
+// slots draws its event uniformly (the high word of a 64-bit uniform times
+// E, a bias below E / 2^64), and the counts are histograms.  The K draws are
+// independent and uniform, so the counts are Multinomial(K, 1/E) as well.
+// One block per kEventHistBlocks share of the slots counts into an LDS
+// histogram and adds it to evcount (zeroed beforehand) with integer atomics,
+// whose sums do not depend on their order.  root: K on the device.
+constexpr int kEventHistMax = 8192, kEventHistBlocks = 256;
+constexpr uint32_t TAG_EVHIST = 12;
+__global__ __launch_bounds__(256) void k_event_hist(const int64_t* __restrict__ root, int E, uint64_t seed,
+                                                    uint32_t chunk, uint32_t* __restrict__ evcount) {
+    __shared__ uint32_t hist[kEventHistMax];
+    for (int e = threadIdx.x; e < E; e += blockDim.x) hist[e] = 0u;
     __syncthreads();
-    for (int d = 0; d < D; d++) {
-        const int64_t nodes = 1ll << d;
-        for (int64_t k = threadIdx.x; k < nodes; k += blockDim.x) {
-            const int64_t n = tree[nodes + k];
-            const int64_t lo = (k * E) >> d, hi = ((k + 1) * E) >> d;
-            const int64_t mid = ((2 * k + 1) * E) >> (d + 1);
-            int64_t nl = 0;
-            if (hi > lo && n > 0) {
-                const double p = static_cast<double>(mid - lo) / static_cast<double>(hi - lo);
-                Stream st(seed, static_cast<uint32_t>(k), static_cast<uint32_t>(d), tag_word(TAG_MULTI, chunk));
-                nl = binomial(n, p, st);
-            }
-            tree[2 * nodes + 2 * k] = nl;
-            tree[2 * nodes + 2 * k + 1] = n - nl;
+    const int64_t K = *root;
+    const int64_t npair = (K + 1) / 2;  // one Philox block gives two draws
+    const int64_t per = (npair + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = static_cast<int64_t>(blockIdx.x) * per;
+    const int64_t p1 = min(npair, p0 + per);
+    const uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+    const uint32_t tw = tag_word(TAG_EVHIST, chunk);
+    const uint64_t Eu = static_cast<uint64_t>(E);
+    for (int64_t q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+        const u32x4 w = philox4x32(u32x4{static_cast<uint32_t>(q), static_cast<uint32_t>(q >> 32), tw, 0u}, k0, k1);
+        const uint64_t u0 = (static_cast<uint64_t>(w.x) << 32) | w.y;
+        atomicAdd(&hist[__umul64hi(u0, Eu)], 1u);
+        if (2 * q + 1 < K) {
+            const uint64_t u1 = (static_cast<uint64_t>(w.z) << 32) | w.w;
+            atomicAdd(&hist[__umul64hi(u1, Eu)], 1u);
         }
-        __syncthreads();
     }
-    const int64_t leaves = 1ll << D;
-    for (int64_t k = threadIdx.x; k < leaves; k += blockDim.x) {
-        const int64_t lo = (k * E) >> D, hi = ((k + 1) * E) >> D;
-        if (hi - lo == 1) evcount[lo] = static_cast<uint32_t>(tree[leaves + k]);
-    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < E; e += blockDim.x)
+        if (hist[e]) atomicAdd(&evcount[e], hist[e]);
 }
 
 // leaves (level D, 2^D >= E): node k covers at most one event
@@ -289,15 +291,32 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
     q.fsg = static_cast<float>(a.sp.sign[s]);
 }
 
-// The cell of every hadron slot of a pass: a thread per cell writes its
-// slots' cell index (the set-up then reads it instead of searching the cell
-// offsets per slot).  Slots at or beyond `cap` (> 0) are not written.
-__global__ void k_cell_index(const int64_t* __restrict__ off, int64_t Nc, int64_t cap, uint32_t* __restrict__ cellidx) {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= Nc) return;
-    int64_t o1 = off[c + 1];
-    if (cap > 0 && o1 > cap) o1 = cap;
-    for (int64_t i = off[c]; i < o1; i++) cellidx[i] = static_cast<uint32_t>(c);
+// The cell of every hadron slot of a pass (the set-up then reads it instead
+// of searching the cell offsets per slot).  One wave per 64 consecutive
+// cells: the lanes load the cells' offsets, then the wave writes each cell's
+// run of slots in turn, lanes side by side, so the stores are contiguous.
+// Slots at or beyond `cap` (> 0) are not written.  Launch with
+// ceil(Nc / 64) waves.
+__global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ off, int64_t Nc, int64_t cap,
+                                                    uint32_t* __restrict__ cellidx) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t c0 = (static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave;
+    if (c0 >= Nc) return;  // whole waves
+    const int64_t c = c0 + lane;
+    int64_t o0 = 0, o1 = 0;
+    if (c < Nc) {
+        o0 = off[c];
+        o1 = off[c + 1];
+        if (cap > 0) {
+            o0 = min(o0, cap);
+            o1 = min(o1, cap);
+        }
+    }
+    const int nc = static_cast<int>(min<int64_t>(kWave, Nc - c0));
+    for (int j = 0; j < nc; j++) {
+        const int64_t b = __shfl(o0, j), e = __shfl(o1, j);
+        for (int64_t i = b + lane; i < e; i += kWave) cellidx[i] = static_cast<uint32_t>(c0 + j);
+    }
 }
 
 // the species-dependent part of a set-up record: the |p| table, its first

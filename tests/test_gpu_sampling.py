@@ -201,12 +201,12 @@ def test_same_seed_same_output():
     assert not np.array_equal(a, ctx.hadrons()[: a.size]) or a.size != ctx.stats().n_hadrons
 
 
-@pytest.mark.parametrize("n_events", [1, 3, 2048, 2049, 5000])
-def test_event_counts_tree_sizes(n_events):
-    """The per-event split of the hadron total (a binomial tree over the
-    events: one workgroup up to 2048 events, one launch per level beyond):
-    counts add up to the total, and are Poisson with the mean of the total
-    per event (chi^2 of the count histogram, p > 1e-3)."""
+@pytest.mark.parametrize("n_events", [1, 3, 2048, 8192, 8193, 20000])
+def test_event_counts_split_sizes(n_events):
+    """The per-event split of the hadron total (up to 8192 events a uniform
+    event per slot and histograms, beyond that a binomial tree, one launch
+    per level): counts add up to the total, and are Poisson with the mean of
+    the total per event (chi^2 of the count histogram, p > 1e-3)."""
     sp = tables.species_arrays(1)
     cells, _ = synth.viscous_lrf(300, sp, seed=8, target_hadrons=30.0)
     ctx = iss_amd.Context(0)

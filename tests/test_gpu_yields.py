@@ -198,3 +198,34 @@ def test_empty_surface_and_single_cell(species):
     ref = orc.yields()
     nz = ref > 0
     assert np.max(np.abs(per[nz] - ref[nz]) / ref[nz]) < FP64_RTOL
+
+
+@pytest.mark.parametrize("name,df", MODES[:2], ids=[m[0] for m in MODES[:2]])
+def test_bessel_arguments_outside_the_staged_table(species, name, df):
+    """Hot cells (T = 0.3 GeV: pion m/T < 0.5, below the K_n table) and cold
+    ones (T = 16 MeV: heavy m/T beyond the rows staged in LDS) make the pass
+    rerun with the exact K_n and the whole global table
+    (src/FSSW.cpp:1646-1703 calls GSL outside the table): FP64 against the
+    oracle, FP32 against FP64, on the same bars as the other cells."""
+    cells, _ = synth.viscous_lrf(400, species, seed=12, muB_max=0.2)
+    cells = cells.copy()
+    cells[:20, 13] = 0.30
+    cells[20:40, 13] = 0.016
+    cells[20:40, 16] = 0.0
+    ctx = _ctx(species, df)
+    ctx.set_surface_lrf(cells)
+    t64, p64 = ctx.compute_yields(precision=64, per_cell=True)
+    orc = ob.Oracle(afterburner=1, **df)
+    orc.set_cells_lrf(cells)
+    ref = orc.yields()
+    nz = ref > 0
+    rel = np.abs(p64[nz] - ref[nz]) / ref[nz]
+    print(name, "max rel FP64", rel.max())
+    assert rel.max() < FP64_RTOL
+    assert np.allclose(t64, p64.sum(axis=0), rtol=1e-12, atol=0)
+    t32, p32 = ctx.compute_yields(precision=32, per_cell=True)
+    big = p64 > 1e-25
+    relc = np.abs(p32[big] - p64[big]) / p64[big]
+    print(name, "max rel FP32 per cell", relc.max())
+    assert relc.max() < FP32_RTOL
+    ctx.close()
