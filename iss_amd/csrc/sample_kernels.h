@@ -293,10 +293,13 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 
 // The cell of every hadron slot of a pass (the set-up then reads it instead
 // of searching the cell offsets per slot).  One wave per 64 consecutive
-// cells: the lanes load the cells' offsets, then the wave writes each cell's
-// run of slots in turn, lanes side by side, so the stores are contiguous.
-// Slots at or beyond `cap` (> 0) are not written.  Launch with
-// ceil(Nc / 64) waves.
+// cells; the lanes load the cells' offsets.  With many slots per cell (more
+// than kCellIdxCoop per lane on average) the wave writes each cell's run of
+// slots in turn, lanes side by side, so the stores are contiguous; with few
+// (batched rounds: about one slot per cell) each lane writes its own cell's
+// slots, which skips the per-cell steps of empty cells.  Slots at or beyond
+// `cap` (> 0) are not written.  Launch with ceil(Nc / 64) waves.
+constexpr int kCellIdxCoop = 4;
 __global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ off, int64_t Nc, int64_t cap,
                                                     uint32_t* __restrict__ cellidx) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -313,6 +316,11 @@ __global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ 
         }
     }
     const int nc = static_cast<int>(min<int64_t>(kWave, Nc - c0));
+    const int64_t first = __shfl(o0, 0), last = __shfl(o1, nc - 1);
+    if (last - first <= static_cast<int64_t>(kCellIdxCoop) * kWave) {
+        for (int64_t i = o0; i < o1; i++) cellidx[i] = static_cast<uint32_t>(c);
+        return;
+    }
     for (int j = 0; j < nc; j++) {
         const int64_t b = __shfl(o0, j), e = __shfl(o1, j);
         for (int64_t i = b + lane; i < e; i += kWave) cellidx[i] = static_cast<uint32_t>(c0 + j);
