@@ -314,7 +314,9 @@ int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* opts, const int32_t* 
  * hadron list of the last iss_sample by its stable descendants, event by
  * event (iss_get_event_counts / iss_get_hadrons read the result): two- and
  * three-body channels at the pole mass, other channel sizes leave no
- * daughters, as in the reference. ---- */
+ * daughters, as in the reference.  After iss_sample_batch it decays the
+ * batch's events the same way and rewrites the batch layout
+ * (iss_get_batch_layout) for the decayed list. ---- */
 typedef struct {
     int32_t n;
     const int32_t* monval;

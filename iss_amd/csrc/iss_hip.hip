@@ -2039,14 +2039,77 @@ int iss_set_decay_table(iss_ctx* ctx, const iss_decay_table* t) {
 }
 
 
+// The output of a batch as one list of events: blocks (round, surface) in
+// order, each holding its events contiguously (the layout of
+// iss_get_batch_layout).  Writes the event offsets to d_evoff, the number of
+// events to *E and each block's first event to first[] (rounds x S + 1).
+static int batch_events(iss_ctx* ctx, int* E, std::vector<int64_t>& first) {
+    const int S = ctx->batch_S, R = ctx->batch_R, rounds = ctx->batch_rounds;
+    const size_t nseg = static_cast<size_t>(rounds) * S;
+    std::vector<int64_t> seg(2 * nseg);
+    std::vector<uint32_t> ev(nseg * R);
+    HIPCHK(hipMemcpy(seg.data(), ctx->d_seg.p, sizeof(int64_t) * seg.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ev.data(), ctx->d_bevcount.p, sizeof(uint32_t) * ev.size(), hipMemcpyDeviceToHost));
+    std::vector<int64_t> off;
+    first.assign(nseg + 1, 0);
+    int64_t o = 0;
+    for (int r = 0; r < rounds; r++)
+        for (int k = 0; k < S; k++) {
+            const size_t b = static_cast<size_t>(r) * S + k;
+            first[b] = static_cast<int64_t>(off.size());
+            if (seg[2 * b] != o) return set_err(ctx, ISS_ERR_STATE, "batch output blocks are not contiguous");
+            const int64_t rem = static_cast<int64_t>(ctx->h_bevents[k]) - static_cast<int64_t>(r) * R;
+            const int n = rem <= 0 ? 0 : static_cast<int>(std::min<int64_t>(rem, R));
+            for (int e = 0; e < n; e++) {
+                off.push_back(o);
+                o += ev[b * R + e];
+            }
+            if (o != seg[2 * b] + seg[2 * b + 1]) return set_err(ctx, ISS_ERR_STATE, "batch event sizes disagree");
+        }
+    first[nseg] = static_cast<int64_t>(off.size());
+    if (o != ctx->K) return set_err(ctx, ISS_ERR_STATE, "batch layout does not cover the hadron list");
+    off.push_back(o);
+    if (off.size() - 1 > static_cast<size_t>(INT32_MAX)) return set_err(ctx, ISS_ERR_RANGE, "too many events");
+    *E = static_cast<int>(off.size() - 1);
+    HIPCHK(ctx->d_evoff.ensure(off.size()));
+    HIPCHK(hipMemcpy(ctx->d_evoff.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
+    return ISS_OK;
+}
+
+// the batch layout (segments, event sizes) from the event offsets in d_evoff
+// after the feed-down, block by block as batch_events laid them out
+static int batch_relayout(iss_ctx* ctx, int E, const std::vector<int64_t>& first) {
+    const int S = ctx->batch_S, R = ctx->batch_R, rounds = ctx->batch_rounds;
+    const size_t nseg = static_cast<size_t>(rounds) * S;
+    std::vector<int64_t> off(static_cast<size_t>(E) + 1), seg(2 * nseg);
+    std::vector<uint32_t> ev(nseg * R, 0u);
+    HIPCHK(hipMemcpy(off.data(), ctx->d_evoff.p, sizeof(int64_t) * off.size(), hipMemcpyDeviceToHost));
+    for (size_t b = 0; b < nseg; b++) {
+        const int64_t e0 = first[b], e1 = first[b + 1];
+        seg[2 * b] = off[e0];
+        seg[2 * b + 1] = off[e1] - off[e0];
+        for (int64_t e = e0; e < e1; e++) ev[b * R + (e - e0)] = static_cast<uint32_t>(off[e + 1] - off[e]);
+    }
+    HIPCHK(hipMemcpy(ctx->d_seg.p, seg.data(), sizeof(int64_t) * seg.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_bevcount.p, ev.data(), sizeof(uint32_t) * ev.size(), hipMemcpyHostToDevice));
+    return ISS_OK;
+}
+
 int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     if (!ctx) return ISS_ERR_ARG;
     if (ctx->dec_n == 0) return set_err(ctx, ISS_ERR_STATE, "no decay table (iss_set_decay_table)");
-    if (ctx->E <= 0 && ctx->K > 0)
-        return set_err(ctx, ISS_ERR_UNSUPPORTED, "decays apply to the hadron list of iss_sample (not to batches)");
     (void)hipSetDevice(ctx->device);
     HIPCHK(ctx->h_pin.ensure(PIN_TOT + sizeof(double) * std::max(ctx->Ns, 1)));
-    const int E = ctx->E;
+    // a batch's output decays as one list of its events, then gets its
+    // layout back
+    const bool batch = ctx->E <= 0 && ctx->K > 0 && ctx->batch_rounds > 0;
+    if (ctx->E <= 0 && ctx->K > 0 && !batch) return set_err(ctx, ISS_ERR_STATE, "hadron list without events");
+    std::vector<int64_t> bfirst;
+    int E = ctx->E;
+    if (batch) {
+        int rc = batch_events(ctx, &E, bfirst);
+        if (rc) return rc;
+    }
     // generation buffers live in the context and only grow
     auto& gens = ctx->dgen;
     if (gens.empty()) gens.emplace_back(new DecayGen);
@@ -2136,6 +2199,10 @@ int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     std::swap(ctx->d_out.p, out.p);
     std::swap(ctx->d_out.cap, out.cap);
     ctx->K = F;
+    if (batch) {
+        rc = batch_relayout(ctx, E, bfirst);
+        if (rc) return rc;
+    }
     ctx->stats.n_hadrons = F;
     ctx->stats.ms_decay = ms;
     ctx->stats.n_decay_generations = last + 1;

@@ -11,7 +11,9 @@ Checks:
     (Poisson mean and variance), species counts against its per-species yields;
   * one surface taken out of the batch against the same surface sampled alone
     (two-sample chi^2 of pT, y and species);
-  * the batch autotuner returns a tuned round size that is then used.
+  * the batch autotuner returns a tuned round size that is then used;
+  * decays of a batch's events equal the decays of the same list given as
+    a plain hadron list, and the layout follows the decayed list.
 """
 import numpy as np
 import pytest
@@ -188,4 +190,46 @@ def test_autotuned_round_size_is_used(species, surfaces):
     seg, ev = ctx.batch_layout()
     assert ev.shape[2] == rep["refill_min"]
     assert seg.shape[0] == int(np.ceil(max(EVENTS) / rep["refill_min"]))
+    ctx.close()
+
+
+def test_decays_of_a_batch(species, surfaces):
+    """iss_decay after iss_sample_batch: the batch's events decay as one list
+    (bit-identical to decaying the same hadrons and event sizes given through
+    iss_set_hadrons with the same seed), and the batch layout is rewritten:
+    segments tile the decayed list and every surface keeps its events.
+    (Charges are not checked per event: a channel of more than three bodies
+    leaves no daughters, as in the reference.)"""
+    ctx = _ctx(species, surfaces)
+    ctx.set_decay_table(tables.decay_table(1))
+    R = 8
+    n0 = ctx.sample_batch(EVENTS, seed=5, events_per_round=R)
+    h0 = ctx.hadrons()
+    seg0, ev0 = ctx.batch_layout()
+    sizes0 = np.concatenate([ev0[r, k, :max(0, min(R, EVENTS[k] - r * R))]
+                             for r in range(seg0.shape[0]) for k in range(len(EVENTS))])
+    assert sizes0.sum() == n0
+    n1 = ctx.decay(seed=23)
+    h1 = ctx.hadrons()
+    seg1, ev1 = ctx.batch_layout()
+    print("batch decays", n0, "->", n1, "events", sizes0.size)
+    assert n1 == h1.size > n0
+    flat = seg1.reshape(-1, 2)
+    assert flat[0, 0] == 0 and np.all(flat[1:, 0] == flat[:-1, 0] + flat[:-1, 1])
+    assert flat[-1, 0] + flat[-1, 1] == n1
+    assert np.array_equal(ev1.sum(axis=2), seg1[:, :, 1])
+    sizes1 = np.concatenate([ev1[r, k, :max(0, min(R, EVENTS[k] - r * R))]
+                             for r in range(seg1.shape[0]) for k in range(len(EVENTS))])
+    for k, E in enumerate(EVENTS):
+        hk, sk = ctx.batch_surface(k, hadrons=h1, layout=(seg1, ev1))
+        assert sk.size == E and sk.sum() == hk.size
+    # the same list decayed through iss_set_hadrons
+    ctx2 = iss_amd.Context(0)
+    ctx2.set_species(species)
+    ctx2.set_decay_table(tables.decay_table(1))
+    ctx2.set_hadrons(sizes0, h0)
+    assert ctx2.decay(seed=23) == n1
+    assert np.array_equal(ctx2.hadrons(), h1)
+    assert np.array_equal(ctx2.event_counts(), sizes1)
+    ctx2.close()
     ctx.close()
