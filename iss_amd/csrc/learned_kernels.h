@@ -281,10 +281,15 @@ __device__ inline void features_of(const SampleArgs& a, int64_t c, int s, const 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-    return static_cast<uint16_t>(u >> 16);
+// FP32 -> BF16, round to nearest even: the hardware conversion
+// (v_cvt_pk_bf16_f32), one instruction per pair
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+// two values as one packed word (a in the low half)
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    const f32x2_t v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
@@ -344,9 +349,12 @@ __device__ __forceinline__ float fcbrt(float x) {
 }
 
 // tanh-form GELU, 0.5 x (1 + tanh(u)) = x sigmoid(2u), and its derivative
+// with u = sqrt(2/pi) (x + 0.044715 x^3): sigmoid(2u) = 1 / (1 + 2^(x (A + B x^2))),
+// A = -2 log2(e) sqrt(2/pi), B = 0.044715 A (constants folded: three VALU
+// operations and two transcendentals)
 __device__ __forceinline__ float gelu_sig(float x) {
-    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-2.8853900817779268f * u));
+    constexpr float A = -2.8853900817779268f * 0.7978845608028654f, B = A * 0.044715f;
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * fmaf(B, x * x, A)));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
 // the logistic function with the hardware exp2 and reciprocal (1 ulp each)
@@ -418,11 +426,10 @@ __device__ __forceinline__ void store_act_s(const f32x16& acc, const float* __re
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int h0 = m0 + 8 * q + 4 * g;
-        uint16_t v[4];
+        float v[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) v[e] = f2bf(gelu(acc[4 * q + e] + bias[h0 + e]));
-        *reinterpret_cast<uint2*>(H + (n0 + r) * kHld + h0) =
-            make_uint2(v[0] | (uint32_t(v[1]) << 16), v[2] | (uint32_t(v[3]) << 16));
+        for (int e = 0; e < 4; e++) v[e] = gelu(acc[4 * q + e] + bias[h0 + e]);
+        *reinterpret_cast<uint2*>(H + (n0 + r) * kHld + h0) = make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
     }
 }
 
@@ -894,8 +901,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_learn_setup(LearnArgs L, LRec
         features_of(a, c, s, P, f);
     }
     for (int k = 0; k < kIn; k += 4)
-        *reinterpret_cast<uint2*>(Xs + lane * kXld + k) =
-            make_uint2(f2bf(f[k]) | (uint32_t(f2bf(f[k + 1])) << 16), f2bf(f[k + 2]) | (uint32_t(f2bf(f[k + 3])) << 16));
+        *reinterpret_cast<uint2*>(Xs + lane * kXld + k) = make_uint2(pack_bf16(f[k], f[k + 1]), pack_bf16(f[k + 2], f[k + 3]));
     mlp_forward_wave(L.W, Xs, Hs, Os);
     if (!live) return;
     float rho[kOut];
