@@ -419,18 +419,25 @@ __global__ void k_mlp_pack(const float* __restrict__ par, uint4* __restrict__ fr
 constexpr int kXld = 24, kHld = 72, kZld = 68;
 
 // store a 32x32 accumulator tile, after bias and GELU, into a sample-major
-// bf16 array H[sample][hid] (rows n0.., columns m0..)
-__device__ __forceinline__ void store_act_s(const f32x16& acc, const float* __restrict__ bias, uint16_t* H,
-                                            int m0, int n0) {
+// bf16 array H[sample][hid] (rows n0.., columns m0..); bv: the lane's 16
+// biases of the tile's rows m0 + 8 q + 4 g + e (bias_tile)
+__device__ __forceinline__ void store_act_s(const f32x16& acc, const float4 (&bv)[4], uint16_t* H, int m0, int n0) {
     const int l = threadIdx.x & 63, r = l & 31, g = l >> 5;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int h0 = m0 + 8 * q + 4 * g;
-        float v[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) v[e] = gelu(acc[4 * q + e] + bias[h0 + e]);
-        *reinterpret_cast<uint2*>(H + (n0 + r) * kHld + h0) = make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
+        const float v0 = gelu(acc[4 * q + 0] + bv[q].x), v1 = gelu(acc[4 * q + 1] + bv[q].y);
+        const float v2 = gelu(acc[4 * q + 2] + bv[q].z), v3 = gelu(acc[4 * q + 3] + bv[q].w);
+        *reinterpret_cast<uint2*>(H + (n0 + r) * kHld + h0) = make_uint2(pack_bf16(v0, v1), pack_bf16(v2, v3));
     }
+}
+
+// the lane's biases for store_act_s of the tile with rows m0.. (16-byte loads,
+// issued before the tile's MFMAs so that their latency overlaps them)
+__device__ __forceinline__ void bias_tile(const float* __restrict__ bias, int m0, float4 (&bv)[4]) {
+    const int g = (threadIdx.x & 63) >> 5;
+#pragma unroll
+    for (int q = 0; q < 4; q++) bv[q] = *reinterpret_cast<const float4*>(bias + m0 + 8 * q + 4 * g);
 }
 
 // Forward pass of one wave's 64 samples: X [64][kXld] (bf16) -> O [64][8]
@@ -440,19 +447,25 @@ __device__ void mlp_forward_wave(const MlpDev& W, uint16_t* Xs, uint16_t* Hs, fl
     const float* b1 = W.par + P_B1;
     const float* b2 = W.par + P_B2;
     const float* b3 = W.par + P_B3;
+    float4 bv0[4], bv1[4];
+    bias_tile(b1, 0, bv0);
+    bias_tile(b1, 32, bv1);
     wave_lds_sync();
-    for (int nt = 0; nt < 2; nt++)
-        for (int mt = 0; mt < 2; mt++) {
-            const f32x16 acc = tile_gl(W.frag + F_W1 + mt * 64, Xs, kXld, nt * 32, 1, zero16());
-            store_act_s(acc, b1, Hs, mt * 32, nt * 32);
-        }
+    for (int nt = 0; nt < 2; nt++) {
+        const f32x16 a0 = tile_gl(W.frag + F_W1 + 0 * 64, Xs, kXld, nt * 32, 1, zero16());
+        store_act_s(a0, bv0, Hs, 0, nt * 32);
+        const f32x16 a1 = tile_gl(W.frag + F_W1 + 1 * 64, Xs, kXld, nt * 32, 1, zero16());
+        store_act_s(a1, bv1, Hs, 32, nt * 32);
+    }
+    bias_tile(b2, 0, bv0);
+    bias_tile(b2, 32, bv1);
     wave_lds_sync();
     for (int nt = 0; nt < 2; nt++) {
         const f32x16 a0 = tile_gl(W.frag + F_W2 + 0 * 4 * 64, Hs, kHld, nt * 32, 4, zero16());
         const f32x16 a1 = tile_gl(W.frag + F_W2 + 1 * 4 * 64, Hs, kHld, nt * 32, 4, zero16());
         wave_lds_sync();
-        store_act_s(a0, b2, Hs, 0, nt * 32);
-        store_act_s(a1, b2, Hs, 32, nt * 32);
+        store_act_s(a0, bv0, Hs, 0, nt * 32);
+        store_act_s(a1, bv1, Hs, 32, nt * 32);
         wave_lds_sync();
     }
     for (int nt = 0; nt < 2; nt++) {
