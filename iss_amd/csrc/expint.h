@@ -1,11 +1,15 @@
 // Exponential integral E_n(x) = int_1^inf exp(-x t) t^-n dt for integer n >= 1
 // and x > 0, in FP64, for the host and the device.
-//   x >= 1: the continued fraction (Abramowitz & Stegun 5.1.22)
-//           E_n(x) = exp(-x) / (x + n - 1 n / (x + n + 2 - 2 (n + 1) / (x + n + 4 - ...)))
-//           evaluated with the modified Lentz method;
-//   x < 1:  the power series (A&S 5.1.12)
-//           E_n(x) = (-x)^(n-1) / (n-1)! (psi(n) - ln x) - sum_{k != n-1} (-x)^k / ((k - n + 1) k!),
-//           psi(n) = -gamma + sum_{m < n} 1 / m.
+//
+//   x < 1:  E_1 from its power series (Abramowitz & Stegun 5.1.11),
+//             E_1(x) = -gamma - ln x + sum_{k>=1} (-1)^(k+1) x^k / (k k!),
+//           then E_2 ... E_n by the upward recurrence (A&S 5.1.14)
+//             E_{m+1}(x) = (exp(-x) - x E_m(x)) / m,
+//           which shrinks an error of E_m by the factor x / m < 1 per step.
+//   x >= 1: the continued fraction of A&S 5.1.22 in its even (contracted)
+//           form, exp(x) E_n(x) = 1 / (b_0 + a_1 / (b_1 + a_2 / (b_2 + ...)))
+//           with b_k = x + n + 2k and a_k = -k (n + k - 1), summed from the
+//           bottom up at a depth that doubles until two depths agree.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,38 +18,42 @@
 
 namespace issdev {
 
+// exp(x) E_n(x) from the even continued fraction truncated after `depth`
+// levels (the tail below the last level is dropped)
+__host__ __device__ inline double expint_cf_tail(int n, double x, int depth) {
+    double t = x + n + 2.0 * depth;
+    for (int k = depth; k >= 1; k--) {
+        const double ak = -static_cast<double>(k) * static_cast<double>(n + k - 1);
+        t = (x + n + 2.0 * (k - 1)) + ak / t;
+    }
+    return 1.0 / t;
+}
+
 __host__ __device__ inline double expint_en(int n, double x) {
-    const double eps = 1e-16, tiny = 1e-300;
     if (x >= 1.0) {
-        double b = x + n, c = 1.0 / tiny, d = 1.0 / b, h = d;
-        for (int i = 1; i < 1000; i++) {
-            const double a = -static_cast<double>(i) * (n - 1 + i);
-            b += 2.0;
-            d = 1.0 / (a * d + b);
-            c = b + a / c;
-            const double del = c * d;
-            h *= del;
-            if (fabs(del - 1.0) < eps) break;
+        double prev = expint_cf_tail(n, x, 8);
+        for (int depth = 16; depth <= 8192; depth *= 2) {
+            const double cur = expint_cf_tail(n, x, depth);
+            const bool agree = fabs(cur - prev) <= 1e-16 * fabs(cur);
+            prev = cur;
+            if (agree) break;
         }
-        return h * exp(-x);
+        return prev * exp(-x);
     }
-    const int nm1 = n - 1;
-    double ans = (nm1 != 0) ? 1.0 / nm1 : -log(x) - 0.5772156649015329;
-    double fact = 1.0;
-    for (int i = 1; i < 1000; i++) {
-        fact *= -x / i;
-        double del;
-        if (i != nm1) {
-            del = -fact / (i - nm1);
-        } else {
-            double psi = -0.5772156649015329;
-            for (int m = 1; m <= nm1; m++) psi += 1.0 / m;
-            del = fact * (-log(x) + psi);
-        }
-        ans += del;
-        if (fabs(del) < fabs(ans) * eps) break;
+    // E_1 by its series: term_k = (-1)^(k+1) x^k / k!, added as term_k / k
+    const double euler_gamma = 0.57721566490153286061;
+    double e1 = -euler_gamma - log(x);
+    double term = 1.0;
+    for (int k = 1; k < 64; k++) {
+        term *= -x / k;
+        const double add = -term / k;
+        e1 += add;
+        if (fabs(add) < 1e-17 * fabs(e1)) break;
     }
-    return ans;
+    const double emx = exp(-x);
+    double em = e1;
+    for (int m = 1; m < n; m++) em = (emx - x * em) / m;
+    return em;
 }
 
 }  // namespace issdev

@@ -255,6 +255,26 @@ int set_err(iss_ctx* ctx, int code, const std::string& m) {
     return code;
 }
 
+// Forget the hadron list of the previous pass (single surface or batch).  A
+// pass that fails part-way, or a batch round that overflowed its reserved
+// output, must leave no count that points past what was written: the
+// getters and iss_decay then see an empty list.
+void invalidate_output(iss_ctx* ctx) {
+    ctx->K = 0;
+    ctx->E = 0;
+    ctx->batch_rounds = 0;
+    ctx->batch_S = 0;
+    ctx->h_bevents.clear();
+    ctx->last_plain = false;
+}
+
+// the context's hadron list is readable: its count fits the output buffer
+int check_output(iss_ctx* ctx) {
+    if (ctx->K < 0 || (ctx->K > 0 && (!ctx->d_out.p || static_cast<size_t>(ctx->K) > ctx->d_out.cap)))
+        return set_err(ctx, ISS_ERR_STATE, "hadron count exceeds the output buffer");
+    return ISS_OK;
+}
+
 int launch_scan(iss_ctx* ctx, const uint32_t* in, int64_t n, int64_t* out) {
     const int64_t ntiles = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
     HIPCHK(ctx->d_scan.ensure(ntiles + 1));
@@ -1046,6 +1066,7 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     }
     if ((model == 10 || model == 20) && !(o->sampling_para1 > 0.))
         return set_err(ctx, ISS_ERR_ARG, "dN_dy_sampling_para1 must be positive for the negative binomial models");
+    invalidate_output(ctx);
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
     const int64_t Nc = ctx->Nc;
     const int Ns = ctx->Ns;
@@ -1150,6 +1171,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     if (o->proposal == 1 && !ctx->mlp_ready)
         return set_err(ctx, ISS_ERR_STATE, "learned proposal: no network parameters (init, set or train)");
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
+    invalidate_output(ctx);
     const int lcc = o->local_charge_conservation ? 1 : 0;
     if (lcc) {
         int rc = lcc_check(ctx);
@@ -1375,6 +1397,7 @@ int iss_get_event_counts(iss_ctx* ctx, int64_t* counts) {
 
 int iss_get_hadrons(iss_ctx* ctx, iss_hadron* dst, int64_t capacity) {
     if (!ctx || (!dst && ctx->K > 0)) return ISS_ERR_ARG;
+    if (int rc = check_output(ctx)) return rc;
     if (capacity < ctx->K) return set_err(ctx, ISS_ERR_RANGE, "capacity smaller than the number of hadrons");
     (void)hipSetDevice(ctx->device);
     static_assert(sizeof(Hadron) == sizeof(iss_hadron), "hadron layout");
@@ -1740,6 +1763,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     }
     const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned;
     if (R > 4096) return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most 4096");
+    invalidate_output(ctx);
     if (!ctx->yields_valid || ctx->yields_lcc ||
         (o->precision != ctx->yield_precision && (o->precision == 32 || o->precision == 64))) {
         int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr);
@@ -1895,9 +1919,14 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     ctx->stats.n_violations = 0;
     ctx->stats.ms_sample = ms;
     ctx->stats.ms_sample_kernel = ms;
-    if (n_hadrons) *n_hadrons = ctx->K;
-    if (st[6] || st[7] || ctx->K > out_cap)
+    if (st[6] || st[7] || ctx->K > out_cap) {
+        // slots past the reserved output or set-up records were never
+        // written: no list is left for the getters or iss_decay
+        invalidate_output(ctx);
+        if (n_hadrons) *n_hadrons = 0;
         return set_err(ctx, ISS_ERR_RANGE, "batched sampling: a round exceeded the reserved output (10 sigma)");
+    }
+    if (n_hadrons) *n_hadrons = ctx->K;
     if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
     return ISS_OK;
 }
@@ -2098,6 +2127,7 @@ static int batch_relayout(iss_ctx* ctx, int E, const std::vector<int64_t>& first
 int iss_decay(iss_ctx* ctx, uint64_t seed, int64_t* n_hadrons) {
     if (!ctx) return ISS_ERR_ARG;
     if (ctx->dec_n == 0) return set_err(ctx, ISS_ERR_STATE, "no decay table (iss_set_decay_table)");
+    if (int rc = check_output(ctx)) return rc;
     (void)hipSetDevice(ctx->device);
     HIPCHK(ctx->h_pin.ensure(PIN_TOT + sizeof(double) * std::max(ctx->Ns, 1)));
     // a batch's output decays as one list of its events, then gets its
@@ -2227,6 +2257,7 @@ int iss_set_hadrons(iss_ctx* ctx, int32_t n_events, const int64_t* counts, const
     const int64_t K = off[n_events];
     if (K > 0 && !hadrons) return ISS_ERR_ARG;
     (void)hipSetDevice(ctx->device);
+    invalidate_output(ctx);
     HIPCHK(ctx->d_out.ensure(std::max<int64_t>(K, 1)));
     HIPCHK(ctx->d_evoff.ensure(n_events + 1));
     if (K > 0) HIPCHK(hipMemcpy(ctx->d_out.p, hadrons, sizeof(Hadron) * K, hipMemcpyHostToDevice));

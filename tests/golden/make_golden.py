@@ -15,6 +15,7 @@ Run from the repository root: python tests/golden/make_golden.py
 import ctypes as C
 import os
 import sys
+import tempfile
 
 import numpy as np
 from scipy.special import kn
@@ -56,30 +57,36 @@ def ref_fixtures():
     np.savez_compressed(os.path.join(HERE, "ref_draws.npz"), **draws)
 
     reader = {}
-    work = os.path.join(ROOT, "_work", "refread")
-    os.makedirs(work, exist_ok=True)
-    for name, c in ob.REF_CASES.items():
-        cfg = ob.case_config(name)
-        with open(os.path.join(work, "music_input"), "w") as f:
-            f.write(f"EOS_to_use {cfg['iEOS']}\nInclude_Bulk_Visc_Yes_1_No_0  1\n"
-                    f"Include_Rhob_Yes_1_No_0  {cfg['rhob']}\nturn_on_baryon_diffusion  0\n"
-                    f"freeze_surface_in_binary  0\nEndOfData\n")
-        src = os.path.join(ob.FIXTURES, c["surface"])
-        dst = os.path.join(work, c["surface"])
-        open(dst, "w").write(open(src).read())
-        ab = 2 if c["params"] == "CE" else 1
-        n = r.ref_read_surface(work.encode(), ob.TABLE_DIR.encode(), c["surface"].encode(), 2, ab, 1,
-                               1, cfg["rhob"], 0)
-        cells = np.zeros((n, 34), dtype=np.float32)
-        r.ref_get_cells(cells.ctypes.data_as(C.c_void_p))
-        reader[name] = cells
-        npart = r.ref_num_particles()
-        arrs = [np.zeros(npart, dtype=np.int32) for _ in range(7)]
-        arrs[1] = np.zeros(npart)
-        r.ref_particles(*[a.ctypes.data_as(C.c_void_p) for a in arrs])
-        reader[name + "_pdg"] = np.stack([arrs[0], arrs[2], arrs[3], arrs[4], arrs[5], arrs[6]])
-        reader[name + "_mass"] = arrs[1]
+    with tempfile.TemporaryDirectory(prefix="issref_") as work:
+        for name, c in ob.REF_CASES.items():
+            reader.update(_read_case(r, ob, work, name, c))
     np.savez_compressed(os.path.join(HERE, "ref_reader.npz"), **reader)
+
+
+def _read_case(r, ob, work, name, c):
+    """One fixture surface through the reference reader, in scratch dir `work`."""
+    reader = {}
+    cfg = ob.case_config(name)
+    with open(os.path.join(work, "music_input"), "w") as f:
+        f.write(f"EOS_to_use {cfg['iEOS']}\nInclude_Bulk_Visc_Yes_1_No_0  1\n"
+                f"Include_Rhob_Yes_1_No_0  {cfg['rhob']}\nturn_on_baryon_diffusion  0\n"
+                f"freeze_surface_in_binary  0\nEndOfData\n")
+    src = os.path.join(ob.FIXTURES, c["surface"])
+    dst = os.path.join(work, c["surface"])
+    open(dst, "w").write(open(src).read())
+    ab = 2 if c["params"] == "CE" else 1
+    n = r.ref_read_surface(work.encode(), ob.TABLE_DIR.encode(), c["surface"].encode(), 2, ab, 1,
+                           1, cfg["rhob"], 0)
+    cells = np.zeros((n, 34), dtype=np.float32)
+    r.ref_get_cells(cells.ctypes.data_as(C.c_void_p))
+    reader[name] = cells
+    npart = r.ref_num_particles()
+    arrs = [np.zeros(npart, dtype=np.int32) for _ in range(7)]
+    arrs[1] = np.zeros(npart)
+    r.ref_particles(*[a.ctypes.data_as(C.c_void_p) for a in arrs])
+    reader[name + "_pdg"] = np.stack([arrs[0], arrs[2], arrs[3], arrs[4], arrs[5], arrs[6]])
+    reader[name + "_mass"] = arrs[1]
+    return reader
 
 
 if __name__ == "__main__":
