@@ -23,6 +23,7 @@ from scipy import special, stats
 
 import iss_amd
 from iss_amd import synth, tables
+from tests import neos22_table
 from tests import oracle_binding as ob
 
 pytestmark = pytest.mark.gpu
@@ -37,6 +38,7 @@ MODES = [
     ("rta_bulk", dict(shear=1, bulk=1, bulk_kind=1)),
     ("osu14", dict(shear=1, bulk=1, bulk_kind=11)),
     ("ce21", dict(shear=1, bulk=1, bulk_kind=21)),
+    ("mom22", dict(shear=1, bulk=1, bulk_kind=20)),   # synthetic table (tests/neos22_table.py)
 ]
 PIDS = [211, -211, 321, 2212, -2212, 3122]
 
@@ -49,8 +51,19 @@ def _ctx(sp, df):
         kw["k11"] = tables.read_kind11(1)
     if df["bulk_kind"] == 21:
         kw["neos"] = tables.read_neos(21, 1)
+    if df["bulk_kind"] == 20:
+        kw["neos"] = neos22_table.table()
     ctx.set_deltaf(**kw)
     return ctx
+
+
+@pytest.fixture(scope="module")
+def table_dir22(tmp_path_factory):
+    return neos22_table.table_dir(str(tmp_path_factory.mktemp("neos22")))
+
+
+def _oracle(df, td22):
+    return ob.Oracle(afterburner=1, table_dir=td22, **df) if df["bulk_kind"] == 20 else ob.Oracle(afterburner=1, **df)
 
 
 def _idx(sp, pids):
@@ -73,14 +86,14 @@ def bins():
 
 
 @pytest.mark.parametrize("name,df", MODES, ids=[m[0] for m in MODES])
-def test_fp64_spectra_match_oracle_reference_grid(species, bins, name, df):
+def test_fp64_spectra_match_oracle_reference_grid(species, bins, name, df, table_dir22):
     lab = synth.viscous_lab(300, species, seed=11)
     ctx = _ctx(species, df)
     ctx.set_surface_lab(lab)
     assert 0 < ctx.num_cells() < 300   # some cells have u.dsigma < 0
     idx = _idx(species, PIDS)
     args = (idx, bins["pT"], bins["phi"], bins["eta"], bins["eta_weight"])
-    orc = ob.Oracle(afterburner=1, **df)
+    orc = _oracle(df, table_dir22)
     orc.set_cells_lab(lab)
     for restrict in (1, 0):
         g = ctx.spectra(*args, precision=64, all_cells=1, restrict_deltaf=restrict)
@@ -90,7 +103,7 @@ def test_fp64_spectra_match_oracle_reference_grid(species, bins, name, df):
         assert err < FP64_ROW_TOL
     # the sampled cells only
     lrf, kept = ob.transform(ob.regulate(lab))
-    orc_k = ob.Oracle(afterburner=1, **df)
+    orc_k = _oracle(df, table_dir22)
     orc_k.set_cells_lab(lab[kept])
     g = ctx.spectra(*args, precision=64, all_cells=0)
     assert _row_err(g, orc_k.spectra(*args)) < FP64_ROW_TOL

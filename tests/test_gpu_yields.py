@@ -9,6 +9,7 @@ import pytest
 
 import iss_amd
 from iss_amd import synth, tables
+from tests import neos22_table
 from tests import oracle_binding as ob
 
 pytestmark = pytest.mark.gpu
@@ -23,6 +24,9 @@ MODES = [
     ("rta_bulk", dict(shear=1, bulk=1, bulk_kind=1)),
     ("osu14", dict(shear=1, bulk=1, bulk_kind=11)),
     ("ce21", dict(shear=1, bulk=1, bulk_kind=21)),
+    # 22-moment NEoS: a synthetic table of the reference's layout (its own
+    # file is not shipped; tests/neos22_table.py)
+    ("mom22", dict(shear=1, bulk=1, bulk_kind=20)),
 ]
 
 
@@ -34,8 +38,17 @@ def _ctx(sp, df):
         kw["k11"] = tables.read_kind11(1)
     if df["bulk_kind"] == 21:
         kw["neos"] = tables.read_neos(21, 1)
+    if df["bulk_kind"] == 20:
+        kw["neos"] = neos22_table.table()
     ctx.set_deltaf(**kw)
     return ctx
+
+
+def _oracle(df, tmp_path_factory):
+    if df["bulk_kind"] == 20:
+        td = neos22_table.table_dir(str(tmp_path_factory.mktemp("neos22")))
+        return ob.Oracle(afterburner=1, table_dir=td, **df)
+    return ob.Oracle(afterburner=1, **df)
 
 
 @pytest.fixture(scope="module")
@@ -44,12 +57,12 @@ def species():
 
 
 @pytest.mark.parametrize("name,df", MODES, ids=[m[0] for m in MODES])
-def test_fp64_per_cell_yields_match_oracle(species, name, df):
+def test_fp64_per_cell_yields_match_oracle(species, name, df, tmp_path_factory):
     cells, _ = synth.viscous_lrf(3000, species, seed=7, muB_max=0.45)
     ctx = _ctx(species, df)
     ctx.set_surface_lrf(cells)
     tot, per = ctx.compute_yields(precision=64, per_cell=True)
-    orc = ob.Oracle(afterburner=1, **df)
+    orc = _oracle(df, tmp_path_factory)
     orc.set_cells_lrf(cells)
     ref = orc.yields()
     assert per.shape == ref.shape
