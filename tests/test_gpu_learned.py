@@ -25,7 +25,8 @@ from scipy import stats as sst
 import iss_amd
 from iss_amd import synth, tables
 from tests import oracle_binding as ob
-from tests.test_gpu_sampling import GROUPS, _obs, _quantile_bins, _two_sample_chi2, tmunu_from_hadrons
+from tests.sample_stats import GROUPS, obs as _obs, quantile_bins as _quantile_bins, two_sample_chi2 as _two_sample_chi2
+from tests.test_gpu_sampling import tmunu_from_hadrons
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import mlp_oracle as mo  # noqa: E402

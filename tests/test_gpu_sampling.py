@@ -15,6 +15,7 @@ from scipy import stats as sst
 import iss_amd
 from iss_amd import synth, tables
 from tests import oracle_binding as ob
+from tests.sample_stats import GROUPS, group_chi2 as _gc
 
 pytestmark = pytest.mark.gpu
 
@@ -67,23 +68,33 @@ def test_reference_tmunu_reconstruction(case):
     assert np.mean(np.abs(diff)) < 1e-3
 
 
+VISCOUS_DF = dict(shear=1, bulk=1, bulk_kind=11)
+
+
 @pytest.fixture(scope="module")
-def viscous_pair():
-    """GPU and oracle samples of the same small viscous surface."""
-    df = dict(shear=1, bulk=1, bulk_kind=11)
-    ctx, sp = _gpu_ctx(1, **df)
+def viscous_oracle():
+    """The oracle's sample of a small viscous surface (FP64 throughout)."""
+    sp = tables.species_arrays(1)
     cells, _ = synth.viscous_lrf(400, sp, seed=31, target_hadrons=800.0)
-    ctx.set_surface_lrf(cells)
-    tot = ctx.compute_yields(precision=64)
-    nev_g = 100000
-    ctx.sample(nev_g, seed=77, precision=64)
-    hg = ctx.hadrons()
-    cg = ctx.event_counts()
-    orc = ob.Oracle(afterburner=1, **df)
+    orc = ob.Oracle(afterburner=1, **VISCOUS_DF)
     orc.set_cells_lrf(cells)
     nev_o = 10000
     co, ho = orc.sample(99, nev_o)
-    return dict(sp=sp, tot=tot, hg=hg, cg=cg, nev_g=nev_g, ho=ho, co=co, nev_o=nev_o)
+    return dict(cells=cells, ho=ho, co=co, nev_o=nev_o)
+
+
+@pytest.fixture(scope="module", params=[64, 32], ids=["fp64_yields", "fp32_yields"])
+def viscous_pair(request, viscous_oracle):
+    """GPU sample of the same surface against the oracle's: FP64 yields (the
+    reference's arithmetic) and the production FP32 yields."""
+    ctx, sp = _gpu_ctx(1, **VISCOUS_DF)
+    ctx.set_surface_lrf(viscous_oracle["cells"])
+    tot = ctx.compute_yields(precision=request.param)
+    nev_g = 100000
+    ctx.sample(nev_g, seed=77 + request.param, precision=request.param)
+    hg = ctx.hadrons()
+    cg = ctx.event_counts()
+    return dict(sp=sp, tot=tot, hg=hg, cg=cg, nev_g=nev_g, **{k: viscous_oracle[k] for k in ("ho", "co", "nev_o")})
 
 
 def test_species_multiplicities_within_3sigma(viscous_pair):
@@ -110,74 +121,13 @@ def test_event_multiplicity_is_poisson(viscous_pair):
     assert abs(c.var() / lam - 1) < 0.1
 
 
-def _two_sample_chi2(a, wa, b, wb, bins):
-    """Two-sample chi^2 of histograms of samples a, b drawn from wa and wb
-    events (scaled for unequal sizes); returns (chi2, ndf)."""
-    ha, _ = np.histogram(a, bins=bins)
-    hb, _ = np.histogram(b, bins=bins)
-    k1, k2 = np.sqrt(wb / wa), np.sqrt(wa / wb)
-    m = (ha + hb) > 0
-    chi2 = np.sum((k1 * ha[m] - k2 * hb[m]) ** 2 / (ha[m] + hb[m]))
-    return chi2, m.sum() - 1
-
-
-def _quantile_bins(a, b, lo, hi, per_bin=25, max_bins=60):
-    """equal-probability bins of the pooled sample, >= per_bin oracle counts"""
-    nb = int(np.clip(min(a.size, b.size) // per_bin, 5, max_bins))
-    pool = np.concatenate([a, b])
-    pool = pool[(pool > lo) & (pool < hi)]
-    edges = np.unique(np.quantile(pool, np.linspace(0, 1, nb + 1)))
-    edges[0], edges[-1] = lo, hi
-    return edges
-
-
-def _obs(h):
-    pT = np.hypot(h["px"], h["py"]).astype(np.float64)
-    mT = np.sqrt(pT ** 2 + h["mass"].astype(np.float64) ** 2)
-    y = np.arcsinh(h["pz"] / mT)
-    phi = np.arctan2(h["py"], h["px"])
-    return pT, y, phi
-
-
-GROUPS = {"pion": [211, -211, 111], "kaon": [321, -321, 311, -311], "proton": [2212, -2212],
-          "other": None}
-
-
-def _group_chi2(d, pids):
-    hg, ho = d["hg"], d["ho"]
-    if pids is None:
-        allp = sum((v for v in GROUPS.values() if v), [])
-        hg = hg[~np.isin(hg["pid"], allp)]
-        ho = ho[~np.isin(ho["pid"], allp)]
-    else:
-        hg = hg[np.isin(hg["pid"], pids)]
-        ho = ho[np.isin(ho["pid"], pids)]
-    pg, yg, fg = _obs(hg)
-    po, yo, fo = _obs(ho)
-    chi, ndf = 0.0, 0
-    for a, b, lo, hi in ((pg, po, 0.0, 10.0), (yg, yo, -12.0, 12.0), (fg, fo, -np.pi, np.pi)):
-        c, n = _two_sample_chi2(a, d["nev_g"], b, d["nev_o"], _quantile_bins(a, b, lo, hi))
-        chi += c
-        ndf += n
-    # v2(pT): <cos 2 phi> in pT bins, difference over the combined error
-    edges = _quantile_bins(pg, po, 0.0, 10.0, per_bin=400, max_bins=12)
-    for k in range(len(edges) - 1):
-        cg = np.cos(2 * fg[(pg >= edges[k]) & (pg < edges[k + 1])])
-        co = np.cos(2 * fo[(po >= edges[k]) & (po < edges[k + 1])])
-        if cg.size > 1 and co.size > 1:
-            chi += (cg.mean() - co.mean()) ** 2 / (cg.var() / cg.size + co.var() / co.size)
-            ndf += 1
-    ks = sst.ks_2samp(pg, po).pvalue
-    return chi, ndf, ks, pg.size, po.size
-
-
 def test_spectra_chi2_and_ks_vs_oracle(viscous_pair):
     """dN/dy, pT, phi and v2(pT) of pions, kaons, protons and the rest:
     combined chi^2/ndf <= 1.1; each group within 4 sigma of chi^2/ndf = 1;
     KS p > 0.01 on every group's pT."""
     tot_chi, tot_ndf = 0.0, 0
     for name, pids in GROUPS.items():
-        chi, ndf, ks, ng, no = _group_chi2(viscous_pair, pids)
+        chi, ndf, ks, ng, no = _gc(viscous_pair, pids)
         print(name, "chi2/ndf %.3f ndf %d KS p %.3g N %d %d" % (chi / ndf, ndf, ks, ng, no))
         assert chi / ndf <= 1 + 4 * np.sqrt(2.0 / ndf)
         assert ks > 0.01
