@@ -258,6 +258,100 @@ struct Permutation {
     }
 };
 
+// LDS traffic between lanes of one wave: wait for this wave's LDS operations
+// and keep the compiler from moving memory accesses across
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// ------------------------------------------------------ DPP lane exchange
+// Data-parallel-primitive moves between lanes (no LDS round trip):
+// row_shr:n (0x110 + n) shifts within rows of 16 lanes; row_bcast:15 (0x142)
+// and row_bcast:31 (0x143) pass lane 15 of each row, and lane 31, to the
+// following rows.  Lanes without a source, and rows outside row_mask, take
+// `old`.
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(v), CTRL,
+                                                             ROW_MASK, 0xf, false));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double old, double v) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK>(static_cast<uint32_t>(o), static_cast<uint32_t>(x));
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK>(static_cast<uint32_t>(o >> 32), static_cast<uint32_t>(x >> 32));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Inclusive sum over the lanes up to this one, FP64, in DPP steps: within
+// rows of 16 (shifts 1, 2, 4, 8), then the rows' totals (rows 1 and 3 add
+// row 0 and 2's last lane, rows 2 and 3 add lane 31).  Every lane's value
+// comes from a fixed sequence of additions (deterministic).
+__device__ __forceinline__ double wave_scan_dpp(double v) {
+    v += dpp_f64<kDppRowShr + 1>(0., v);
+    v += dpp_f64<kDppRowShr + 2>(0., v);
+    v += dpp_f64<kDppRowShr + 4>(0., v);
+    v += dpp_f64<kDppRowShr + 8>(0., v);
+    v += dpp_f64<kDppRowBcast15, 0xa>(0., v);
+    v += dpp_f64<kDppRowBcast31, 0xc>(0., v);
+    return v;
+}
+// the sum over all lanes, returned uniformly (from lane 63 of the scan)
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = wave_scan_dpp(v);
+    const uint64_t x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(x), 63);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), 63);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// The 64-lane sums of 8 values at once (a transposed butterfly): lanes
+// exchange half of their values with the partner lane and add, halving the
+// values each step (lane l ^ 32 through v_permlane32_swap, l ^ 16 through
+// v_permlane16_swap, l ^ 8 through DPP row_ror:8), then the group of 8 lanes
+// adds its values pairwise (row_half_mirror, then the quad).  Lane l returns
+// the sum of v[j], j = b3 + 2 b4 + 4 b5 (b_k bit k of l), bit-identical in
+// every lane of its group of 8 (the same pairs in the same order); ~5
+// instructions per value instead of ~21 for a DPP scan of each.
+__device__ __forceinline__ double pair_add32(double a, double b) {
+    const uint64_t x = __builtin_bit_cast(uint64_t, a), y = __builtin_bit_cast(uint64_t, b);
+    const auto lo = __builtin_amdgcn_permlane32_swap(static_cast<uint32_t>(x), static_cast<uint32_t>(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(static_cast<uint32_t>(x >> 32), static_cast<uint32_t>(y >> 32),
+                                                     false, false);
+    const double r0 = __builtin_bit_cast(double, (static_cast<uint64_t>(hi[0]) << 32) | lo[0]);
+    const double r1 = __builtin_bit_cast(double, (static_cast<uint64_t>(hi[1]) << 32) | lo[1]);
+    return r0 + r1;
+}
+__device__ __forceinline__ double pair_add16(double a, double b) {
+    const uint64_t x = __builtin_bit_cast(uint64_t, a), y = __builtin_bit_cast(uint64_t, b);
+    const auto lo = __builtin_amdgcn_permlane16_swap(static_cast<uint32_t>(x), static_cast<uint32_t>(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(static_cast<uint32_t>(x >> 32), static_cast<uint32_t>(y >> 32),
+                                                     false, false);
+    const double r0 = __builtin_bit_cast(double, (static_cast<uint64_t>(hi[0]) << 32) | lo[0]);
+    const double r1 = __builtin_bit_cast(double, (static_cast<uint64_t>(hi[1]) << 32) | lo[1]);
+    return r0 + r1;
+}
+constexpr int kDppRowRor8 = 0x128, kDppRowHalfMirror = 0x141, kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E;
+
+__device__ __forceinline__ double wave_sum8(const double (&v)[8]) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double a[4], b[2];
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[k] = pair_add32(v[k], v[k + 4]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) b[k] = pair_add16(a[k], a[k + 2]);
+    const bool b3 = (lane & 8) != 0;
+    const double send = b3 ? b[0] : b[1], keep = b3 ? b[1] : b[0];
+    double d = keep + dpp_f64<kDppRowRor8>(0., send);
+    d += dpp_f64<kDppRowHalfMirror>(0., d);
+    d += dpp_f64<kDppQuadXor1>(0., d);
+    d += dpp_f64<kDppQuadXor2>(0., d);
+    return d;
+}
+__device__ __forceinline__ int wave_sum8_index(int lane) {
+    return ((lane >> 3) & 1) | (((lane >> 4) & 1) << 1) | (((lane >> 5) & 1) << 2);
+}
+
 // ------------------------------------------------------------- wave scans
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_scan(T v) {
@@ -266,6 +360,18 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
     for (int d = 1; d < kWave; d <<= 1) {
         const T o = __shfl_up(v, d, kWave);
         if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// running maximum over the lanes up to this one
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_max(T v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T o = __shfl_up(v, d, kWave);
+        if (lane >= d) v = v > o ? v : o;
     }
     return v;
 }

@@ -172,9 +172,16 @@ struct iss_ctx {
     DBuf<uint32_t> d_u32;
     DBuf<int64_t> d_i64, d_scan;
     DBuf<double> d_eos;
-    // yields
+    // yields: LAYOUT_ROWS (k_yield_rows, the Poisson pass of one surface) or
+    // LAYOUT_CDF (k_yield, FP64 species CDF rows: batches, multiplicity models)
     bool yields_valid = false;
     int yield_precision = 0;
+    int yield_layout = 0;
+    int nlight = 0;  // leading species below 0.7 GeV (the 10-term series of the row pass)
+    bool light_beyond = false;  // a species below 0.7 GeV after the first nlight ones
+    DBuf<char> d_rows;
+    DBuf<RowSp> d_rowsp;       // [2][row_stride(Ns, 4)]: all species, then negative charges suppressed
+    bool charges_outside = false;  // a species charge beyond the row pass's fugacity rows
     DBuf<double> d_cdf, d_ctot, d_partial, d_stot;
     DBuf<int> d_oob;
     DBuf<char> d_raw;
@@ -254,6 +261,9 @@ int set_err(iss_ctx* ctx, int code, const std::string& m) {
     ctx->err = m;
     return code;
 }
+
+enum YieldLayout { LAYOUT_ROWS = 1, LAYOUT_CDF = 2 };
+constexpr int kPartialBlocksPerCU = 8;  // yield-pass workgroups per CU at most (species partials)
 
 // Forget the hadron list of the previous pass (single surface or batch).  A
 // pass that fails part-way, or a batch round that overflowed its reserved
@@ -420,6 +430,7 @@ int launch_yield_t(iss_ctx* ctx, const YieldArgs& a) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kYieldThreads, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
+    per_cu = std::min(per_cu, kPartialBlocksPerCU);
     const int64_t ntiles = (a.Nc + kYieldThreads - 1) / kYieldThreads;
     const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ntiles, int64_t(ctx->num_cu) * per_cu)));
     ctx->yield_grid = nblk;
@@ -444,22 +455,65 @@ int launch_yield(iss_ctx* ctx, const YieldArgs& a, bool raw, bool exact) {
     }
 }
 
+template <typename R, int MODE, bool EXACT>
+int launch_rows_t(iss_ctx* ctx, const YieldArgs& a) {
+    const size_t lds = yield_rows_lds_bytes<R>(ktables_for(MODE), ctx->Ns);
+    auto kern = k_yield_rows<R, MODE, EXACT>;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds)) != hipSuccess)
+        return ISS_ERR_HIP;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kRowThreads, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    per_cu = std::min(per_cu, kPartialBlocksPerCU);  // the species partials hold num_cu * 8 blocks
+    const int64_t ntiles = (a.Nc + kRowThreads - 1) / kRowThreads;
+    const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ntiles, int64_t(ctx->num_cu) * per_cu)));
+    ctx->yield_grid = nblk;
+    kern<<<nblk, kRowThreads, lds, ctx->stream>>>(a, a.spk, a.rowsp);
+    return hipGetLastError() == hipSuccess ? ISS_OK : ISS_ERR_HIP;
+}
+
+template <typename R, int MODE>
+int launch_rows_m(iss_ctx* ctx, const YieldArgs& a, bool exact) {
+    return exact ? launch_rows_t<R, MODE, true>(ctx, a) : launch_rows_t<R, MODE, false>(ctx, a);
+}
+
+template <typename R>
+int launch_rows(iss_ctx* ctx, const YieldArgs& a, bool exact) {
+    switch (a.bulk) {
+        case BULK_RTA: return launch_rows_m<R, BULK_RTA>(ctx, a, exact);
+        case BULK_CE: return launch_rows_m<R, BULK_CE>(ctx, a, exact);
+        case BULK_14MOM: return launch_rows_m<R, BULK_14MOM>(ctx, a, exact);
+        case BULK_22MOM: return launch_rows_m<R, BULK_22MOM>(ctx, a, exact);
+        default: return launch_rows_m<R, BULK_NONE>(ctx, a, exact);
+    }
+}
+
+// The yield pass.  LAYOUT_ROWS writes every (cell, species) yield in the
+// working precision (d_rows, [Nc][Ns]); LAYOUT_CDF the FP64 species CDF row
+// of every cell (d_cdf) and, with ycol, the FP64 species-major yields.
 int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_host, int lcc = 0,
-               double* ycol = nullptr) {
+               double* ycol = nullptr, int layout = LAYOUT_ROWS) {
     if (ctx->Ns == 0) return set_err(ctx, ISS_ERR_STATE, "species not set");
     if (precision != 32 && precision != 64) return set_err(ctx, ISS_ERR_ARG, "precision must be 32 or 64");
     int rc = ensure_ktab(ctx);
     if (rc) return rc;
+    if (layout != LAYOUT_ROWS && layout != LAYOUT_CDF) return set_err(ctx, ISS_ERR_ARG, "unknown yield layout");
+    const bool rowsl = layout == LAYOUT_ROWS;
+    if (rowsl && ycol) return set_err(ctx, ISS_ERR_ARG, "species-major yields come with the CDF layout");
     const int64_t Nc = ctx->Nc;
     const int Ns = ctx->Ns;
-    HIPCHK(ctx->d_cdf.ensure(std::max<int64_t>(1, Nc) * cdf_stride(Ns)));
+    const size_t esz = precision == 64 ? sizeof(double) : sizeof(float);
+    const int stride = row_stride(Ns, static_cast<int>(esz));
+    if (rowsl) HIPCHK(ctx->d_rows.ensure(std::max<int64_t>(1, Nc) * stride * esz));
+    else HIPCHK(ctx->d_cdf.ensure(std::max<int64_t>(1, Nc) * cdf_stride(Ns)));
     HIPCHK(ctx->d_ctot.ensure(std::max<int64_t>(1, Nc)));
     HIPCHK(ctx->d_stot.ensure(Ns));
-    // partials sized for the largest grid the occupancy can give
-    const int max_blk = ctx->num_cu * 8;
+    // partials sized for the largest grid the launches use
+    const int max_blk = ctx->num_cu * kPartialBlocksPerCU;
     HIPCHK(ctx->d_partial.ensure(static_cast<size_t>(max_blk) * Ns));
-    const size_t esz = precision == 64 ? sizeof(double) : sizeof(float);
-    const bool raw = per_cell_host != nullptr;
+    // per-cell yields for the caller: the rows themselves, or a staging copy
+    const bool raw = per_cell_host != nullptr && !rowsl;
     if (raw) HIPCHK(ctx->d_raw.ensure(std::max<int64_t>(1, Nc) * Ns * esz));
     YieldArgs a;
     a.cells = ctx->d_cells.p;
@@ -478,6 +532,10 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     a.spk = ctx->d_spk.p;
     a.lcc = lcc;
     a.ycol = ycol;
+    a.rows = rowsl ? static_cast<void*>(ctx->d_rows.p) : nullptr;
+    a.nlight = ctx->nlight;
+    a.rowsp = ctx->d_rowsp.p ? ctx->d_rowsp.p + (lcc ? row_stride(Ns, 4) : 0) : nullptr;
+    a.all_general = (ctx->light_beyond || ctx->charges_outside) ? 1 : 0;
     HIPCHK(ctx->d_oob.ensure(1));
     a.oob = ctx->d_oob.p;
     HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
@@ -497,7 +555,11 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     double* h_tot = ctx->h_pin.at<double>(PIN_TOT);
     for (int exact = 0; exact < 2; exact++) {
         if (Nc > 0) {
-            int rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, exact != 0)
+            int rc;
+            if (rowsl)
+                rc = (precision == 64) ? launch_rows<double>(ctx, a, exact != 0) : launch_rows<float>(ctx, a, exact != 0);
+            else
+                rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, exact != 0)
                                        : launch_yield<float>(ctx, a, raw, exact != 0);
             if (rc) return set_err(ctx, rc, "yield kernel launch failed");
             nblk = ctx->yield_grid;
@@ -512,8 +574,13 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
         if (check) HIPCHK(hipMemcpyAsync(h_oob, ctx->d_oob.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         if (totals_host)
             HIPCHK(hipMemcpyAsync(h_tot, ctx->d_stot.p, sizeof(double) * Ns, hipMemcpyDeviceToHost, ctx->stream));
-        if (raw && Nc > 0)
-            HIPCHK(hipMemcpyAsync(per_cell_host, ctx->d_raw.p, esz * Nc * Ns, hipMemcpyDeviceToHost, ctx->stream));
+        if (per_cell_host && Nc > 0) {
+            if (rowsl)  // the rows without their padding
+                HIPCHK(hipMemcpy2DAsync(per_cell_host, esz * Ns, ctx->d_rows.p, esz * stride, esz * Ns, Nc,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            else
+                HIPCHK(hipMemcpyAsync(per_cell_host, ctx->d_raw.p, esz * Nc * Ns, hipMemcpyDeviceToHost, ctx->stream));
+        }
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (totals_host) std::memcpy(totals_host, h_tot, sizeof(double) * Ns);
         if (!(check && *h_oob)) break;
@@ -525,6 +592,7 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     ctx->stats.ms_yield = ms2;
     ctx->yields_valid = true;
     ctx->yield_precision = precision;
+    ctx->yield_layout = layout;
     ctx->yields_lcc = lcc;
     return ISS_OK;
 }
@@ -805,6 +873,15 @@ int iss_set_species(iss_ctx* ctx, const iss_species_desc* sp) {
         k.sgn = static_cast<float>(sp->sign[i]);
         k.md = sp->mass[i];
         k.prefd = static_cast<double>(sp->degeneracy[i]) * uf;
+        k.m2 = static_cast<float>(sp->mass[i] * sp->mass[i]);
+        k.m3 = static_cast<float>(sp->mass[i] * sp->mass[i] * sp->mass[i]);
+        k.m4 = static_cast<float>(sp->mass[i] * sp->mass[i] * sp->mass[i] * sp->mass[i]);
+        k.g = static_cast<float>(sp->degeneracy[i]);
+        k.Bi = sp->baryon[i];
+        k.Si = sp->strange[i];
+        k.Qi = sp->charge[i];
+        k.pad = 0;
+        k.md20 = 20. * sp->mass[i];
     }
     HIPCHK(ctx->d_mass.ensure(Ns));
     HIPCHK(ctx->d_ints.ensure(6 * Ns));
@@ -814,6 +891,45 @@ int iss_set_species(iss_ctx* ctx, const iss_species_desc* sp) {
     HIPCHK(hipMemcpyAsync(ctx->d_spk.p, spk.data(), sizeof(SpeciesK) * Ns, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->Ns = Ns;
+    // the leading species below 0.7 GeV take the 10-term series (the row
+    // pass gives them lanes of their own; lighter species further down the
+    // list, if any, are still summed correctly, by the one-term lanes' loop)
+    ctx->nlight = 0;
+    while (ctx->nlight < std::min(Ns, kLightMax) && sp->mass[ctx->nlight] < 0.7) ctx->nlight++;
+    ctx->light_beyond = false;
+    for (int i = ctx->nlight; i < Ns; i++)
+        if (sp->mass[i] < 0.7) ctx->light_beyond = true;
+    // the row pass's padded one-term table: all species, then the same with
+    // the negative species suppressed (local charge conservation)
+    {
+        const int st = row_stride(Ns, 4);
+        std::vector<RowSp> rs(2 * static_cast<size_t>(st));
+        ctx->charges_outside = false;
+        for (int v = 0; v < 2; v++)
+            for (int i = 0; i < st; i++) {
+                RowSp& r = rs[static_cast<size_t>(v) * st + i];
+                const double m = i < Ns ? sp->mass[i] : 1.0;
+                const int B = i < Ns ? sp->baryon[i] : 0, S = i < Ns ? sp->strange[i] : 0,
+                          Q = i < Ns ? sp->charge[i] : 0;
+                r.md20 = 20. * m;
+                r.m = static_cast<float>(m);
+                r.m2 = static_cast<float>(m * m);
+                r.m3 = static_cast<float>(m * m * m);
+                r.m4 = static_cast<float>(m * m * m * m);
+                r.m2B = static_cast<float>(m * m * B);
+                const bool off = i >= Ns || (v == 1 && Q < 0);
+                r.g = off ? 0.f : static_cast<float>(sp->degeneracy[i]);
+                if (B < -1 || B > 1 || S < -3 || S > 3 || Q < -2 || Q > 2) {
+                    ctx->charges_outside = true;
+                    r.frow = 1 | (6 << 8) | (12 << 16);
+                } else {
+                    r.frow = (B + 1) | ((S + 6) << 8) | ((Q + 12) << 16);
+                }
+                r.sgn = i < Ns ? static_cast<float>(sp->sign[i]) : 0.f;
+            }
+        HIPCHK(ctx->d_rowsp.ensure(rs.size()));
+        HIPCHK(hipMemcpy(ctx->d_rowsp.p, rs.data(), sizeof(RowSp) * rs.size(), hipMemcpyHostToDevice));
+    }
     ctx->h_pid.assign(sp->pid, sp->pid + Ns);
     ctx->h_charge.assign(sp->charge, sp->charge + Ns);
     // antiparticle entries (local charge conservation)
@@ -988,7 +1104,10 @@ int iss_get_surface_lrf(iss_ctx* ctx, float* cells28) {
 int iss_compute_yields(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals, void* per_cell) {
     if (!ctx || !opts) return ISS_ERR_ARG;
     (void)hipSetDevice(ctx->device);
-    return run_yields(ctx, opts->precision, per_cell, species_totals);
+    // batched surfaces are sampled from FP64 species CDF rows, one surface
+    // from its yield rows
+    return run_yields(ctx, opts->precision, per_cell, species_totals, 0, nullptr,
+                      ctx->S > 0 ? LAYOUT_CDF : LAYOUT_ROWS);
 }
 
 // local charge conservation needs every positive species' antiparticle
@@ -1073,7 +1192,7 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     const int E = o->n_events;
     // yields with their species-major copy, scanned along the cells
     HIPCHK(ctx->d_ycol.ensure(std::max<int64_t>(Nc, 1) * Ns));
-    int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc, ctx->d_ycol.p);
+    int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc, ctx->d_ycol.p, LAYOUT_CDF);
     if (rc) return rc;
     const bool binv = o->hydro_mode != 2;
     const double mult = binv ? (o->y_RB - o->y_LB) : 1.0;
@@ -1177,8 +1296,8 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         int rc = lcc_check(ctx);
         if (rc) return rc;
     }
-    if (!ctx->yields_valid || ctx->yields_lcc != lcc) {
-        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc);
+    if (!ctx->yields_valid || ctx->yields_lcc != lcc || ctx->yield_layout != LAYOUT_ROWS) {
+        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, lcc, nullptr, LAYOUT_ROWS);
         if (rc) return rc;
     }
     const int64_t Nc = ctx->Nc;
@@ -1259,22 +1378,45 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.stats = ctx->d_stats.p;
     a.Kdev = nullptr;
     a.anti = ctx->d_anti.p;
+    // each slot's cell and species from its cell's yield row; the learned
+    // pass also gets each cell's species bitmap and pair count
+    const bool learned = o->proposal == 1;
+    const int Wb = (ctx->Ns + 31) / 32;
     HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(K, 1)));
+    HIPCHK(ctx->d_sp16.ensure(std::max<int64_t>(K, 1)));
+    if (learned) {
+        HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(Nc, 1) * Wb));
+        HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(Nc, 1)));
+    }
+    a.cellidx = ctx->d_cellidx.p;
+    a.slot_species = ctx->d_sp16.p;
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
     if (K > 0) {
-        k_cell_index<<<(Nc + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, Nc, 0, ctx->d_cellidx.p);  // 4 waves of 64 cells
+        const int pblk = static_cast<int>((Nc + kPickCells - 1) / kPickCells);
+        const bool f64 = ctx->yield_precision == 64;
+        const void* rows = ctx->d_rows.p;
+        uint32_t* gb = learned ? ctx->d_lbits.p : nullptr;
+        uint32_t* gc = learned ? ctx->d_lcnt.p : nullptr;
+        if (f64 && learned)
+            k_pick<double, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+        else if (f64)
+            k_pick<double, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+        else if (learned)
+            k_pick<float, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+        else
+            k_pick<float, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
         HIPCHK(hipGetLastError());
-        a.cellidx = ctx->d_cellidx.p;
     }
     int per_cu_trials = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, 256, 0) != hipSuccess ||
         per_cu_trials < 1)
         per_cu_trials = 1;
     const int per_cu_plain = ctx->blocks_per_cu[0] > 0 ? std::min(ctx->blocks_per_cu[0], per_cu_trials) : per_cu_trials;
-    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
     if (o->proposal == 0) {
         HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
         if (K > 0) {
+            HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
             k_setup<false><<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
             HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
             const int64_t want = (K + 255) / 256;
@@ -1296,19 +1438,12 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             L.fb_list = ctx->d_fb_list.p;
             L.fb_count = ctx->d_fb_count.p;
             L.stats2 = ctx->d_stats.p + 4;
-            // the pass's (cell, species) pairs: one network evaluation and
-            // band envelope per pair
-            const int Wb = (ctx->Ns + 31) / 32;
+            // the pass's (cell, species) pairs (k_pick counted them per
+            // cell): one network evaluation and band envelope per pair
             HIPCHK(ctx->d_pkey.ensure(K));
             HIPCHK(ctx->d_lslot.ensure(K));
-            HIPCHK(ctx->d_sp16.ensure(K));
-            HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(Nc, 1) * Wb));
-            HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(Nc, 1)));
             HIPCHK(ctx->d_pairoff.ensure(Nc + 1));
             const int pblk = static_cast<int>((Nc + kPairCellsPerBlock - 1) / kPairCellsPerBlock);
-            k_lpair_pick<<<pblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, ctx->d_lbits.p, ctx->d_lcnt.p,
-                                                                            ctx->d_sp16.p);
-            HIPCHK(hipGetLastError());
             rc = launch_scan(ctx, ctx->d_lcnt.p, Nc, ctx->d_pairoff.p);
             if (rc) return rc;
             k_lpair_fill<<<pblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, Wb, ctx->d_lbits.p, ctx->d_pairoff.p,
@@ -1363,8 +1498,9 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
     HIPCHK(hipEventElapsedTime(&ms_k, ctx->ev[4], ctx->ev[5]));
     float ms_su = 0.f, ms_tr = 0.f;
-    // the set-up kernel alone (the learned pass's pair passes run before it)
-    HIPCHK(hipEventElapsedTime(&ms_su, o->proposal == 1 && K > 0 ? ctx->ev[7] : ctx->ev[4], ctx->ev[6]));
+    // the set-up kernel alone (the species pick and the learned pass's pair
+    // passes run before it)
+    HIPCHK(hipEventElapsedTime(&ms_su, K > 0 ? ctx->ev[7] : ctx->ev[4], K > 0 ? ctx->ev[6] : ctx->ev[4]));
     HIPCHK(hipEventElapsedTime(&ms_tr, ctx->ev[6], ctx->ev[5]));
     ctx->stats.ms_setup_kernel = ms_su;
     ctx->stats.ms_trials_kernel = ms_tr;
@@ -1764,9 +1900,9 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned;
     if (R > 4096) return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most 4096");
     invalidate_output(ctx);
-    if (!ctx->yields_valid || ctx->yields_lcc ||
+    if (!ctx->yields_valid || ctx->yields_lcc || ctx->yield_layout != LAYOUT_CDF ||
         (o->precision != ctx->yield_precision && (o->precision == 32 || o->precision == 64))) {
-        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr);
+        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, 0, nullptr, LAYOUT_CDF);
         if (rc) return rc;
     }
     const int64_t N = ctx->Nc;

@@ -60,7 +60,25 @@ struct SpeciesDev {
 struct SpeciesK {
     float m, pref, B, S, Q, sgn;
     double md, prefd;
+    // the one-term yields of the row pass: m^2, m^3, m^4, degeneracy, the
+    // charges as integers, 20 m (FP64: places m / T in the K table)
+    float m2, m3, m4, g;
+    int Bi, Si, Qi, pad;
+    double md20;
 };
+
+// Species constants of the yield-row pass (padded to whole 64-byte segments
+// of the row; padding and suppressed species have g = 0): 20 m in FP64, m,
+// m^2, m^3, m^4, m^2 B, g, the LDS rows of the fugacity factors e^{beta mu q}
+// of its baryon number, strangeness and charge, and the quantum-statistics
+// sign (+1 fermions, -1 bosons)
+struct RowSp {
+    double md20;
+    float m, m2, m3, m4, m2B, g;
+    int frow;   // row of e^{beta mu_B B} | row of e^{beta mu_S S} << 8 | row of e^{beta mu_Q Q} << 16
+    float sgn;
+};
+static_assert(sizeof(RowSp) == 40 && alignof(RowSp) == 8, "RowSp is copied to LDS as 10 words");
 
 struct PTabDev {
     // per table: first E~, step, length, m0, truncation, fermion flag
@@ -89,6 +107,11 @@ struct YieldArgs {
     const SpeciesK* spk;  // [Ns] species constants of the yield pass
     int lcc;              // local charge conservation: negative species get no yield
     double* ycol;         // optional [Ns][Nc] FP64 yields, species-major (multiplicity models)
+    void* rows;           // k_yield_rows: [Nc][row stride] yields in working precision, cell-major
+    const struct RowSp* rowsp;  // k_yield_rows: species constants, padded (zero weight beyond Ns)
+    int nlight;           // k_yield_rows: leading species below 0.7 GeV (the 10-term series)
+    int all_general;      // k_yield_rows: a species below 0.7 GeV after the leading ones (at most 16),
+                          // or a charge outside the fugacity rows: every species takes the general series
 };
 
 struct SampleArgs {
@@ -129,8 +152,10 @@ struct SampleArgs {
     // local charge conservation: the antiparticle's species index ([Ns], -1
     // none); partner records carry kPartnerBit in j
     const int* anti;
-    // the cell of every slot of the pass (k_cell_index); null: search the offsets
+    // the cell of every slot of the pass (k_cell_index or k_pick); null: search the offsets
     const uint32_t* cellidx;
+    // the species of every slot (k_pick); null: drawn from the FP64 species CDF rows
+    const uint16_t* slot_species;
 };
 
 constexpr uint32_t kPartnerBit = 0x80000000u;

@@ -303,9 +303,6 @@ __device__ __forceinline__ f32x16 zero16() {
     for (int i = 0; i < 16; i++) z[i] = 0.f;
     return z;
 }
-// LDS traffic between lanes of one wave: wait for this wave's LDS operations
-// and keep the compiler from moving memory accesses across
-__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // Tiles are 32 x 32 in the layout of v_mfma_f32_32x32x16_bf16: operand A
 // (rows m0.., 16 k per step) comes from lane l as row l%32, k = 8 (l/32)..+7;
@@ -796,51 +793,13 @@ static_assert(sizeof(LSlot) == 16, "LSlot layout");
 
 // The pairs of a pass.  One hadron in 1.5 shares its (cell, species) with
 // another in the 200k-cell workload, so the network and the band envelope are
-// evaluated once per pair.  One wave per cell, twice:
-//   k_lpair_pick   the cell's FP64 species CDF row goes to LDS, each slot
-//                  draws its species there (the same draw and search as
-//                  k_setup); the cell's species bitmap and its number of
-//                  pairs are stored
+// evaluated once per pair.  k_pick (the species of every slot) also leaves each
+// cell's species bitmap and its number of pairs; then
 //   scan           the cells' first pair indices
 //   k_lpair_fill   the compact pair list (cell, species) in bitmap order and
-//                  each slot's record with its pair
+//                  each slot's record with its pair (one wave per cell)
 constexpr int kPairCellsPerBlock = 4;
 constexpr int kPairWords = kMaxSpecies / 32;
-
-__global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_pick(SampleArgs a, uint32_t* __restrict__ gbits,
-                                                                          uint32_t* __restrict__ cnt,
-                                                                          uint16_t* __restrict__ sp16) {
-    __shared__ double s_row[kPairCellsPerBlock][kMaxSpecies];
-    __shared__ uint32_t s_bits[kPairCellsPerBlock][kPairWords];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kPairCellsPerBlock + w;
-    if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
-    const int Ns = a.Ns, Wb = (Ns + 31) / 32;
-    const int64_t o0 = a.off[c], n = a.off[c + 1] - o0;
-    uint32_t* bits = s_bits[w];
-    if (n == 0) {
-        if (lane < Wb) gbits[c * Wb + lane] = 0u;
-        if (lane == 0) cnt[c] = 0u;
-        return;
-    }
-    double* row = s_row[w];
-    for (int s = lane; s < Ns; s += 64) row[s] = a.cdf[c * cdf_stride(Ns) + s];
-    if (lane < Wb) bits[lane] = 0u;
-    const double tot = a.ctot[c];
-    wave_lds_sync();
-    for (int64_t r = lane; r < n; r += 64) {
-        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, a.chunk));
-        int s = upper_bound(row, 0, Ns, st.uniform() * tot);
-        if (s >= Ns) s = Ns - 1;
-        sp16[o0 + r] = static_cast<uint16_t>(s);
-        atomicOr(&bits[s >> 5], 1u << (s & 31));
-    }
-    wave_lds_sync();
-    const uint32_t wd = lane < Wb ? bits[lane] : 0u;
-    if (lane < Wb) gbits[c * Wb + lane] = wd;
-    const uint32_t np = wave_sum(static_cast<uint32_t>(__popc(wd)));
-    if (lane == 0) cnt[c] = np;
-}
 
 __global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_fill(SampleArgs a, int Wb,
                                                                           const uint32_t* __restrict__ gbits,

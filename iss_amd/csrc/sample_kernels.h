@@ -327,6 +327,89 @@ __global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ 
     }
 }
 
+// Species of every hadron slot of a pass, one wave per cell.  By Poisson
+// splitting a hadron of cell c is of species s with probability
+// N_{c,s} / N_c.  The cell's yield row (k_yield_rows, working precision)
+// becomes its FP64 inclusive species prefix in LDS: a wave scan per chunk of
+// 64 species, made non-decreasing by a running maximum (the scan's tree of
+// roundings could otherwise step down where a yield is zero), plus the
+// previous chunks' total.  Slot j draws u from its stream (c, j, TAG_SETUP)
+// and takes the first species whose prefix exceeds u times the cell total; a
+// species of zero yield is never taken (the nearest species of positive
+// yield replaces one that the roundings of the scan leave an interval of
+// measure ~1e-16).  Writes each slot's species and cell; with PAIRS also the
+// cell's species bitmap and its number of distinct species (the learned
+// sampler's (cell, species) pairs).  Launch with ceil(Nc / kPickCells)
+// workgroups of kPickCells waves.
+constexpr int kPickCells = 4;
+constexpr int kPickWords = kMaxSpecies / 32;
+
+template <typename R, bool PAIRS>
+__global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const R* __restrict__ rows, int stride,
+                                                             uint16_t* __restrict__ sp16, uint32_t* __restrict__ cellidx,
+                                                             uint32_t* __restrict__ gbits, uint32_t* __restrict__ cnt) {
+    __shared__ double s_pre[kPickCells][kMaxSpecies];
+    __shared__ uint32_t s_pos[kPickCells][kPickWords];
+    __shared__ uint32_t s_bits[kPickCells][kPickWords];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
+    if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
+    const int Ns = a.Ns, Wb = (Ns + 31) / 32;
+    const int64_t o0 = a.off[c], n = a.off[c + 1] - o0;
+    if (n == 0) {
+        if (PAIRS) {
+            if (lane < Wb) gbits[c * Wb + lane] = 0u;
+            if (lane == 0) cnt[c] = 0u;
+        }
+        return;
+    }
+    double* pre = s_pre[w];
+    uint32_t* pos = s_pos[w];
+    uint32_t* bits = s_bits[w];
+    if (PAIRS && lane < Wb) bits[lane] = 0u;
+    const R* row = rows + c * stride;
+    double carry = 0.;
+    for (int s0 = 0; s0 < Ns; s0 += kWave) {
+        const int s = s0 + lane;
+        const double y = s < Ns ? static_cast<double>(row[s]) : 0.;
+        const double incl = wave_inclusive_max(wave_inclusive_scan(y)) + carry;
+        if (s < Ns) pre[s] = incl;
+        const unsigned long long m = __ballot(y > 0.);
+        if (lane == 0) {
+            pos[s0 >> 5] = static_cast<uint32_t>(m);
+            if ((s0 >> 5) + 1 < Wb) pos[(s0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
+        }
+        carry = __shfl(incl, kWave - 1, kWave);
+    }
+    wave_lds_sync();
+    const double tot = carry;
+    auto positive = [&](int s) { return (pos[s >> 5] >> (s & 31)) & 1u; };
+    for (int64_t r = lane; r < n; r += kWave) {
+        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, a.chunk));
+        int sp = upper_bound(pre, 0, Ns, st.uniform() * tot);
+        if (sp >= Ns) sp = Ns - 1;
+        if (!positive(sp)) {
+            int t = sp + 1;
+            while (t < Ns && !positive(t)) t++;
+            if (t >= Ns) {
+                t = sp - 1;
+                while (t > 0 && !positive(t)) t--;
+            }
+            sp = max(t, 0);
+        }
+        sp16[o0 + r] = static_cast<uint16_t>(sp);
+        cellidx[o0 + r] = static_cast<uint32_t>(c);
+        if (PAIRS) atomicOr(&bits[sp >> 5], 1u << (sp & 31));
+    }
+    if (PAIRS) {
+        wave_lds_sync();
+        const uint32_t wd = lane < Wb ? bits[lane] : 0u;
+        if (lane < Wb) gbits[c * Wb + lane] = wd;
+        const uint32_t np = wave_sum(static_cast<uint32_t>(__popc(wd)));
+        if (lane == 0) cnt[c] = np;
+    }
+}
+
 // the species-dependent part of a set-up record: the |p| table, its first
 // row and the proposal's normalisation at E~ = m~ - mu~
 __device__ __forceinline__ HSetup setup_record(const SampleArgs& a, int64_t c, uint32_t jj, int s) {
@@ -375,7 +458,9 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
     const int64_t c = a.cellidx ? static_cast<int64_t>(a.cellidx[i]) : upper_bound(a.off + 1, int64_t(0), a.Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
     int s;
-    {
+    if (a.slot_species) {
+        s = a.slot_species[i];
+    } else {
         Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, chunk));
         const double* row = a.cdf + c * cdf_stride(a.Ns);
         s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
@@ -634,10 +719,15 @@ __global__ void k_lcc_flag(SampleArgs a, uint32_t* __restrict__ pflag, int64_t* 
     if (i >= a.K) return;
     const int64_t c = a.cellidx ? static_cast<int64_t>(a.cellidx[i]) : upper_bound(a.off + 1, int64_t(0), a.Nc, i);
     const uint32_t jj = static_cast<uint32_t>(i - a.off[c]);
-    Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
-    const double* row = a.cdf + c * cdf_stride(a.Ns);
-    int s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
-    if (s >= a.Ns) s = a.Ns - 1;
+    int s;
+    if (a.slot_species) {
+        s = a.slot_species[i];
+    } else {
+        Stream st(a.seed, static_cast<uint32_t>(c), jj, tag_word(TAG_SETUP, a.chunk));
+        const double* row = a.cdf + c * cdf_stride(a.Ns);
+        s = upper_bound(row, 0, a.Ns, st.uniform() * a.ctot[c]);
+        if (s >= a.Ns) s = a.Ns - 1;
+    }
     const uint64_t o = a.perm(static_cast<uint64_t>(i));
     pflag[o] = (a.sp.Q[s] > 0) ? 1u : 0u;
     src[o] = (i << 10) | s;

@@ -764,11 +764,13 @@ bool FSSWOracle::setup(const std::string& dir, const Config& c, std::string* err
     }
     if (neos_kind() == 1) {
         if (!dft.load_neos(dir + "/deltaf_tables" + sub + "/NEoSBQS_CE_deltafCoeff.dat", 5)) {
-            if (err) *err = "CE table missing"; return false;
+            if (err) *err = "CE table missing";
+            return false;
         }
     } else if (neos_kind() == 0) {
         if (!dft.load_neos(dir + "/deltaf_tables" + sub + "/NEoSBQS_22mom_deltafCoeff.dat", 8)) {
-            if (err) *err = "22-moment table missing"; return false;
+            if (err) *err = "22-moment table missing";
+            return false;
         }
     }
     sf.init(cfg.include_bulk == 1, cfg.include_diffusion == 1);

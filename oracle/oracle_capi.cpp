@@ -1,5 +1,6 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY.  extern "C" surface of the CPU oracle for
 // the tests and the cpu_baseline leg of bench.py (loaded with ctypes).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <thread>
@@ -168,6 +169,30 @@ long orc_sample(void* p, unsigned seed, int nev) {
     long tot = 0;
     for (auto& e : o.events) tot += static_cast<long>(e.size());
     return tot;
+}
+
+// The same sampler on `nthreads` host threads (the CPU baseline on all
+// cores): thread t samples its share of the nev events with its own copy of
+// the oracle and its own seeds, as separate runs of the single-threaded
+// reference would.  Returns the total number of hadrons; the handle's own
+// event list is left untouched.
+long orc_sample_mt(void* p, unsigned seed, int nev, int nthreads) {
+    const FSSWOracle& o = static_cast<Handle*>(p)->o;
+    nthreads = std::max(1, std::min(nthreads, nev));
+    std::vector<FSSWOracle> copies(nthreads, o);
+    std::vector<long> tot(nthreads, 0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) {
+        const int n_t = nev / nthreads + (t < nev % nthreads ? 1 : 0);
+        th.emplace_back([&, t, n_t]() {
+            copies[t].sample(seed + 7919u * static_cast<unsigned>(t), n_t);
+            for (auto& e : copies[t].events) tot[t] += static_cast<long>(e.size());
+        });
+    }
+    for (auto& x : th) x.join();
+    long all = 0;
+    for (long v : tot) all += v;
+    return all;
 }
 
 long orc_num_failed(void* p) { return static_cast<Handle*>(p)->o.n_failed; }

@@ -39,6 +39,7 @@ def lib():
             "orc_yields": (None, [v, v]),
             "orc_time_yields": (d, [v, L, L, i]),
             "orc_sample": (L, [v, C.c_uint, i]),
+            "orc_sample_mt": (L, [v, C.c_uint, i, i]),
             "orc_num_failed": (L, [v]),
             "orc_event_counts": (None, [v, v]),
             "orc_hadrons": (None, [v, v]),

@@ -207,16 +207,17 @@ def test_c5_fp32_yields_within_1e5_of_fp64_everywhere(c5):
     t64, p64 = ctx.compute_yields(precision=64, per_cell=True)
     t32, p32 = ctx.compute_yields(precision=32, per_cell=True)
     relt = np.abs(t32 - t64) / np.maximum(t64, 1e-300)
-    worst, zeros_ok = 0.0, True
+    worst, where, zeros_ok = 0.0, None, True
     for a in range(0, p64.shape[0], 250_000):
         b64 = p64[a:a + 250_000]
         b32 = p32[a:a + 250_000].astype(np.float64)
-        big = b64 > 1e-25
-        if big.any():
-            worst = max(worst, float((np.abs(b32[big] - b64[big]) / b64[big]).max()))
+        rel = np.where(b64 > 1e-25, np.abs(b32 - b64) / np.maximum(b64, 1e-300), 0.0)
+        k = int(np.argmax(rel))
+        if rel.flat[k] > worst:
+            worst, where = float(rel.flat[k]), (a + k // rel.shape[1], k % rel.shape[1])
         zeros_ok &= bool(np.all(b32[b64 == 0] == 0))
-    print("C5 FP32 vs FP64: max rel per (cell, species) %.3g, species totals %.3g, hadrons/event %.1f"
-          % (worst, relt.max(), t64.sum()))
+    print("C5 FP32 vs FP64: max rel per (cell, species) %.3g at (cell, species) %s, species totals %.3g, "
+          "hadrons/event %.1f" % (worst, where, relt.max(), t64.sum()))
     assert worst < 1e-5 and relt.max() < 1e-5 and zeros_ok
     assert 5e3 < t64.sum() < 2e4
 
