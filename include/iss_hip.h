@@ -164,6 +164,21 @@ int iss_get_surface_lrf(iss_ctx* ctx, float* cells28);
 int iss_compute_yields(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals,
                        void* per_cell);
 
+/* ---- the same yields with no allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops).  Both outputs are DEVICE pointers:
+ * species_totals [n_species] FP64; per_cell [n_cells][stride] in the working
+ * precision, stride = n_species rounded up to a multiple of 16 (FP32) or
+ * 8 (FP64), entries past n_species zero, 16-byte aligned, holding at least
+ * per_cell_capacity elements.  Values equal iss_compute_yields' bit for bit.
+ * Scratch (species partials, the out-of-table flag) is a static 4 MiB array
+ * per GPU: at most 2048 species; calls that overlap on one GPU race on it.
+ * Needs the Bessel tables (built by the context's first iss_compute_yields
+ * or iss_sample); diffusion delta f is refused (ISS_ERR_UNSUPPORTED).  The
+ * context's own yields and sampling state are left unchanged. ---- */
+int iss_compute_yields_noalloc(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals,
+                               void* per_cell, int64_t per_cell_capacity);
+
 /* ---- sampling: sample_using_dN_dxtdy_4all_particles_conventional
  * (src/FSSW.cpp:873-1071) with sample_momemtum_from_a_fluid_cell,
  * the boost and add_one_sampled_particle (src/FSSW.cpp:1852-2010). ---- */

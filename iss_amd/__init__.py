@@ -144,6 +144,8 @@ _SIGNATURES = {
     "iss_num_cells": (C.c_int64, [C.c_void_p]),
     "iss_get_surface_lrf": (C.c_int, [C.c_void_p, C.c_void_p]),
     "iss_compute_yields": (C.c_int, [C.c_void_p, C.POINTER(_YieldOpts), C.c_void_p, C.c_void_p]),
+    "iss_compute_yields_noalloc": (C.c_int, [C.c_void_p, C.POINTER(_YieldOpts), C.c_void_p, C.c_void_p,
+                                             C.c_int64]),
     "iss_sample": (C.c_int, [C.c_void_p, C.POINTER(_SampleOpts), C.POINTER(C.c_int64)]),
     "iss_get_event_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "iss_get_hadrons": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
@@ -297,6 +299,21 @@ class Context:
         self._check(self.lib.iss_compute_yields(self.h, C.byref(o), _ptr(tot), _ptr(pc)),
                     "iss_compute_yields")
         return (tot, pc) if per_cell else tot
+
+    def compute_yields_noalloc(self, species_totals_dev: int, per_cell_dev: int, capacity: int, precision=32,
+                               hydro_mode=2, y_LB=-5.0, y_RB=5.0) -> int:
+        """iss_compute_yields_noalloc into caller DEVICE buffers (addresses as
+        integers): species totals [n_species] FP64, per-cell rows
+        [n_cells][row_stride(precision)].  Returns the status code."""
+        o = _YieldOpts(precision, hydro_mode, y_LB, y_RB)
+        return self.lib.iss_compute_yields_noalloc(self.h, C.byref(o), C.c_void_p(species_totals_dev),
+                                                   C.c_void_p(per_cell_dev), capacity)
+
+    def row_stride(self, precision=32) -> int:
+        """Row length of the allocation-free yields: the species count rounded
+        up to 16 (FP32) or 8 (FP64)."""
+        q = 16 if precision == 32 else 8
+        return (self.n_species + q - 1) // q * q
 
     # -------------------------------------------------------------- sampling
     def sample(self, n_events, seed=1, event0=0, hydro_mode=2, y_LB=-5.0, y_RB=5.0,

@@ -28,6 +28,7 @@
 #include <memory>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,9 +45,44 @@ using namespace issdev;
 #include "batch_kernels.h"
 #include "decay_kernels.h"
 
+// Scratch of the allocation-free yield entry point: a 256-byte header (the
+// out-of-table flag in its first word), then the species partials of up to
+// kNoAllocPartials (block, species) pairs: 4 MiB in all.  Each GPU holds its
+// own copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+constexpr int kNoAllocHeader = 32;         // doubles
+constexpr int kNoAllocPartials = 1 << 19;  // doubles
+constexpr int kNoAllocMaxSpecies = kNoAllocPartials / 256;  // keeps >= 256 blocks (one per CU)
+static __device__ __align__(256) double scratch[kNoAllocHeader + kNoAllocPartials];
+
 // =================================================================== host
 
 namespace {
+
+// the device address of `scratch`, looked up once per device (with that
+// device current); null with the error text if the lookup failed
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+std::string g_scratch_err[kMaxDevices];
+
+double* scratch_address(int device, std::string* err) {
+    if (device < 0 || device >= kMaxDevices) {
+        *err = "device index beyond the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device] && g_scratch_err[device].empty()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        else
+            g_scratch_addr[device] = static_cast<double*>(p);
+    }
+    *err = g_scratch_err[device];
+    return g_scratch_addr[device];
+}
 
 #define HIPCHK(expr)                                                                  \
     do {                                                                              \
@@ -130,6 +166,8 @@ struct DecayGen {
 struct iss_ctx {
     PinBuf h_pin;  // pinned read-back slots (PIN_*)
     int device = 0;
+    double* scratch = nullptr;  // this device's `scratch` (allocation-free entry point)
+    std::string scratch_err;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     std::string err;
@@ -456,7 +494,7 @@ int launch_yield(iss_ctx* ctx, const YieldArgs& a, bool raw, bool exact) {
 }
 
 template <typename R, int MODE, bool EXACT>
-int launch_rows_t(iss_ctx* ctx, const YieldArgs& a) {
+int launch_rows_t(iss_ctx* ctx, const YieldArgs& a, int max_blocks, int force_blocks) {
     const size_t lds = yield_rows_lds_bytes<R>(ktables_for(MODE), ctx->Ns);
     auto kern = k_yield_rows<R, MODE, EXACT>;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -467,26 +505,53 @@ int launch_rows_t(iss_ctx* ctx, const YieldArgs& a) {
         per_cu = 1;
     per_cu = std::min(per_cu, kPartialBlocksPerCU);  // the species partials hold num_cu * 8 blocks
     const int64_t ntiles = (a.Nc + kRowThreads - 1) / kRowThreads;
-    const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ntiles, int64_t(ctx->num_cu) * per_cu)));
+    // (a forced count: the same grid as an earlier pass, whose partials a
+    // skipped pass leaves in place)
+    const int nblk = force_blocks > 0 ? force_blocks
+                                      : static_cast<int>(std::max<int64_t>(
+                                            1, std::min<int64_t>({ntiles, int64_t(ctx->num_cu) * per_cu,
+                                                                  int64_t(max_blocks)})));
     ctx->yield_grid = nblk;
     kern<<<nblk, kRowThreads, lds, ctx->stream>>>(a, a.spk, a.rowsp);
     return hipGetLastError() == hipSuccess ? ISS_OK : ISS_ERR_HIP;
 }
 
 template <typename R, int MODE>
-int launch_rows_m(iss_ctx* ctx, const YieldArgs& a, bool exact) {
-    return exact ? launch_rows_t<R, MODE, true>(ctx, a) : launch_rows_t<R, MODE, false>(ctx, a);
+int launch_rows_m(iss_ctx* ctx, const YieldArgs& a, bool exact, int max_blocks, int force_blocks) {
+    return exact ? launch_rows_t<R, MODE, true>(ctx, a, max_blocks, force_blocks)
+                 : launch_rows_t<R, MODE, false>(ctx, a, max_blocks, force_blocks);
 }
 
 template <typename R>
-int launch_rows(iss_ctx* ctx, const YieldArgs& a, bool exact) {
+int launch_rows(iss_ctx* ctx, const YieldArgs& a, bool exact, int max_blocks = 1 << 30, int force_blocks = 0) {
     switch (a.bulk) {
-        case BULK_RTA: return launch_rows_m<R, BULK_RTA>(ctx, a, exact);
-        case BULK_CE: return launch_rows_m<R, BULK_CE>(ctx, a, exact);
-        case BULK_14MOM: return launch_rows_m<R, BULK_14MOM>(ctx, a, exact);
-        case BULK_22MOM: return launch_rows_m<R, BULK_22MOM>(ctx, a, exact);
-        default: return launch_rows_m<R, BULK_NONE>(ctx, a, exact);
+        case BULK_RTA: return launch_rows_m<R, BULK_RTA>(ctx, a, exact, max_blocks, force_blocks);
+        case BULK_CE: return launch_rows_m<R, BULK_CE>(ctx, a, exact, max_blocks, force_blocks);
+        case BULK_14MOM: return launch_rows_m<R, BULK_14MOM>(ctx, a, exact, max_blocks, force_blocks);
+        case BULK_22MOM: return launch_rows_m<R, BULK_22MOM>(ctx, a, exact, max_blocks, force_blocks);
+        default: return launch_rows_m<R, BULK_NONE>(ctx, a, exact, max_blocks, force_blocks);
     }
+}
+
+// The yield-pass arguments every layout shares (the resident surface,
+// species and tables); the output and scratch pointers are left null.
+YieldArgs base_yield_args(iss_ctx* ctx, int lcc) {
+    YieldArgs a{};
+    const int Ns = ctx->Ns;
+    a.cells = ctx->d_cells.p;
+    a.coef = ctx->d_coef.p;
+    a.Nc = ctx->Nc;
+    a.Ns = Ns;
+    a.sp = species_dev(ctx);
+    a.ktab = ctx->d_ktab.p;
+    a.ktabf = ctx->d_ktabf.p;
+    a.bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
+    a.spk = ctx->d_spk.p;
+    a.lcc = lcc;
+    a.nlight = ctx->nlight;
+    a.rowsp = ctx->d_rowsp.p ? ctx->d_rowsp.p + (lcc ? row_stride(Ns, 4) : 0) : nullptr;
+    a.all_general = (ctx->light_beyond || ctx->charges_outside) ? 1 : 0;
+    return a;
 }
 
 // The yield pass.  LAYOUT_ROWS writes every (cell, species) yield in the
@@ -515,27 +580,13 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     // per-cell yields for the caller: the rows themselves, or a staging copy
     const bool raw = per_cell_host != nullptr && !rowsl;
     if (raw) HIPCHK(ctx->d_raw.ensure(std::max<int64_t>(1, Nc) * Ns * esz));
-    YieldArgs a;
-    a.cells = ctx->d_cells.p;
-    a.coef = ctx->d_coef.p;
-    a.Nc = Nc;
-    a.Ns = Ns;
-    a.sp = species_dev(ctx);
-    a.ktab = ctx->d_ktab.p;
-    a.ktabf = ctx->d_ktabf.p;
-    a.bulk = ctx->df.include_bulk ? ctx->bulk_mode : BULK_NONE;
+    YieldArgs a = base_yield_args(ctx, lcc);
     a.cdf = ctx->d_cdf.p;
     a.ctot = ctx->d_ctot.p;
     a.raw = raw ? static_cast<void*>(ctx->d_raw.p) : nullptr;
     a.partial = ctx->d_partial.p;
-    a.dnq = nullptr;
-    a.spk = ctx->d_spk.p;
-    a.lcc = lcc;
     a.ycol = ycol;
     a.rows = rowsl ? static_cast<void*>(ctx->d_rows.p) : nullptr;
-    a.nlight = ctx->nlight;
-    a.rowsp = ctx->d_rowsp.p ? ctx->d_rowsp.p + (lcc ? row_stride(Ns, 4) : 0) : nullptr;
-    a.all_general = (ctx->light_beyond || ctx->charges_outside) ? 1 : 0;
     HIPCHK(ctx->d_oob.ensure(1));
     a.oob = ctx->d_oob.p;
     HIPCHK(hipMemsetAsync(ctx->d_oob.p, 0, sizeof(int), ctx->stream));
@@ -556,8 +607,13 @@ int run_yields(iss_ctx* ctx, int precision, void* per_cell_host, double* totals_
     for (int exact = 0; exact < 2; exact++) {
         if (Nc > 0) {
             int rc;
+            // rows: at most 2^19 / Ns blocks, the exact-K rerun on the first
+            // pass's grid (the grid fixes the order of the species sums; the
+            // allocation-free entry point uses the same)
+            const int max_rows_blk = kNoAllocPartials / Ns, force = exact ? nblk : 0;
             if (rowsl)
-                rc = (precision == 64) ? launch_rows<double>(ctx, a, exact != 0) : launch_rows<float>(ctx, a, exact != 0);
+                rc = (precision == 64) ? launch_rows<double>(ctx, a, exact != 0, max_rows_blk, force)
+                                       : launch_rows<float>(ctx, a, exact != 0, max_rows_blk, force);
             else
                 rc = (precision == 64) ? launch_yield<double>(ctx, a, raw, exact != 0)
                                        : launch_yield<float>(ctx, a, raw, exact != 0);
@@ -795,6 +851,7 @@ int iss_ctx_create(int device, iss_ctx** out) {
         return ISS_ERR_HIP;
     }
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
+    ctx->scratch = scratch_address(device, &ctx->scratch_err);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
     if (ctx->d_stats.ensure(8) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
@@ -1108,6 +1165,58 @@ int iss_compute_yields(iss_ctx* ctx, const iss_yield_opts* opts, double* species
     // from its yield rows
     return run_yields(ctx, opts->precision, per_cell, species_totals, 0, nullptr,
                       ctx->S > 0 ? LAYOUT_CDF : LAYOUT_ROWS);
+}
+
+int iss_compute_yields_noalloc(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals, void* per_cell,
+                               int64_t per_cell_capacity) {
+    if (!ctx || !opts || !species_totals || !per_cell) return ISS_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->scratch) return set_err(ctx, ISS_ERR_HIP, "no scratch on this device: " + ctx->scratch_err);
+    if (ctx->Ns == 0) return set_err(ctx, ISS_ERR_STATE, "species not set");
+    const int precision = opts->precision;
+    if (precision != 32 && precision != 64) return set_err(ctx, ISS_ERR_ARG, "precision must be 32 or 64");
+    if (ctx->ktab_bulk != 1)
+        return set_err(ctx, ISS_ERR_STATE,
+                       "Bessel tables not built: the first iss_compute_yields or iss_sample of the context builds them");
+    if (ctx->df.include_diffusion)
+        return set_err(ctx, ISS_ERR_UNSUPPORTED,
+                       "diffusion needs a per-(cell, species) buffer: use iss_compute_yields");
+    const int Ns = ctx->Ns;
+    const int64_t Nc = ctx->Nc;
+    if (Ns > kNoAllocMaxSpecies)
+        return set_err(ctx, ISS_ERR_RANGE,
+                       "more than " + std::to_string(kNoAllocMaxSpecies) + " species: beyond the static scratch");
+    const int esz = precision == 64 ? 8 : 4;
+    const int stride = row_stride(Ns, esz);
+    if (per_cell_capacity < Nc * stride)
+        return set_err(ctx, ISS_ERR_RANGE,
+                       "per_cell holds fewer than n_cells x " + std::to_string(stride) + " entries");
+    if (reinterpret_cast<uintptr_t>(per_cell) % 16 != 0)
+        return set_err(ctx, ISS_ERR_ARG, "per_cell must be 16-byte aligned");
+    int* flag = reinterpret_cast<int*>(ctx->scratch);
+    double* partial = ctx->scratch + kNoAllocHeader;
+    YieldArgs a = base_yield_args(ctx, 0);
+    a.rows = per_cell;
+    a.partial = partial;
+    a.oob = flag;
+    k_clear_flag<<<1, 64, 0, ctx->stream>>>(flag);
+    HIPCHK(hipGetLastError());
+    int nblk = 0;
+    if (Nc > 0) {
+        // the first pass, then the exact-K pass on the same grid, which
+        // returns at once on the device unless the first pass set the flag
+        for (int exact = 0; exact < 2; exact++) {
+            if (exact) a.gate = flag;
+            const int rc = precision == 64 ? launch_rows<double>(ctx, a, exact != 0, kNoAllocPartials / Ns, nblk)
+                                           : launch_rows<float>(ctx, a, exact != 0, kNoAllocPartials / Ns, nblk);
+            if (rc) return set_err(ctx, rc, "yield kernel launch failed");
+            nblk = ctx->yield_grid;
+        }
+    }
+    k_species_reduce<<<Ns, 256, 0, ctx->stream>>>(partial, nblk, Ns, species_totals);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ISS_OK;
 }
 
 // local charge conservation needs every positive species' antiparticle

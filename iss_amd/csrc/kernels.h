@@ -103,7 +103,7 @@ struct YieldArgs {
     void* raw;       // optional per-(cell, species) yields in working precision
     double* partial; // [gridDim.x][Ns]
     int* oob;        // set when a Bessel argument leaves the table
-    const double* dnq;   // [Ns][Nc] diffusion corrections (k_diffusion_dn); null: none
+    const double* dnq;   // [Nc][Ns] diffusion corrections (k_diffusion_dn); null: none
     const SpeciesK* spk;  // [Ns] species constants of the yield pass
     int lcc;              // local charge conservation: negative species get no yield
     double* ycol;         // optional [Ns][Nc] FP64 yields, species-major (multiplicity models)
@@ -112,6 +112,8 @@ struct YieldArgs {
     int nlight;           // k_yield_rows: leading species below 0.7 GeV (the 10-term series)
     int all_general;      // k_yield_rows: a species below 0.7 GeV after the leading ones (at most 16),
                           // or a charge outside the fugacity rows: every species takes the general series
+    const int* gate;      // k_yield_rows, exact-K pass: if set, the pass runs only where *gate != 0
+                          // (the first pass's out-of-table flag, read on the device)
 };
 
 struct SampleArgs {

@@ -422,6 +422,12 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const Spec
 }
 
 // FP64 per-species totals: one workgroup per species, fixed summation order
+// an int flag cleared by a kernel (a memset node would not be safe in a
+// captured graph)
+__global__ void k_clear_flag(int* flag) {
+    if (threadIdx.x == 0) *flag = 0;
+}
+
 __global__ void k_species_reduce(const double* __restrict__ partial, int nblk, int Ns,
                                  double* __restrict__ out) {
     __shared__ double red[256];
@@ -753,6 +759,9 @@ __host__ __device__ constexpr size_t yield_rows_lds_bytes(int ntab, int ns) {
 template <typename R, int MODE, bool EXACT>
 __global__ __launch_bounds__(kRowThreads) void k_yield_rows(YieldArgs a, const SpeciesK* __restrict__ spk,
                                                             const RowSp* __restrict__ rowsp) {
+    // the exact-K pass after a first pass that stayed inside the staged rows:
+    // nothing to redo (uniform over the grid; the flag only ever turns on)
+    if (EXACT && a.gate && *a.gate == 0) return;
     extern __shared__ __align__(16) unsigned char smem[];
     using KT = typename RowK<R>::T;
     constexpr int NT = ktables_for(MODE);
@@ -1016,7 +1025,7 @@ __global__ __launch_bounds__(kRowThreads) void k_yield_rows(YieldArgs a, const S
             if (__any(bad && valid) && lane == 0) *a.oob = 1;
         }
         if (!fast) one_by_one();
-        if (valid) a.ctot[c] = ctot;
+        if (valid && a.ctot) a.ctot[c] = ctot;
     }
     __syncthreads();
     for (int s = threadIdx.x; s < Ns; s += blockDim.x) {
