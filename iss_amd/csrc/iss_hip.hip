@@ -218,6 +218,7 @@ struct iss_ctx {
     int nlight = 0;  // leading species below 0.7 GeV (the 10-term series of the row pass)
     bool light_beyond = false;  // a species below 0.7 GeV after the first nlight ones
     DBuf<char> d_rows;
+    DBuf<float4> d_ecell;      // [2 Nc]: per cell flow and 3+1D position (k_emit_cells)
     DBuf<RowSp> d_rowsp;       // [2][row_stride(Ns, 4)]: all species, then negative charges suppressed
     bool charges_outside = false;  // a species charge beyond the row pass's fugacity rows
     DBuf<double> d_cdf, d_ctot, d_partial, d_stot;
@@ -441,6 +442,9 @@ int cell_prep(iss_ctx* ctx) {
     k_cell_prep<<<nb, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->Nc, deltaf_dev(ctx), ctx->d_coef.p);
     HIPCHK(ctx->d_aux.ensure(static_cast<size_t>(kAuxFields) * ctx->Nc));
     k_cell_aux<<<nb, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->Nc, ctx->d_aux.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(ctx->d_ecell.ensure(2 * ctx->Nc));
+    k_emit_cells<<<nb, 256, 0, ctx->stream>>>(ctx->d_cells.p, ctx->Nc, ctx->d_ecell.p);
     HIPCHK(hipGetLastError());
     return ISS_OK;
 }
@@ -873,7 +877,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_ctot.release(); ctx->d_partial.release(); ctx->d_oob.release(); ctx->d_stot.release(); ctx->d_raw.release();
     ctx->d_counts.release(); ctx->d_evcount.release(); ctx->d_off.release(); ctx->d_evoff.release();
     ctx->d_tree0.release(); ctx->d_tree1.release(); ctx->d_out.release(); ctx->d_setup.release(); ctx->d_stats.release();
-    ctx->d_cellidx.release();
+    ctx->d_cellidx.release(); ctx->d_ecell.release();
     ctx->d_ycol.release(); ctx->d_mcnt.release(); ctx->d_msegoff.release(); ctx->d_msp.release();
     ctx->d_anti.release(); ctx->d_lcc_flag.release(); ctx->d_lcc_g.release(); ctx->d_lcc_src.release();
     ctx->d_lcc_evoff.release(); ctx->d_lcc_rec.release(); ctx->d_out2.release();
@@ -1335,6 +1339,7 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     a.seed = o->seed;
     a.chunk = chunk;
     a.boost_invariant = binv ? 1 : 0;
+    a.ecell = binv ? nullptr : ctx->d_ecell.p;  // 3+1D: the cells' flow and position words
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
     a.max_trials = o->max_trials;
@@ -1479,6 +1484,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.neos = ctx->neos;
     a.diff = ctx->df.include_diffusion;
     a.boost_invariant = binv ? 1 : 0;
+    a.ecell = binv ? nullptr : ctx->d_ecell.p;  // 3+1D: the cells' flow and position words
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
     a.max_trials = o->max_trials;
@@ -2076,6 +2082,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(setup_cap, 1)));
     a.cellidx = ctx->d_cellidx.p;
     a.boost_invariant = binv ? 1 : 0;
+    a.ecell = binv ? nullptr : ctx->d_ecell.p;  // 3+1D: the cells' flow and position words
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
     a.max_trials = o->max_trials;

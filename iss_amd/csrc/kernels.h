@@ -158,6 +158,9 @@ struct SampleArgs {
     const uint32_t* cellidx;
     // the species of every slot (k_pick); null: drawn from the FP64 species CDF rows
     const uint16_t* slot_species;
+    // 3+1D surfaces: per cell (u0, u1, u2, u3) and (t, x, y, z) (k_emit_cells),
+    // loaded when a trial lane takes a hadron; null: emit_lab gathers them
+    const float4* ecell;
 };
 
 constexpr uint32_t kPartnerBit = 0x80000000u;

@@ -339,6 +339,19 @@ __device__ __forceinline__ f32x16 tile_ll(const uint16_t* A, int lda, int m0, in
 // (fexp, the exp2-based exponential, comes with the yield kernels)
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+// log1p(u (e^w - 1)): s (tau - a) of a draw from exp(s tau) on a band [a, b]
+// of width w / s (inverse CDF at u), with hardware exp2 / log2 and Taylor
+// series where either argument is below 1/8 (truncation below 1e-8 relative)
+__device__ __forceinline__ float trunc_exp_inv(float u, float w) {
+    const float em1 = fabsf(w) < 0.125f
+                          ? w * (1.f + w * (0.5f + w * (1.f / 6.f + w * (1.f / 24.f + w * (1.f / 120.f + w * (1.f / 720.f))))))
+                          : __builtin_amdgcn_exp2f(w * 1.4426950408889634f) - 1.f;
+    const float y = u * em1;
+    return fabsf(y) < 0.125f
+               ? y * (1.f - y * (0.5f - y * (1.f / 3.f - y * (0.25f - y * (0.2f - y * (1.f / 6.f - y * (1.f / 7.f - y * 0.125f)))))))
+               : __builtin_amdgcn_logf(1.f + y) * 0.6931471805599453f;
+}
+
 // cube root with sign, through exp2(log2(|x|) / 3)
 __device__ __forceinline__ float fcbrt(float x) {
     const float r = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fabsf(x)) * (1.f / 3.f));
@@ -929,6 +942,8 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
     uint64_t outp = 0;
     // band table in registers: thresholds (2 x u16 per word), lines, kappas (2 x fp16), betas (4 x u8)
     uint32_t Qw[4], CS[kBands], KW[4], BW[3];
+    float4 eu = make_float4(0.f, 0.f, 0.f, 0.f), ep = eu;  // the cell's flow and position (3+1D)
+    int pid = 0;
     for (int k = 0; k < 4; k++) { Qw[k] = 0u; KW[k] = 0u; }
     for (int k = 0; k < kBands; k++) CS[k] = 0u;
     for (int k = 0; k < 3; k++) BW[k] = 0u;
@@ -967,6 +982,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
                     KW[0] = h4.x; KW[1] = h4.y; KW[2] = h4.z; KW[3] = h4.w;
                     BW[0] = h5.x; BW[1] = h5.y; BW[2] = h5.z;
                     lparams_fast(a, L.aux, c, s, __uint_as_float(h5.w), P);
+                    if (a.ecell) {
+                        eu = a.ecell[2 * c];
+                        ep = a.ecell[2 * c + 1];
+                        pid = a.sp.pid[s];
+                    }
                     it = 0;
                     active = true;
                 }
@@ -1007,12 +1027,12 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         // tau from exp(s tau) on [ba, bb]
         const float ut = u24(w1r.y);
         const float sw = sk * bw;
-        float tau = fabsf(sw) > 1e-5f ? ba + fdiv(log1pf(ut * expm1f(sw)), sk) : ba + ut * bw;
+        float tau = fabsf(sw) > 1e-5f ? ba + trunc_exp_inv(ut, sw) * __builtin_amdgcn_rcpf(sk) : ba + ut * bw;
         tau = fminf(fmaxf(tau, ba), bb);
         const float t = P.x + tau;
         const float E = P.T * t + P.mu;
         const float p2 = fmaxf(E * E - P.m * P.m, 0.f);
-        const float p = sqrtf(p2);
+        const float p = __builtin_amdgcn_sqrtf(p2);
         const float et = fexp(-t);
         const float fe = fdiv(1.f, 1.f + P.sgn * et);     // e^t f0 = 1 - sign f0
         const float G = flog(p * E * fe) - tau;
@@ -1026,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         else { ax = 2; rr = fdiv(uy - P1, 1.f - P1); }
         rr = fminf(fmaxf(rr, 0.f), 1.f);
         const float cz = fcbrt(2.f * rr - 1.f);
-        const float sz = sqrtf(fmaxf(1.f - cz * cz, 0.f));
+        const float sz = __builtin_amdgcn_sqrtf(fmaxf(1.f - cz * cz, 0.f));
         float sph, cph;
         __sincosf(2.f * static_cast<float>(kPi) * u24(w1r.w), &sph, &cph);
         float n0, n1, n2;
@@ -1053,7 +1073,8 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const float px = p * (R00 * n0 + R01 * n1 + R02 * n2);
             const float py = p * (R10 * n0 + R11 * n1 + R12 * n2);
             const float pz = p * (R20 * n0 + R21 * n1 + R22 * n2);
-            emit_lab(a, a.chunk, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
+            if (a.ecell) emit_cell_lab(a, eu, ep, pid, P.m, px, py, pz, E, outp);
+            else emit_lab(a, a.chunk, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
             done = true;
         } else if (it >= max_it) {
             n_fail++;

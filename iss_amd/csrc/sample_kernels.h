@@ -225,6 +225,42 @@ __device__ __forceinline__ void emit_lab(const SampleArgs& a, uint32_t chunk, in
     static_cast<Hadron*>(a.out)[outp] = h;
 }
 
+// The per-cell part of the lab-frame record on 3+1D surfaces: the flow and
+// the position (t, x, y, z) = (tau cosh eta, x, y, tau sinh eta), once per
+// surface with the operations of emit_lab, so that a trial lane loads two
+// 16-byte words when it takes a hadron instead of gathering eight fields
+// (and evaluating cosh and sinh) at every acceptance.
+__global__ void k_emit_cells(const float* __restrict__ cells, int64_t Nc, float4* __restrict__ ec) {
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < Nc;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const float tau = cells[F_TAU * Nc + c], eta = cells[F_ETA * Nc + c];
+        ec[2 * c] = make_float4(cells[F_U0 * Nc + c], cells[F_U1 * Nc + c], cells[F_U2 * Nc + c],
+                                cells[F_U3 * Nc + c]);
+        ec[2 * c + 1] = make_float4(tau * coshf(eta), cells[F_X * Nc + c], cells[F_Y * Nc + c], tau * sinhf(eta));
+    }
+}
+
+// emit_lab on a 3+1D surface from the cell's words eu = u, ep = (t, x, y, z):
+// the same arithmetic; the 40-byte record as five 8-byte stores
+__device__ __forceinline__ void emit_cell_lab(const SampleArgs& a, float4 eu, float4 ep, int pid, float fm, float px,
+                                              float py, float pz, float p0, uint64_t outp) {
+    const float pdu = px * eu.y + py * eu.z + pz * eu.w;
+    const float k1 = pdu / (eu.x + 1.f) + p0;
+    const float lx = px + k1 * eu.y, ly = py + k1 * eu.z, lz = pz + k1 * eu.w;
+    const float mT = sqrtf(lx * lx + ly * ly + fm * fm);
+    const float E = sqrtf(mT * mT + lz * lz);
+    if (a.out_cap && outp >= a.out_cap) {  // batched rounds: beyond the reserved output
+        atomicAdd(a.stats + 7, 1ull);
+        return;
+    }
+    uint2* dst = reinterpret_cast<uint2*>(static_cast<Hadron*>(a.out) + outp);
+    dst[0] = make_uint2(static_cast<uint32_t>(pid), __float_as_uint(fm));
+    dst[1] = make_uint2(__float_as_uint(E), __float_as_uint(lx));
+    dst[2] = make_uint2(__float_as_uint(ly), __float_as_uint(lz));
+    dst[3] = make_uint2(__float_as_uint(ep.x), __float_as_uint(ep.y));
+    dst[4] = make_uint2(__float_as_uint(ep.z), __float_as_uint(ep.w));
+}
+
 // CDF_0, CDF_1, CDF_2 of one |p| table at E~ together: the per-term
 // exponentials become powers of exp(m0 - E~) and exp(-m0), so one hadron's
 // normalisation costs two exponentials and a logarithm.
@@ -523,11 +559,13 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     int s = 0, tb = 0, i0 = 0, imax = 0, it = 0, trials = 1;
     double mterm = 0., cmax = 0., rmin0 = 0.;
     PState q{};
-    float dsfac = 1.f, dfpref = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, Pi = 0.f;
+    float dsfac = 1.f, rdsfac = 1.f, rT = 1.f, dfpref = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, Pi = 0.f;
     float Bq = 0.f, Sq = 0.f, Qq = 0.f;
     float pixx = 0.f, pixy = 0.f, pixz = 0.f, piyy = 0.f, piyz = 0.f;
     float cf0 = 0.f, cf1 = 0.f, cf2 = 0.f, cf3 = 0.f, cf4 = 0.f, cf5 = 0.f;
     float qx = 0.f, qy = 0.f, qz = 0.f, pqm = 0.f;  // diffusion: q / D_qmu, n_B / (e + P)
+    float4 eu = make_float4(0.f, 0.f, 0.f, 0.f), ep = eu;  // the cell's flow and position (3+1D)
+    int pid = 0;
 
     while (true) {
         const unsigned long long need = __ballot(!active);
@@ -544,11 +582,18 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
                 mterm = r.mterm;
                 outp = r.out;
                 pstate_of(a, c, s, q);
+                if (a.ecell) {
+                    eu = a.ecell[2 * c];
+                    ep = a.ecell[2 * c + 1];
+                    pid = a.sp.pid[s];
+                }
                 d0 = a.cells[F_DA0 * Nc + c]; d1 = a.cells[F_DA1 * Nc + c];
                 d2 = a.cells[F_DA2 * Nc + c]; d3 = a.cells[F_DA3 * Nc + c];
                 const float Ed = a.cells[F_E * Nc + c], Pd = a.cells[F_P * Nc + c];
                 Bq = a.sp.B[s]; Sq = a.sp.S[s]; Qq = a.sp.Q[s];
                 dsfac = fabsf(d0) + sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
+                rdsfac = 1.0f / dsfac;
+                rT = 1.0f / q.T;
                 dfpref = 1.0f / (2.0f * q.T * q.T * (Ed + Pd));
                 if (a.shear) {
                     pixx = a.cells[F_PIXX * Nc + c]; pixy = a.cells[F_PIXY * Nc + c];
@@ -631,22 +676,25 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
         const double Etl = E0 + (Ehi - E0) / fmax(1e-16, rhi - rlo) * (r - rlo);
         const float fm = q.fm;
         const float Ep = static_cast<float>(q.Td * Etl + q.mu);
-        const float pm = sqrtf(fmaxf(Ep * Ep - fm * fm, 0.f));
-        const float ratio = (pm / Ep) / (1.f - fm * fm / (2.f * Ep * Ep));
+        // (hardware square root and reciprocals: 1 ulp)
+        const float pm = __builtin_amdgcn_sqrtf(fmaxf(Ep * Ep - fm * fm, 0.f));
+        const float rEp = __builtin_amdgcn_rcpf(Ep);
+        const float ratio = (pm * rEp) * __builtin_amdgcn_rcpf(1.f - fm * fm * (0.5f * rEp * rEp));
         bool done = false;
         if (u24(w.y) <= ratio) {
             n_trial++;
             const float T = q.T;
             const float phi = 2.f * static_cast<float>(kPi) * u24(w.z);
             const float cth = 2.f * u24(w.w) - 1.f;
-            const float sth = sqrtf(fmaxf(1.f - cth * cth, 0.f));
+            const float sth = __builtin_amdgcn_sqrtf(fmaxf(1.f - cth * cth, 0.f));
             float sphi, cphi;
             __sincosf(phi, &sphi, &cphi);
             const float pTc = pm * sth;
             const float px = pTc * cphi, py = pTc * sphi, pz = pm * cth;
-            const float p0 = sqrtf(fm * fm + pm * pm);
+            const float p0 = __builtin_amdgcn_sqrtf(fm * fm + pm * pm);
+            const float rp0 = __builtin_amdgcn_rcpf(p0);
             const float pds = p0 * d0 + px * d1 + py * d2 + pz * d3;
-            const float f0 = 1.f / (__expf((p0 - q.fmu) / T) + q.fsg);
+            const float f0 = __builtin_amdgcn_rcpf(__expf((p0 - q.fmu) * rT) + q.fsg);
             const float omf = 1.f - q.fsg * f0;
             float dfs = 0.f, dfb = 0.f;
             if (a.shear) {
@@ -667,14 +715,15 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
             float dfq = 0.f;
             if (a.diff) {
                 // (src/FSSW.cpp:1931-1937)
-                dfq = omf * (pqm - Bq / p0) * (-(px * qx + py * qy + pz * qz));
+                dfq = omf * (pqm - Bq * rp0) * (-(px * qx + py * qy + pz * qz));
             }
-            const float w1 = fminf(fmaxf(pds / p0 / dsfac, 0.f), 1.f);
+            const float w1 = fminf(fmaxf(pds * rp0 * rdsfac, 0.f), 1.f);
             const float w2 = fminf(fmaxf((1.f + dfs + dfb + dfq) * 0.5f, 0.f), 1.f);
             const uint32_t u4 = (w.y & 0xFFu) | ((w.z & 0xFFu) << 8) | ((w.w & 0xFFu) << 16);
             const float ua = (static_cast<float>(u4) + 0.5f) * (1.0f / 16777216.0f);
             if (ua < w1 * w2) {
-                emit_lab(a, chunk, c, jj, a.sp.pid[s], fm, px, py, pz, p0, outp);
+                if (a.ecell) emit_cell_lab(a, eu, ep, pid, fm, px, py, pz, p0, outp);
+                else emit_lab(a, chunk, c, jj, a.sp.pid[s], fm, px, py, pz, p0, outp);
                 done = true;
             } else {
                 trials++;
