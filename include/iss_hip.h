@@ -306,6 +306,8 @@ typedef struct {
     int32_t max_trials;
     int32_t precision;           /* yields: 64 or 32 */
     int32_t use_graph;           /* 1: one captured round replayed; 0: launched per round */
+    int32_t proposal;            /* 0 plain rejection, 1 learned (needs network parameters); the
+                                    species of every slot are the same under both */
 } iss_batch_opts;
 
 int iss_set_surfaces_lrf(iss_ctx* ctx, int32_t n_surfaces, const int64_t* n_cells, const float* cells28);

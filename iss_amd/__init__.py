@@ -118,7 +118,7 @@ class _DecayTable(C.Structure):
 class _BatchOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("hydro_mode", C.c_int32), ("y_LB", C.c_double), ("y_RB", C.c_double),
                 ("events_per_round", C.c_int32), ("max_trials", C.c_int32), ("precision", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("proposal", C.c_int32)]
 
 
 class _SpectraOpts(C.Structure):
@@ -440,13 +440,16 @@ class Context:
         self._check(self.lib.iss_set_surfaces_lrf(self.h, n.size, _ptr(n), _ptr(a)), "iss_set_surfaces_lrf")
         self.n_surfaces = n.size
 
-    def _batch_opts(self, seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph):
-        return _BatchOpts(seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph)
+    def _batch_opts(self, seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph,
+                    proposal=0):
+        return _BatchOpts(seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph,
+                          proposal)
 
     def sample_batch(self, n_events, seed=1, hydro_mode=2, y_LB=-5.0, y_RB=5.0, events_per_round=0,
-                     max_trials=5000, precision=32, use_graph=1):
+                     max_trials=5000, precision=32, use_graph=1, proposal=0):
         ev = np.ascontiguousarray(n_events, dtype=np.int32)
-        o = self._batch_opts(seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph)
+        o = self._batch_opts(seed, hydro_mode, y_LB, y_RB, events_per_round, max_trials, precision, use_graph,
+                             proposal)
         n = C.c_int64()
         self._check(self.lib.iss_sample_batch(self.h, C.byref(o), _ptr(ev), C.byref(n)), "iss_sample_batch")
         self._batch_events = ev.copy()

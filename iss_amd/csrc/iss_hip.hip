@@ -2006,6 +2006,9 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     (void)hipSetDevice(ctx->device);
     if (ctx->S <= 0) return set_err(ctx, ISS_ERR_STATE, "no batched surfaces (iss_set_surfaces_lrf)");
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
+    if (o->proposal != 0 && o->proposal != 1) return set_err(ctx, ISS_ERR_ARG, "proposal must be 0 or 1");
+    const bool learned = o->proposal == 1;
+    if (learned && !ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters");
     const int S = ctx->S;
     int maxE = 0;
     for (int k = 0; k < S; k++) {
@@ -2086,15 +2089,52 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
     a.max_trials = o->max_trials;
-    a.refill_min = ctx->refill_min[0];
+    a.refill_min = ctx->refill_min[learned ? 1 : 0];
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int per_cu_fb = per_cu;  // (the learned rounds' plain fallback)
     if (ctx->blocks_per_cu[0] > 0) per_cu = std::min(ctx->blocks_per_cu[0], per_cu);
     const int trial_blocks = ctx->num_cu * per_cu;
     const int setup_blocks = ctx->num_cu * 8;
     const int64_t ntiles = std::max<int64_t>(1, (N + kScanTile - 1) / kScanTile);
+    // learned proposal: the round's (cell, species) pairs, one network
+    // evaluation and band envelope per pair, the learned trials, and the
+    // plain sampler for the hadrons without an envelope; all bounded by the
+    // round's set-up records
+    LearnArgs L{};
+    int learn_blocks = 0, lsetup_blocks = 0;
+    const int Wb = (ctx->Ns + 31) / 32;
+    if (learned) {
+        const int64_t cap = std::max<int64_t>(setup_cap, 1);
+        HIPCHK(ctx->d_sp16.ensure(cap));
+        HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(N, 1) * Wb));
+        HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(N, 1)));
+        HIPCHK(ctx->d_pairoff.ensure(N + 1));
+        HIPCHK(ctx->d_pkey.ensure(cap));
+        HIPCHK(ctx->d_lslot.ensure(cap));
+        HIPCHK(ctx->d_lrec.ensure(cap));
+        HIPCHK(ctx->d_fb_list.ensure(cap));
+        a.slot_species = ctx->d_sp16.p;  // the fallback set-up reads the drawn species
+        L.a = a;
+        L.aux = ctx->d_aux.p;
+        L.W = MlpDev{ctx->d_mlp_frag.p, ctx->d_mlp_par.p};
+        L.fb_list = ctx->d_fb_list.p;
+        L.fb_count = ctx->d_fb_count.p;
+        L.stats2 = ctx->d_stats.p + 4;
+        L.pkey = ctx->d_pkey.p;
+        L.P = ctx->d_pairoff.p + N;
+        L.slot = ctx->d_lslot.p;
+        int pl = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pl, k_learn_trials, 256, 0) != hipSuccess || pl < 1) pl = 1;
+        if (ctx->blocks_per_cu[1] > 0) pl = std::min(ctx->blocks_per_cu[1], pl);
+        learn_blocks = ctx->num_cu * pl;
+        lsetup_blocks = static_cast<int>((cap + kSetupThreads - 1) / kSetupThreads);
+    }
+    SampleArgs af = a;  // fallback: the listed slots, counted on the device
+    af.Kdev = ctx->d_fb_count.p;
+    af.slot_species = ctx->d_sp16.p;
     // one round: only kernel launches, every size read on the device
     auto round = [&]() {
         if (N > 0) k_batch_counts<<<(N + 255) / 256, 256, 0, ctx->stream>>>(b, ctx->d_ctot.p, N, o->seed, ctx->d_counts.p);
@@ -2113,8 +2153,28 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         k_batch_multi_leaves<<<((static_cast<int64_t>(S) << D) + 255) / 256, 256, 0, ctx->stream>>>(
             b, cur, D, ctx->d_bevcount.p);
         if (N > 0) k_cell_index<<<(N + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, N, setup_cap, ctx->d_cellidx.p);
-        k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
-        k_trials<<<trial_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+        if (!learned) {
+            k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
+            k_trials<<<trial_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+        } else {
+            const int pblk = static_cast<int>(std::max<int64_t>(1, (N + kPickCells - 1) / kPickCells));
+            k_batch_pick<<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, ctx->d_sp16.p, ctx->d_lbits.p,
+                                                                       ctx->d_lcnt.p);
+            k_scan_tiles<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p);
+            k_scan_top<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan.p, ntiles);
+            k_scan_apply<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p, ctx->d_pairoff.p);
+            const int fblk = static_cast<int>(std::max<int64_t>(1, (N + kPairCellsPerBlock - 1) / kPairCellsPerBlock));
+            k_lpair_fill<<<fblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, Wb, ctx->d_lbits.p, ctx->d_pairoff.p,
+                                                                            ctx->d_sp16.p, ctx->d_lslot.p,
+                                                                            ctx->d_pkey.p);
+            k_zero_u64<<<1, 64, 0, ctx->stream>>>(ctx->d_fb_count.p);
+            k_learn_setup<<<lsetup_blocks, kSetupThreads, kMlpBytes * (kSetupThreads / kWave), ctx->stream>>>(
+                L, ctx->d_lrec.p);
+            k_learn_trials<<<learn_blocks, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
+            k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p,
+                                                                 ctx->d_fb_count.p);
+            k_trials<<<ctx->num_cu * per_cu_fb, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+        }
         k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);
     };
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -2167,8 +2227,8 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     ctx->stats.n_inner = static_cast<int64_t>(st[1]);
     ctx->stats.n_failed = static_cast<int64_t>(st[2]);
     ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
-    ctx->stats.n_fallback = 0;
-    ctx->stats.n_violations = 0;
+    ctx->stats.n_fallback = learned ? static_cast<int64_t>(st[4]) : 0;
+    ctx->stats.n_violations = learned ? static_cast<int64_t>(st[5]) : 0;
     ctx->stats.ms_sample = ms;
     ctx->stats.ms_sample_kernel = ms;
     if (st[6] || st[7] || ctx->K > out_cap) {

@@ -825,7 +825,10 @@ __global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_fill(SampleAr
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kPairCellsPerBlock + w;
     if (c >= a.Nc) return;
-    const int64_t o0 = a.off[c], n = a.off[c + 1] - o0;
+    const int64_t o0 = a.off[c];
+    int64_t n = a.off[c + 1] - o0;
+    // (batched rounds: only the slots with set-up records)
+    if (a.setup_cap > 0) n = max<int64_t>(0, min<int64_t>(n, a.setup_cap - o0));
     if (n == 0) return;
     uint32_t* bits = s_bits[w];
     uint32_t* pre = s_pre[w];
@@ -923,12 +926,16 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int64_t base = a.K / nwaves, rem = a.K % nwaves;
+    // (batched rounds: the round index and hadron count live on the device)
+    const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
+    int64_t Kn = a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K;
+    if (a.setup_cap > 0 && Kn > a.setup_cap) Kn = a.setup_cap;
+    const int64_t base = Kn / nwaves, rem = Kn % nwaves;
     const int64_t beg = wid * base + min(wid, rem);
     const int64_t end = beg + base + (wid < rem ? 1 : 0);
     const int64_t Nc = a.Nc;
     const uint32_t key0 = static_cast<uint32_t>(a.seed), key1 = static_cast<uint32_t>(a.seed >> 32);
-    const uint32_t tw1 = tag_word(TAG_LEARN, a.chunk);
+    const uint32_t tw1 = tag_word(TAG_LEARN, chunk);
     // one proposal is one outer trial of the reference (src/FSSW.cpp:1872)
     const int max_it = a.max_trials;
 
@@ -964,12 +971,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
                 s = static_cast<int>(h0.z & 0xFFFFu);
                 const int flags = static_cast<int>((h0.z >> 16) & 0xFFu);
                 klast = static_cast<int>(h0.z >> 24);
-                outp = a.perm(static_cast<uint64_t>(cur));
+                outp = a.perms ? batch_out(a, cur, c) : a.perm(static_cast<uint64_t>(cur));
                 if (flags & 1) {
                     // m/T - mu/T outside the table: a hole, as in the plain sampler
                     n_range++;
-                    Hadron h{};
-                    static_cast<Hadron*>(a.out)[outp] = h;
+                    write_hole(a, outp);
                 } else if (flags & 2) {
                     n_fb++;
                     const unsigned long long pos = atomicAdd(L.fb_count, 1ull);
@@ -1074,12 +1080,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const float py = p * (R10 * n0 + R11 * n1 + R12 * n2);
             const float pz = p * (R20 * n0 + R21 * n1 + R22 * n2);
             if (a.ecell) emit_cell_lab(a, eu, ep, pid, P.m, px, py, pz, E, outp);
-            else emit_lab(a, a.chunk, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
+            else emit_lab(a, chunk, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
             done = true;
         } else if (it >= max_it) {
             n_fail++;
-            Hadron h{};
-            static_cast<Hadron*>(a.out)[outp] = h;
+            write_hole(a, outp);
             done = true;
         }
         if (done) active = false;

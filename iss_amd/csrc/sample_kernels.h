@@ -446,6 +446,61 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     }
 }
 
+// an empty record (out of trials, or outside the momentum tables); batched
+// rounds count a slot beyond the reserved output instead of writing it
+__device__ __forceinline__ void write_hole(const SampleArgs& a, uint64_t outp) {
+    if (a.out_cap && outp >= a.out_cap) {
+        atomicAdd(a.stats + 7, 1ull);
+        return;
+    }
+    Hadron h{};
+    static_cast<Hadron*>(a.out)[outp] = h;
+}
+
+// Batched rounds: the output slot of the round's hadron i (cell c), within
+// its surface's block of this round, after the hadrons of the earlier rounds
+__device__ __forceinline__ uint64_t batch_out(const SampleArgs& a, int64_t i, int64_t c) {
+    const int k = upper_bound(a.surf_off + 1, 0, a.S, c);
+    const int64_t h0 = a.off[a.surf_off[k]];
+    return *a.out_base + static_cast<uint64_t>(h0) + a.perms[k](static_cast<uint64_t>(i - h0));
+}
+
+// Batched rounds with the learned proposal: each slot's species drawn from its
+// cell's FP64 CDF row exactly as k_setup draws it (the same stream, so the
+// species of every slot equal the plain rounds'), and the cell's species
+// bitmap and number of distinct species (the (cell, species) pairs).  Slots
+// at or beyond the round's set-up records are left out.  One wave per cell;
+// launch with ceil(Nc / kPickCells) workgroups of kPickCells waves.
+__global__ __launch_bounds__(kWave * kPickCells) void k_batch_pick(SampleArgs a, uint16_t* __restrict__ sp16,
+                                                                   uint32_t* __restrict__ gbits,
+                                                                   uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t s_bits[kPickCells][kPickWords];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
+    if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
+    const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
+    const int Ns = a.Ns, Wb = (Ns + 31) / 32;
+    const int64_t o0 = a.off[c];
+    int64_t n = a.off[c + 1] - o0;
+    if (a.setup_cap > 0) n = max<int64_t>(0, min<int64_t>(n, a.setup_cap - o0));
+    uint32_t* bits = s_bits[w];
+    if (lane < Wb) bits[lane] = 0u;
+    wave_lds_sync();
+    const double* row = a.cdf + c * cdf_stride(Ns);
+    for (int64_t r = lane; r < n; r += kWave) {
+        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, chunk));
+        int sp = upper_bound(row, 0, Ns, st.uniform() * a.ctot[c]);
+        if (sp >= Ns) sp = Ns - 1;
+        sp16[o0 + r] = static_cast<uint16_t>(sp);
+        atomicOr(&bits[sp >> 5], 1u << (sp & 31));
+    }
+    wave_lds_sync();
+    const uint32_t wd = lane < Wb ? bits[lane] : 0u;
+    if (lane < Wb) gbits[c * Wb + lane] = wd;
+    const uint32_t np = wave_sum(static_cast<uint32_t>(__popc(wd)));
+    if (lane == 0) cnt[c] = np;
+}
+
 // the species-dependent part of a set-up record: the |p| table, its first
 // row and the proposal's normalisation at E~ = m~ - mu~
 __device__ __forceinline__ HSetup setup_record(const SampleArgs& a, int64_t c, uint32_t jj, int s) {
@@ -503,15 +558,7 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
         if (s >= a.Ns) s = a.Ns - 1;
     }
     HSetup r = setup_record(a, c, jj, s);
-    if (BATCH) {
-        // batched surfaces: slot of the hadron within its surface's block of
-        // this round, after the hadrons of the earlier rounds
-        const int k = upper_bound(a.surf_off + 1, 0, a.S, c);
-        const int64_t h0 = a.off[a.surf_off[k]];
-        r.out = *a.out_base + static_cast<uint64_t>(h0) + a.perms[k](static_cast<uint64_t>(i - h0));
-    } else {
-        r.out = a.perm(static_cast<uint64_t>(i));
-    }
+    r.out = BATCH ? batch_out(a, i, c) : a.perm(static_cast<uint64_t>(i));
     rec[ri] = r;
     }
 }
@@ -622,8 +669,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
                     // m/T - mu/T outside the table: the reference exits
                     // (src/MomentumSamplerBase.cpp:33-43); leave a hole
                     n_range++;
-                    Hadron h{};
-                    static_cast<Hadron*>(a.out)[outp] = h;
+                    write_hole(a, outp);
                 } else {
                     const double* e0 = pt.C + (static_cast<int64_t>(tb) * kPTabLen + i0) * 4;
                     const double* em = pt.C + (static_cast<int64_t>(tb) * kPTabLen + imax) * 4;
@@ -731,8 +777,7 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
         }
         if (!done && (trials >= a.max_trials || it >= max_it)) {
             n_fail++;
-            Hadron h{};
-            static_cast<Hadron*>(a.out)[outp] = h;
+            write_hole(a, outp);
             done = true;
         }
         if (done) active = false;
