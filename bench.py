@@ -25,8 +25,13 @@ FP32 and FP64.
 log-normal size (median 3e4 cells, 1e3..5e5), each with its own event count
 in [10, 2000]; one step = one yield pass over all of them + one batched
 sampling pass (rounds of autotuned size replayed from a captured hipGraph).
+--workload c5 is the stress configuration: one 3+1D surface of 2M cells with
+mu_B up to 0.45 GeV, 5000 events of ~1e4 hadrons per step, in the same
+batched persistent rounds.  Both run the plain proposal, then the learned one
+(network trained on ~1e7 hadrons of a plain pass, outside the timed region);
+value = the learned rate.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload c2|c3]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload c2|c3|c5]
 """
 from __future__ import annotations
 
@@ -206,74 +211,130 @@ def spectra_extra(ctx, sp, n_cells):
     return out
 
 
-def run_c3(args):
+def run_batched(args):
+    """--workload c3 (256 uneven surfaces) or c5 (one 2M-cell surface with
+    mu_B up to 0.45 GeV, 5000 events): batched persistent rounds replayed from
+    a captured hipGraph, plain proposal, then the learned one on the same
+    surfaces; value = the learned rate."""
     import iss_amd
     from iss_amd import synth, tables
 
+    c5 = args.workload == "c5"
     sp = tables.species_arrays(1)
     t0 = time.perf_counter()
-    surfaces, ev = synth.batched_surfaces(sp, n_surfaces=args.surfaces)
+    if c5:
+        cells, _ = synth.viscous_lrf(2_000_000, sp, seed=5, muB_max=0.45, target_hadrons=1.0e4)
+        surfaces, ev = [cells], np.array([5000], dtype=np.int32)
+    else:
+        surfaces, ev = synth.batched_surfaces(sp, n_surfaces=args.surfaces)
     t_gen = time.perf_counter() - t0
     ncell = np.array([s.shape[0] for s in surfaces])
     ctx = iss_amd.Context(0)
     ctx.set_species(sp)
     ctx.set_deltaf(shear=1, bulk=1, bulk_kind=11, k11=tables.read_kind11(1), fermion_cdf2_quirk=0)
+    if c5:
+        ctx.set_surface_lrf(cells)
+        tot = ctx.compute_yields(precision=64)
+        cells[:, 4:8] *= np.float32(1.0e4 / tot.sum())
     ctx.set_surfaces_lrf(surfaces)
+    sub = surfaces[0][:2000].copy() if c5 else None
     del surfaces
     tune = {}
     if not args.no_tune:
         ctx.compute_yields(precision=args.precision)
         tune["plain"] = ctx.autotune(proposal=0, n_events=2, seed=7, repeats=1)
         small = np.maximum(ev // 10, 1).astype(np.int32)
-        tune["batch"] = ctx.autotune_batch(small, seed=7, precision=args.precision)
+        tune["batch_plain"] = ctx.autotune_batch(small, seed=7, precision=args.precision, proposal=0)
+    # the network, trained on ~1e7 hadrons accepted by a plain pass over the
+    # whole record (outside the timed region)
+    per_event = float(ctx.compute_yields(precision=args.precision).sum())
+    n_train_ev = int(np.clip(1.0e7 / max(per_event, 1.0), 1, 1000))
+    ctx.sample(n_train_ev, seed=3, precision=args.precision)
+    n_train = ctx.mlp_collect(10_000_000)
+    ctx.mlp_init(1)
+    rep = ctx.mlp_train(batch=8192, steps=args.train_steps, lr=3e-3, seed=5)
+    if not args.no_tune:
+        tune["learned"] = ctx.autotune(proposal=1, n_events=2, seed=7, repeats=1)
+        tune["batch_learned"] = ctx.autotune_batch(np.maximum(ev // 10, 1).astype(np.int32), seed=7,
+                                                   precision=args.precision, proposal=1)
 
-    def step(i):
-        ctx.compute_yields(precision=args.precision)
-        n = ctx.sample_batch(ev, seed=100 + i, precision=args.precision, use_graph=1)
-        return n, ctx.stats(), ctx.batch_ms()
+    def run(proposal):
+        def step(i):
+            ctx.compute_yields(precision=args.precision)
+            n = ctx.sample_batch(ev, seed=100 + i, precision=args.precision, use_graph=1, proposal=proposal)
+            return n, ctx.stats(), ctx.batch_ms()
 
-    for i in range(args.warmup):
-        step(i)
-    tot, ms_y, ms_b = 0, [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        n, st, mb = step(args.warmup + i)
-        tot += n
-        ms_y.append(st.ms_yield_kernel)
-        ms_b.append(mb)
-    el = time.perf_counter() - t0
+        for i in range(args.warmup):
+            step(i)
+        acc = {"hadrons": 0, "ms_yield": [], "ms_batch": [], "violations": 0, "fallback": 0, "trials": 0}
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            n, st, mb = step(args.warmup + i)
+            acc["hadrons"] += n
+            acc["ms_yield"].append(st.ms_yield_kernel)
+            acc["ms_batch"].append(mb)
+            acc["violations"] += st.n_violations
+            acc["fallback"] += st.n_fallback
+            acc["trials"] += st.n_trials
+        acc["elapsed"] = time.perf_counter() - t0
+        acc["R"] = ctx.lib.iss_batch_events_per_round(ctx.h)
+        return acc
+
+    plain = run(0)
+    learned = run(1)
     nc, ns = ctx.num_cells(), ctx.n_species
     ybytes = yield_cdf_bytes(nc, ns, True)
-    my = float(np.mean(ms_y))
+    my = float(np.mean(learned["ms_yield"]))
     y_gbs = ybytes / (my * 1e-3) / 1e9
+    v_learn = learned["hadrons"] / learned["elapsed"]
+    v_plain = plain["hadrons"] / plain["elapsed"]
+    batch_rate = {k: (r["hadrons"] / args.steps) / (1e-3 * float(np.mean(r["ms_batch"])))
+                  for k, r in (("learned", learned), ("plain", plain))}
+    if c5:
+        config = {"workload": "stress: one 3+1D viscous surface, 2M cells, mu_B <= 0.45 GeV, 5000 events per step, "
+                              "learned proposal in batched persistent rounds",
+                  "cells": int(nc), "events_per_step": int(ev[0]), "species": ns}
+    else:
+        config = {"workload": "batched uneven surfaces, learned proposal", "surfaces": int(ncell.size),
+                  "cells": int(nc), "cells_min_median_max": [int(ncell.min()), int(np.median(ncell)),
+                                                             int(ncell.max())],
+                  "events_min_max": [int(ev.min()), int(ev.max())], "species": ns}
+    config.update({"hadrons_per_step": learned["hadrons"] / args.steps,
+                   "events_per_round": {k: r["R"] for k, r in (("learned", learned), ("plain", plain))},
+                   "generation_s": t_gen,
+                   "deltaf": "shear + OSU 14-moment bulk", "train_samples": int(n_train),
+                   "train_steps": args.train_steps})
     out = {
         "metric": "sampled hadrons/s (learned proposal vs plain GPU) at fixed chi2; yield-pass GB/s",
-        "value": tot / el,
+        "value": v_learn,
         "unit": "hadrons/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps,
+        "ms_per_step": 1e3 * learned["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == 32 else "f64",
         "data": "synthetic",
-        "config": {"workload": "batched uneven surfaces (plain proposal)", "surfaces": int(ncell.size),
-                   "cells": int(nc), "cells_min_median_max": [int(ncell.min()), int(np.median(ncell)),
-                                                              int(ncell.max())],
-                   "events_min_max": [int(ev.min()), int(ev.max())], "species": ns,
-                   "hadrons_per_step": tot / args.steps, "events_per_round": ctx.lib.iss_batch_events_per_round(ctx.h),
-                   "rounds": ctx.lib.iss_batch_rounds(ctx.h), "generation_s": t_gen},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": y_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": y_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_yield", "ms": my,
                      "bytes": ybytes},
         "cpu_baseline": None,
-        "batch_ms": float(np.mean(ms_b)),
-        "batch_hadrons_per_s": (tot / args.steps) / (1e-3 * float(np.mean(ms_b))),
+        "plain_hadrons_per_s": v_plain,
+        "learned_vs_plain": v_learn / v_plain,
+        "batch_pass_hadrons_per_s": batch_rate,
+        "proposals_per_hadron": {k: r["trials"] / max(r["hadrons"], 1) for k, r in
+                                 (("learned", learned), ("plain", plain))},
+        "envelope_violations": learned["violations"],
+        "fallback_hadrons": learned["fallback"],
+        "training": rep,
         "autotune": tune,
     }
     ctx.close()
+    if c5 and not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sub)
     print(json.dumps(out))
 
 
@@ -289,7 +350,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-spectra", action="store_true", help="skip the smooth-spectra measurement")
-    ap.add_argument("--workload", choices=["c2", "c3"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2")
     ap.add_argument("--surfaces", type=int, default=256, help="c3: number of surfaces")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel for "
@@ -299,8 +360,8 @@ def main():
         print(f"bench.py runs on one GPU only (the sampler has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "c3":
-        run_c3(args)
+    if args.workload in ("c3", "c5"):
+        run_batched(args)
         return
 
     import iss_amd

@@ -477,9 +477,9 @@ class Context:
         sizes = ev[:, k, :].ravel()[: int(self._batch_events[k])]
         return (np.concatenate(parts) if parts else h[:0]), sizes
 
-    def autotune_batch(self, n_events, seed=1, hydro_mode=2, precision=32):
+    def autotune_batch(self, n_events, seed=1, hydro_mode=2, precision=32, proposal=0):
         ev = np.ascontiguousarray(n_events, dtype=np.int32)
-        o = self._batch_opts(seed, hydro_mode, -5.0, 5.0, 0, 5000, precision, 1)
+        o = self._batch_opts(seed, hydro_mode, -5.0, 5.0, 0, 5000, precision, 1, proposal)
         r = TuneReport()
         self._check(self.lib.iss_autotune_batch(self.h, C.byref(o), _ptr(ev), C.byref(r)), "iss_autotune_batch")
         return r.as_dict()

@@ -279,7 +279,7 @@ struct iss_ctx {
     DBuf<double> d_lam;
     int batch_rounds = 0, batch_R = 0, batch_S = 0;
     std::vector<int32_t> h_bevents;
-    int batch_R_tuned = 16;
+    int batch_R_tuned[2] = {16, 256};  // events per round, plain and learned proposal
     double batch_ms = 0., batch_ms_graph_launch = 0.;
     // resonance decay table
     int dec_n = 0;
@@ -2015,7 +2015,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         if (n_events[k] < 0) return set_err(ctx, ISS_ERR_ARG, "negative event count");
         maxE = std::max(maxE, n_events[k]);
     }
-    const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned;
+    const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned[o->proposal == 1 ? 1 : 0];
     if (R > 4096) return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most 4096");
     invalidate_output(ctx);
     if (!ctx->yields_valid || ctx->yields_lcc || ctx->yield_layout != LAYOUT_CDF ||
@@ -2273,11 +2273,17 @@ int iss_get_batch_layout(iss_ctx* ctx, int64_t* segments, int64_t* event_sizes) 
 
 int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_events, iss_tune_report* rep) {
     if (!ctx || !o || !n_events) return ISS_ERR_ARG;
-    const int cand[] = {4, 8, 16, 32, 64};
+    // the same candidates for both proposals (the learned one evaluates its
+    // network and envelopes once per (cell, species) pair of a round, so
+    // longer rounds share them among more hadrons)
+    const bool learned = o->proposal == 1;
+    const int cand[] = {4, 16, 64, 256, 1024, 4096};
+    const int ncand = 6, R_def = learned ? 256 : 16;
     iss_batch_opts bo = *o;
     double best = 1e30, ms_def = 0.;
-    int best_R = 16, tried = 0;
-    for (int R : cand) {
+    int best_R = R_def, tried = 0;
+    for (int ci = 0; ci < ncand; ci++) {
+        const int R = cand[ci];
         bo.events_per_round = R;
         double t = 1e30;
         for (int rep_i = 0; rep_i < 2; rep_i++) {
@@ -2287,10 +2293,10 @@ int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_e
             t = std::min(t, ctx->batch_ms);
         }
         tried++;
-        if (R == 16) ms_def = t;
+        if (R == R_def) ms_def = t;
         if (t < best) { best = t; best_R = R; }
     }
-    ctx->batch_R_tuned = best_R;
+    ctx->batch_R_tuned[learned ? 1 : 0] = best_R;
     if (rep) {
         rep->refill_min = best_R;   // events per round
         rep->blocks_per_cu = ctx->blocks_per_cu[0];
