@@ -50,15 +50,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector, packed (78.6 unpacked)
 
-# algorithmic FP32 operations (FMA = 2; add, mul, compare, min/max, select,
-# reciprocal, sqrt, exp, log, sin/cos = 1 each), counted from the kernels'
-# arithmetic; random-number and index arithmetic are not counted
-OPS_PLAIN_PROPOSAL = 115.0     # |p| proposal: table search step terms, ratio test
-OPS_PLAIN_TRIAL = 85.0         # direction, f0, delta f weights, w1 w2 test
-OPS_ACCEPTED = 60.0            # boost to the lab frame and the record
-OPS_PLAIN_SETUP = 190.0        # species pick, |p| normalisation (FP64 counted as FP32)
-OPS_LEARN_SETUP = 2900.0       # per (cell, species) pair: 14 inputs, 128 GELU, 9-band envelope, thresholds
-OPS_LEARN_PROPOSAL = 250.0     # band pick, t, f0, direction, weights, envelope ratio
+# Algorithmic units (the survey of the path's measurement): a sampling pass
+# costs 200 FP32 ops per trial (|p| proposal ~115 incl. inner tries, outer
+# trial ~85) plus 250 per accepted hadron (CDF refresh ~200, boost and record
+# ~50), FMA = 2; one network evaluation 2 (14 x 64 + 64 x 64 + 64 x 8) FLOP
+# (8 outputs) on the BF16 MFMA
+OPS_PER_TRIAL = 200.0
+OPS_PER_ACCEPTED = 250.0
+NET_FLOP_PER_PAIR = 2.0 * (14 * 64 + 64 * 64 + 64 * 8)   # 11,008
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
 # one smooth-spectrum integrand point as the reference writes it
 # (src/emissionfunction.cpp:756-806): pt, pz, p.u, exponent, exp, f0,
 # p.dsigma, the ten-term shear contraction, shear and bulk delta f, the
@@ -67,20 +67,11 @@ OPS_SPECTRA_POINT = 70.0
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 
 
-def yield_bytes(n_cells: int, n_species: int, with_bulk: bool, esz: int = 4) -> int:
-    """Algorithmic HBM bytes of one yield pass of one surface (k_yield_rows):
-    per cell T, muB, muS, muQ, u.dsigma, Pi (6 x 4 B) and, with bulk delta f,
-    its 3 FP64 coefficients; per (cell, species) one yield in the working
-    precision (esz bytes); per cell its FP64 total."""
-    per_cell_in = 6 * 4 + (3 * 8 if with_bulk else 0)
-    return n_cells * (per_cell_in + 8) + n_cells * n_species * esz
-
-
-def yield_cdf_bytes(n_cells: int, n_species: int, with_bulk: bool) -> int:
-    """The same for the batched surfaces' yield pass (k_yield), which stores
-    the FP64 species CDF of every cell (8 B per (cell, species))."""
-    per_cell_in = 6 * 4 + (3 * 8 if with_bulk else 0)
-    return n_cells * (per_cell_in + 8) + n_cells * n_species * 8
+def yield_bytes(n_cells: int, n_species: int, esz: int = 4) -> int:
+    """Algorithmic HBM bytes of one yield pass: per cell T, muB, muS, muQ,
+    u.dsigma, Pi (6 x 4 B), per (cell, species) one yield in the working
+    precision (esz bytes), and the FP64 species totals."""
+    return n_cells * 4 * 6 + n_cells * n_species * esz + 8 * n_species
 
 
 def cpu_model() -> str:
@@ -183,10 +174,33 @@ def run_steps(ctx, args, proposal, first_event):
     return acc
 
 
+def hw_from_profiles(kernel: str):
+    """Measured utilisation of `kernel` (VALU issue-slot share, FP32 and BF16
+    MFMA FLOP/s against peak) from the committed rocprofv3 counter summary,
+    or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "utilisation.json")) as fh:
+            u = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for k, v in u.items():
+        if isinstance(v, dict) and k.split("<")[0] == kernel:
+            return v
+    return None
+
+
 def roofline_valu(kernel, ops, ms):
     tf = ops / (ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": tf / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms}
+            "frac": tf / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms,
+            "hardware": hw_from_profiles(kernel)}
+
+
+def roofline_mfma(kernel, flop, ms):
+    tf = flop / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / BF16_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms,
+            "hardware": hw_from_profiles(kernel)}
 
 
 def spectra_extra(ctx, sp, n_cells):
@@ -283,7 +297,7 @@ def run_batched(args):
     plain = run(0)
     learned = run(1)
     nc, ns = ctx.num_cells(), ctx.n_species
-    ybytes = yield_cdf_bytes(nc, ns, True)
+    ybytes = yield_bytes(nc, ns, 8)  # (the batched pass stores FP64 species CDF rows)
     my = float(np.mean(learned["ms_yield"]))
     y_gbs = ybytes / (my * 1e-3) / 1e9
     v_learn = learned["hadrons"] / learned["elapsed"]
@@ -396,18 +410,21 @@ def main():
     v_learn = learned["hadrons"] / learned["elapsed"]
     v_plain = plain["hadrons"] / plain["elapsed"]
     hl, hp = learned["hadrons"] / args.steps, plain["hadrons"] / args.steps
-    ybytes = yield_bytes(nc, ns, True, 4 if args.precision == 32 else 8)
+    ybytes = yield_bytes(nc, ns, 4 if args.precision == 32 else 8)
     y_gbs = ybytes / (learned["ms_yield"] * 1e-3) / 1e9
     roof_y = {"bound": "hbm", "achieved": y_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
               "frac": y_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles("k_yield_rows"), "kernel": "k_yield_rows",
-              "ms": learned["ms_yield"], "bytes": ybytes}
+              "ms": learned["ms_yield"], "bytes": ybytes, "hardware": hw_from_profiles("k_yield_rows")}
     pairs = learned["pairs"] / args.steps
-    l_setup = roofline_valu("k_learn_setup", OPS_LEARN_SETUP * pairs, learned["ms_setup"])
-    l_trials = roofline_valu("k_learn_trials", OPS_LEARN_PROPOSAL * learned["inner"] / args.steps +
-                             OPS_ACCEPTED * hl, learned["ms_trials"])
-    p_setup = roofline_valu("k_setup", OPS_PLAIN_SETUP * hp, plain["ms_setup"])
-    p_trials = roofline_valu("k_trials", (OPS_PLAIN_PROPOSAL * plain["inner"] + OPS_PLAIN_TRIAL * plain["trials"])
-                             / args.steps + OPS_ACCEPTED * hp, plain["ms_trials"])
+    # the network (one evaluation per pair) against the BF16 MFMA peak; the
+    # sampling pass and its trial kernels against the FP32 vector peak
+    l_setup = roofline_mfma("k_learn_setup", NET_FLOP_PER_PAIR * pairs, learned["ms_setup"])
+    lt, pt_ = learned["trials"] / args.steps, plain["trials"] / args.steps
+    l_trials = roofline_valu("k_learn_trials", OPS_PER_TRIAL * lt + OPS_PER_ACCEPTED * hl, learned["ms_trials"])
+    p_trials = roofline_valu("k_trials", OPS_PER_TRIAL * pt_ + OPS_PER_ACCEPTED * hp, plain["ms_trials"])
+    pass_l = roofline_valu("sampling pass (learned)", OPS_PER_TRIAL * lt + OPS_PER_ACCEPTED * hl, learned["ms_sample"])
+    pass_p = roofline_valu("sampling pass (plain)", OPS_PER_TRIAL * pt_ + OPS_PER_ACCEPTED * hp, plain["ms_sample"])
+    p_setup = {"kernel": "k_setup", "ms": plain["ms_setup"], "hardware": hw_from_profiles("k_setup")}
     cands = [(learned["ms_yield"], roof_y), (learned["ms_setup"], l_setup), (learned["ms_trials"], l_trials)]
     dominant = max(cands, key=lambda t: t[0])[1]
     out = {
@@ -445,8 +462,8 @@ def main():
                        "k_learn_trials": learned["ms_trials"], "k_setup": plain["ms_setup"],
                        "k_trials": plain["ms_trials"]},
         "roofline_yield": roof_y,
-        "roofline_sample": {"k_learn_setup": l_setup, "k_learn_trials": l_trials, "k_setup": p_setup,
-                            "k_trials": p_trials},
+        "roofline_sample": {"pass_learned": pass_l, "pass_plain": pass_p, "k_learn_setup": l_setup,
+                            "k_learn_trials": l_trials, "k_setup": p_setup, "k_trials": p_trials},
         "yield_GBs": y_gbs,
     }
     # resonance feed-down of the last learned step's hadrons (outside the timed region)

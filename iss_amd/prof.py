@@ -11,6 +11,14 @@
         bench.py --calibration-copy-mib copied.
     python -m iss_amd.prof counters <pmc_dir> [<pmc_dir> ...]
         mean of every counter per kernel
+    python -m iss_amd.prof util <pmc_dir> [<pmc_dir> ...] [out.json]
+        hardware utilisation of each kernel's full-size dispatches from
+        `rocprofv3 --pmc` runs (rocpd databases: counters and durations per
+        dispatch; each counter averaged over the runs that hold it) with
+        SQ_WAVES, SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32, SQ_INSTS_MFMA,
+        SQ_INSTS_VALU_MFMA_MOPS_BF16, SQ_INSTS_VALU_FLOPS_FP32 and
+        GRBM_GUI_ACTIVE: clock, VALU issue-slot share, FP32 FLOP/s and BF16
+        MFMA FLOP/s against the MI355X peaks
 """
 from __future__ import annotations
 
@@ -86,6 +94,71 @@ def counters(*dirs):
             print(f"    {c:26s} {v:14.5g}")
 
 
+# MI355X: 8 XCDs of 32 CUs x 4 SIMDs; a wave64 VALU instruction occupies its
+# SIMD for 4 cycles; GRBM_GUI_ACTIVE counts each XCD's active cycles;
+# SQ_INSTS_VALU_FLOPS_FP32 counts FLOP per wave instruction (FMA = 2), so x 64
+# lanes (an upper bound: inactive lanes count); peaks FP32 vector 157.3
+# TFLOP/s, dense BF16 MFMA 2.5 PFLOP/s
+N_XCD = 8
+N_SIMD = 1024
+WAVE = 64
+FP32_PEAK = 157.3e12
+BF16_PEAK = 2.5e15
+MOP_FLOP = 512  # SQ_INSTS_VALU_MFMA_MOPS_*: units of 512 FLOP
+
+
+def util(*args):
+    import sqlite3
+    out = args[-1] if args and args[-1].endswith(".json") else None
+    dirs = args[:-1] if out else args
+    # kernel -> (run, dispatch) -> counters
+    per = collections.defaultdict(lambda: collections.defaultdict(dict))
+    for run, d in enumerate(dirs):
+        for db in glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True):
+            con = sqlite3.connect(db)
+            for disp, name, cname, val, dur in con.execute(
+                    "select dispatch_id, kernel_name, counter_name, value, duration from counters_collection"):
+                e = per[_short(name)][(run, db, disp)]
+                e[cname] = e.get(cname, 0.0) + float(val)
+                e["_dur_ns"] = float(dur)
+    res = {"N_SIMD": N_SIMD, "FP32_peak_TFLOPs": FP32_PEAK / 1e12, "BF16_peak_TFLOPs": BF16_PEAK / 1e12}
+    print("| kernel | dispatches | us | clock GHz | VALU/wave | VALU issue | trans share | FP32 TFLOP/s | "
+          "FP32 frac | BF16 MFMA TFLOP/s | MFMA frac |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|")
+    for k in sorted(per):
+        if "rocclr" in k:
+            continue
+        ds = list(per[k].values())
+        top = max(e["_dur_ns"] for e in ds)
+        full = [e for e in ds if e["_dur_ns"] >= 0.5 * top]  # the full-size launches
+        names = set().union(*full)
+        m = {}
+        for c in names:  # each counter over the launches (of any run) that hold it
+            v = [e[c] for e in full if c in e]
+            m[c] = sum(v) / len(v)
+        # rates use each counter's own runs' durations: the full-size launches
+        # of one kernel take the same time in every run
+        dur = m["_dur_ns"] * 1e-9
+        if dur <= 0:
+            continue
+        clk = m.get("GRBM_GUI_ACTIVE", 0.0) / (N_XCD * dur) if m.get("GRBM_GUI_ACTIVE") else 2.4e9
+        valu = m.get("SQ_INSTS_VALU", 0.0)
+        waves = max(m.get("SQ_WAVES", 1.0), 1.0)
+        issue = valu * 4.0 / (N_SIMD * clk * dur)
+        f32 = m.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) * WAVE / dur
+        bf16 = m.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0) * MOP_FLOP / dur
+        trans = m.get("SQ_INSTS_VALU_TRANS_F32", 0.0) / valu if valu else 0.0
+        res[k] = {"dispatches": len(full), "us": dur * 1e6, "clock_GHz": clk / 1e9, "valu_per_wave": valu / waves,
+                  "valu_issue_frac": issue, "trans_share": trans, "fp32_TFLOPs": f32 / 1e12,
+                  "fp32_frac": f32 / FP32_PEAK, "bf16_mfma_TFLOPs": bf16 / 1e12, "mfma_frac": bf16 / BF16_PEAK}
+        r = res[k]
+        print(f"| `{k}` | {r['dispatches']} | {r['us']:.1f} | {r['clock_GHz']:.2f} | {r['valu_per_wave']:.0f} | "
+              f"{r['valu_issue_frac']:.3f} | {r['trans_share']:.3f} | {r['fp32_TFLOPs']:.2f} | {r['fp32_frac']:.4f} | "
+              f"{r['bf16_mfma_TFLOPs']:.2f} | {r['mfma_frac']:.4f} |")
+    if out:
+        open(out, "w").write(json.dumps(res, indent=1) + "\n")
+
+
 if __name__ == "__main__":
     cmd, args = sys.argv[1], sys.argv[2:]
-    {"stats": stats, "traffic": traffic, "counters": counters}[cmd](*args)
+    {"stats": stats, "traffic": traffic, "counters": counters, "util": util}[cmd](*args)
