@@ -11,7 +11,8 @@ Checks:
     (Poisson mean and variance), species counts against its per-species yields;
   * one surface taken out of the batch against the same surface sampled alone
     (two-sample chi^2 of pT, y and species);
-  * the batch autotuner returns a tuned round size that is then used;
+  * the batch autotuner returns a tuned round size per proposal that the
+    proposal's batches then use;
   * decays of a batch's events equal the decays of the same list given as
     a plain hadron list, and the layout follows the decayed list.
 """
@@ -182,14 +183,20 @@ def test_surface_in_batch_matches_surface_alone(species, surfaces):
 
 
 def test_autotuned_round_size_is_used(species, surfaces):
+    """one tuned round size per proposal, each used by its own batches"""
     ctx = _ctx(species, surfaces)
+    cand = (4, 16, 64, 256, 1024, 4096)
     rep = ctx.autotune_batch(EVENTS, seed=2)
-    print("batch autotune", rep)
-    assert rep["refill_min"] in (4, 8, 16, 32, 64) and rep["n_tried"] == 5
-    ctx.sample_batch(EVENTS, seed=2)
-    seg, ev = ctx.batch_layout()
-    assert ev.shape[2] == rep["refill_min"]
-    assert seg.shape[0] == int(np.ceil(max(EVENTS) / rep["refill_min"]))
+    ctx.mlp_init(1)
+    rep_l = ctx.autotune_batch(EVENTS, seed=2, proposal=1)
+    print("batch autotune", rep, "learned", rep_l)
+    for r in (rep, rep_l):
+        assert r["refill_min"] in cand and r["n_tried"] == len(cand)
+    for proposal, r in ((0, rep), (1, rep_l)):
+        ctx.sample_batch(EVENTS, seed=2, proposal=proposal)
+        seg, ev = ctx.batch_layout()
+        assert ev.shape[2] == r["refill_min"]
+        assert seg.shape[0] == int(np.ceil(max(EVENTS) / r["refill_min"]))
     ctx.close()
 
 
