@@ -296,13 +296,25 @@ __device__ __forceinline__ double wave_scan_dpp(double v) {
     v += dpp_f64<kDppRowBcast31, 0xc>(0., v);
     return v;
 }
-// the sum over all lanes, returned uniformly (from lane 63 of the scan)
-__device__ __forceinline__ double wave_sum_dpp(double v) {
-    v = wave_scan_dpp(v);
+// lane 63's value, returned uniformly
+__device__ __forceinline__ double wave_last(double v) {
     const uint64_t x = __builtin_bit_cast(uint64_t, v);
     const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(x), 63);
     const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), 63);
     return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+// the sum over all lanes, returned uniformly (from lane 63 of the scan)
+__device__ __forceinline__ double wave_sum_dpp(double v) { return wave_last(wave_scan_dpp(v)); }
+// Running maximum over the lanes up to this one, in the steps of
+// wave_scan_dpp; for non-negative values (lanes without a source take 0)
+__device__ __forceinline__ double wave_max_scan_dpp(double v) {
+    v = fmax(v, dpp_f64<kDppRowShr + 1>(0., v));
+    v = fmax(v, dpp_f64<kDppRowShr + 2>(0., v));
+    v = fmax(v, dpp_f64<kDppRowShr + 4>(0., v));
+    v = fmax(v, dpp_f64<kDppRowShr + 8>(0., v));
+    v = fmax(v, dpp_f64<kDppRowBcast15, 0xa>(0., v));
+    v = fmax(v, dpp_f64<kDppRowBcast31, 0xc>(0., v));
+    return v;
 }
 
 // The 64-lane sums of 8 values at once (a transposed butterfly): lanes

@@ -408,14 +408,15 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     for (int s0 = 0; s0 < Ns; s0 += kWave) {
         const int s = s0 + lane;
         const double y = s < Ns ? static_cast<double>(row[s]) : 0.;
-        const double incl = wave_inclusive_max(wave_inclusive_scan(y)) + carry;
+        // (DPP scans: no LDS round trips)
+        const double incl = wave_max_scan_dpp(wave_scan_dpp(y)) + carry;
         if (s < Ns) pre[s] = incl;
         const unsigned long long m = __ballot(y > 0.);
         if (lane == 0) {
             pos[s0 >> 5] = static_cast<uint32_t>(m);
             if ((s0 >> 5) + 1 < Wb) pos[(s0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
         }
-        carry = __shfl(incl, kWave - 1, kWave);
+        carry = wave_last(incl);
     }
     wave_lds_sync();
     const double tot = carry;
