@@ -154,7 +154,7 @@ def run_steps(ctx, args, proposal, first_event):
     for i in range(args.warmup):
         step(i)
     acc = {"hadrons": 0, "trials": 0, "inner": 0, "ms_yield": [], "ms_sample": [], "ms_setup": [],
-           "ms_trials": [], "violations": 0, "fallback": 0, "pairs": 0}
+           "ms_trials": [], "violations": 0, "fallback": 0, "pairs": 0, "routed": 0}
     t0 = time.perf_counter()
     for i in range(args.steps):
         n, st = step(args.warmup + i)
@@ -168,6 +168,7 @@ def run_steps(ctx, args, proposal, first_event):
         acc["violations"] += st.n_violations
         acc["fallback"] += st.n_fallback
         acc["pairs"] += st.n_pairs if proposal == 1 else 0
+        acc["routed"] += st.n_routed if proposal == 1 else 0
     acc["elapsed"] = time.perf_counter() - t0
     for k in ("ms_yield", "ms_sample", "ms_setup", "ms_trials"):
         acc[k] = float(np.mean(acc[k]))
@@ -280,7 +281,8 @@ def run_batched(args):
 
         for i in range(args.warmup):
             step(i)
-        acc = {"hadrons": 0, "ms_yield": [], "ms_batch": [], "violations": 0, "fallback": 0, "trials": 0}
+        acc = {"hadrons": 0, "ms_yield": [], "ms_batch": [], "violations": 0, "fallback": 0, "trials": 0,
+               "routed": 0}
         t0 = time.perf_counter()
         for i in range(args.steps):
             n, st, mb = step(args.warmup + i)
@@ -290,6 +292,7 @@ def run_batched(args):
             acc["violations"] += st.n_violations
             acc["fallback"] += st.n_fallback
             acc["trials"] += st.n_trials
+            acc["routed"] += st.n_routed
         acc["elapsed"] = time.perf_counter() - t0
         acc["R"] = ctx.lib.iss_batch_events_per_round(ctx.h)
         return acc
@@ -343,6 +346,7 @@ def run_batched(args):
                                  (("learned", learned), ("plain", plain))},
         "envelope_violations": learned["violations"],
         "fallback_hadrons": learned["fallback"],
+        "routed_hadrons_per_step": learned["routed"] / args.steps,
         "training": rep,
         "autotune": tune,
     }
@@ -416,6 +420,7 @@ def main():
               "frac": y_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles("k_yield_rows"), "kernel": "k_yield_rows",
               "ms": learned["ms_yield"], "bytes": ybytes, "hardware": hw_from_profiles("k_yield_rows")}
     pairs = learned["pairs"] / args.steps
+    routed = learned["routed"] / args.steps
     # the network (one evaluation per pair) against the BF16 MFMA peak; the
     # sampling pass and its trial kernels against the FP32 vector peak
     l_setup = roofline_mfma("k_learn_setup", NET_FLOP_PER_PAIR * pairs, learned["ms_setup"])
@@ -454,7 +459,8 @@ def main():
                                  "plain": plain["inner"] / plain["hadrons"]},
         "envelope_violations": learned["violations"],
         "learned_pairs_per_step": pairs,
-        "hadrons_per_pair": hl / max(pairs, 1.0),
+        "hadrons_per_pair": (hl - routed) / max(pairs, 1.0),
+        "routed_hadrons_per_step": routed,
         "fallback_hadrons": learned["fallback"],
         "training": rep,
         "autotune": tune,

@@ -134,6 +134,8 @@ typedef struct {
     double ms_decay;         /* device time of the last iss_decay */
     int64_t n_decay_generations;
     int64_t n_pairs;         /* learned proposal: (cell, species) pairs of the last pass */
+    int64_t n_routed;        /* learned proposal: hadrons routed to the plain sampler
+                                (iss_set_learned_routing) */
 } iss_stats;
 
 /* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */
@@ -225,6 +227,13 @@ int iss_mlp_num_params(void);
 int iss_mlp_init(iss_ctx* ctx, uint64_t seed);
 int iss_mlp_set_params(iss_ctx* ctx, const float* params, int n);
 int iss_mlp_get_params(iss_ctx* ctx, float* params, int n);
+/* Routing of the learned passes (iss_sample, iss_sample_batch): a (cell,
+ * species) pair with fewer than min_hadrons hadrons in the pass takes the
+ * plain sampler, whose per-hadron set-up costs less than one network
+ * evaluation and envelope.  1: every pair takes the learned proposal;
+ * 2 (the default): the pairs of one hadron take the plain sampler
+ * (iss_stats.n_routed counts their hadrons).  The output law is the same. */
+int iss_set_learned_routing(iss_ctx* ctx, int32_t min_hadrons);
 
 typedef struct {
     int64_t batch;           /* samples per step, a multiple of 64 */

@@ -78,7 +78,7 @@ class Stats(C.Structure):
                 ("n_fallback", C.c_int64), ("n_violations", C.c_int64),
                 ("ms_setup_kernel", C.c_double), ("ms_trials_kernel", C.c_double),
                 ("ms_decay", C.c_double), ("n_decay_generations", C.c_int64),
-                ("n_pairs", C.c_int64)]
+                ("n_pairs", C.c_int64), ("n_routed", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -155,6 +155,7 @@ _SIGNATURES = {
     "iss_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "iss_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "iss_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "iss_set_learned_routing": (C.c_int, [C.c_void_p, C.c_int32]),
     "iss_mlp_collect": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "iss_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(_TrainOpts), C.POINTER(TrainReport)]),
     "iss_mlp_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -368,6 +369,11 @@ class Context:
         out = np.zeros(self.num_params(), dtype=np.float32)
         self._check(self.lib.iss_mlp_get_params(self.h, _ptr(out), out.size), "iss_mlp_get_params")
         return out
+
+    def set_learned_routing(self, min_hadrons: int = 2):
+        """Pairs with fewer hadrons than min_hadrons (1 or 2) in a learned
+        pass take the plain sampler."""
+        self._check(self.lib.iss_set_learned_routing(self.h, min_hadrons), "iss_set_learned_routing")
 
     def mlp_collect(self, max_samples: int = 10_000_000) -> int:
         n = C.c_int64()

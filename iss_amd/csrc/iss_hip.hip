@@ -250,10 +250,14 @@ struct iss_ctx {
     DBuf<int64_t> d_fb_list;
     DBuf<LRec> d_lrec;
     // learned sampler's (cell, species) pairs: compact list, species
-    // bitmaps, per-cell counts and first pairs, slot species and records
+    // bitmaps, per-cell counts and first pairs, slot species and records;
+    // per-cell counts and first positions of the slots routed to the plain
+    // sampler
     DBuf<uint2> d_pkey;
-    DBuf<uint32_t> d_lbits, d_lcnt;
-    DBuf<int64_t> d_pairoff;
+    DBuf<uint32_t> d_lbits, d_lcnt, d_lscnt;
+    DBuf<int64_t> d_pairoff, d_soff;
+    // pairs of fewer hadrons than this take the plain sampler (1 or 2)
+    int learn_route = 2;
     DBuf<uint16_t> d_sp16;
     DBuf<LSlot> d_lslot;
     DBuf<unsigned long long> d_fb_count;
@@ -858,7 +862,7 @@ int iss_ctx_create(int device, iss_ctx** out) {
     ctx->scratch = scratch_address(device, &ctx->scratch_err);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
-    if (ctx->d_stats.ensure(8) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
+    if (ctx->d_stats.ensure(9) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
         delete ctx;
         return ISS_ERR_NOMEM;
     }
@@ -884,6 +888,7 @@ int iss_ctx_destroy(iss_ctx* ctx) {
     ctx->d_aux.release(); ctx->d_mlp_par.release(); ctx->d_adam_m.release(); ctx->d_adam_v.release();
     ctx->d_grad.release(); ctx->d_mlp_frag.release(); ctx->d_fb_list.release(); ctx->d_fb_count.release(); ctx->d_lrec.release();
     ctx->d_pkey.release(); ctx->d_lbits.release(); ctx->d_lcnt.release(); ctx->d_pairoff.release();
+    ctx->d_lscnt.release(); ctx->d_soff.release();
     ctx->d_sp16.release(); ctx->d_lslot.release();
     ctx->d_tpart.release(); ctx->d_tloss.release(); ctx->d_tout.release(); ctx->d_tr_cell.release();
     ctx->d_ev_cell.release(); ctx->d_tr_spec.release(); ctx->d_ev_spec.release(); ctx->d_i32a.release();
@@ -1390,6 +1395,7 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
     ctx->stats.n_fallback = 0;
     ctx->stats.n_violations = 0;
+    ctx->stats.n_routed = 0;
     if (n_hadrons) *n_hadrons = K;
     if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
     return ISS_OK;
@@ -1496,12 +1502,15 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     // each slot's cell and species from its cell's yield row; the learned
     // pass also gets each cell's species bitmap and pair count
     const bool learned = o->proposal == 1;
+    const bool route = learned && ctx->learn_route > 1;
     const int Wb = (ctx->Ns + 31) / 32;
     HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(K, 1)));
     HIPCHK(ctx->d_sp16.ensure(std::max<int64_t>(K, 1)));
     if (learned) {
         HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(Nc, 1) * Wb));
         HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(Nc, 1)));
+        HIPCHK(ctx->d_lscnt.ensure(std::max<int64_t>(Nc, 1)));
+        HIPCHK(ctx->d_soff.ensure(Nc + 1));
     }
     a.cellidx = ctx->d_cellidx.p;
     a.slot_species = ctx->d_sp16.p;
@@ -1512,14 +1521,15 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         const void* rows = ctx->d_rows.p;
         uint32_t* gb = learned ? ctx->d_lbits.p : nullptr;
         uint32_t* gc = learned ? ctx->d_lcnt.p : nullptr;
+        uint32_t* gs = route ? ctx->d_lscnt.p : nullptr;
         if (f64 && learned)
-            k_pick<double, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+            k_pick<double, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
         else if (f64)
-            k_pick<double, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+            k_pick<double, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
         else if (learned)
-            k_pick<float, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+            k_pick<float, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
         else
-            k_pick<float, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc);
+            k_pick<float, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
         HIPCHK(hipGetLastError());
     }
     int per_cu_trials = 0;
@@ -1561,10 +1571,18 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             const int pblk = static_cast<int>((Nc + kPairCellsPerBlock - 1) / kPairCellsPerBlock);
             rc = launch_scan(ctx, ctx->d_lcnt.p, Nc, ctx->d_pairoff.p);
             if (rc) return rc;
+            // routing: the slots of single-hadron pairs head the plain list
+            if (route) {
+                rc = launch_scan(ctx, ctx->d_lscnt.p, Nc, ctx->d_soff.p);
+                if (rc) return rc;
+            }
+            const int64_t* soff = route ? ctx->d_soff.p : nullptr;
             k_lpair_fill<<<pblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, Wb, ctx->d_lbits.p, ctx->d_pairoff.p,
                                                                             ctx->d_sp16.p, ctx->d_lslot.p,
-                                                                            ctx->d_pkey.p);
+                                                                            ctx->d_pkey.p, soff, ctx->d_fb_list.p);
+            k_plain_count<<<1, 64, 0, ctx->stream>>>(route ? soff + Nc : nullptr, ctx->d_fb_count.p, nullptr);
             HIPCHK(hipGetLastError());
+            L.nsingle = route ? soff + Nc : nullptr;
             // the pair count stays on the device: the set-up grid covers the
             // bound P <= K and the waves beyond P return at once
             L.pkey = ctx->d_pkey.p;
@@ -1586,11 +1604,15 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             const int nblk = static_cast<int>(
                 std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
             k_learn_trials<<<nblk, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
-            // hadrons without an envelope: the plain sampler, sized on the device
-            k_setup<false><<<ctx->num_cu * 4, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p, ctx->d_fb_count.p);
+            // routed hadrons and hadrons without an envelope: the plain
+            // sampler (its own launch shape), sized on the device
+            const int64_t fb_want = route ? (K + 255) / 256 : ctx->num_cu * 4;
+            k_setup<false><<<static_cast<int>(std::max<int64_t>(1, fb_want)), 256, 0, ctx->stream>>>(
+                a, ctx->d_setup.p, ctx->d_fb_list.p, ctx->d_fb_count.p);
             SampleArgs af = a;
             af.Kdev = ctx->d_fb_count.p;
-            k_trials<<<ctx->num_cu * per_cu_trials, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            af.refill_min = ctx->refill_min[0];
+            k_trials<<<ctx->num_cu * per_cu_plain, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
         }
     }
     HIPCHK(hipGetLastError());
@@ -1601,14 +1623,18 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     unsigned long long* h_st = ctx->h_pin.at<unsigned long long>(PIN_STATS);
     HIPCHK(hipMemcpyAsync(h_st, ctx->d_stats.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    int64_t npairs = 0;
+    int64_t npairs = 0, nrouted = 0;
     if (o->proposal == 1 && K > 0)
         HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_AUX), ctx->d_pairoff.p + Nc, sizeof(int64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    if (route && K > 0)
+        HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_AUX + 8), ctx->d_soff.p + Nc, sizeof(int64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     unsigned long long st[6];
     std::memcpy(st, h_st, sizeof(st));
     if (o->proposal == 1 && K > 0) npairs = *ctx->h_pin.at<int64_t>(PIN_AUX);
+    if (route && K > 0) nrouted = *ctx->h_pin.at<int64_t>(PIN_AUX + 8);
     float ms_all = 0.f, ms_k = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
     HIPCHK(hipEventElapsedTime(&ms_k, ctx->ev[4], ctx->ev[5]));
@@ -1631,6 +1657,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     ctx->stats.n_fallback = static_cast<int64_t>(st[4]);
     ctx->stats.n_violations = static_cast<int64_t>(st[5]);
     if (o->proposal == 1) ctx->stats.n_pairs = npairs;
+    ctx->stats.n_routed = nrouted;
     if (n_hadrons) *n_hadrons = K;
     if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
     return ISS_OK;
@@ -1720,6 +1747,14 @@ int iss_autotune(iss_ctx* ctx, const iss_tune_opts* o, iss_tune_report* rep) {
 }
 
 int iss_mlp_num_params(void) { return kNumParams; }
+
+int iss_set_learned_routing(iss_ctx* ctx, int32_t min_hadrons) {
+    if (!ctx) return ISS_ERR_ARG;
+    if (min_hadrons != 1 && min_hadrons != 2)
+        return set_err(ctx, ISS_ERR_ARG, "learned routing: min_hadrons must be 1 or 2");
+    ctx->learn_route = min_hadrons;
+    return ISS_OK;
+}
 
 int iss_mlp_init(iss_ctx* ctx, uint64_t seed) {
     if (!ctx) return ISS_ERR_ARG;
@@ -2065,7 +2100,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     HIPCHK(ctx->d_bbase.ensure(1));
     HIPCHK(hipMemsetAsync(ctx->d_bround.p, 0, sizeof(uint32_t), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->d_bbase.p, 0, sizeof(unsigned long long), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 9 * sizeof(unsigned long long), ctx->stream));
     BatchDev b{ctx->d_surf_off.p, ctx->d_bevents.p, S, R, ctx->d_bround.p, ctx->d_bbase.p, ymult};
     SampleArgs a = base_args(ctx);
     a.cdf = ctx->d_cdf.p;
@@ -2106,12 +2141,15 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     LearnArgs L{};
     int learn_blocks = 0, lsetup_blocks = 0;
     const int Wb = (ctx->Ns + 31) / 32;
+    const bool route = learned && ctx->learn_route > 1;
     if (learned) {
         const int64_t cap = std::max<int64_t>(setup_cap, 1);
         HIPCHK(ctx->d_sp16.ensure(cap));
         HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(N, 1) * Wb));
         HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(N, 1)));
+        HIPCHK(ctx->d_lscnt.ensure(std::max<int64_t>(N, 1)));
         HIPCHK(ctx->d_pairoff.ensure(N + 1));
+        HIPCHK(ctx->d_soff.ensure(N + 1));
         HIPCHK(ctx->d_pkey.ensure(cap));
         HIPCHK(ctx->d_lslot.ensure(cap));
         HIPCHK(ctx->d_lrec.ensure(cap));
@@ -2126,6 +2164,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         L.pkey = ctx->d_pkey.p;
         L.P = ctx->d_pairoff.p + N;
         L.slot = ctx->d_lslot.p;
+        L.nsingle = route ? ctx->d_soff.p + N : nullptr;
         int pl = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pl, k_learn_trials, 256, 0) != hipSuccess || pl < 1) pl = 1;
         if (ctx->blocks_per_cu[1] > 0) pl = std::min(ctx->blocks_per_cu[1], pl);
@@ -2135,6 +2174,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     SampleArgs af = a;  // fallback: the listed slots, counted on the device
     af.Kdev = ctx->d_fb_count.p;
     af.slot_species = ctx->d_sp16.p;
+    af.refill_min = ctx->refill_min[0];
     // one round: only kernel launches, every size read on the device
     auto round = [&]() {
         if (N > 0) k_batch_counts<<<(N + 255) / 256, 256, 0, ctx->stream>>>(b, ctx->d_ctot.p, N, o->seed, ctx->d_counts.p);
@@ -2159,21 +2199,29 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         } else {
             const int pblk = static_cast<int>(std::max<int64_t>(1, (N + kPickCells - 1) / kPickCells));
             k_batch_pick<<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, ctx->d_sp16.p, ctx->d_lbits.p,
-                                                                       ctx->d_lcnt.p);
+                                                                       ctx->d_lcnt.p, route ? ctx->d_lscnt.p : nullptr);
             k_scan_tiles<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p);
             k_scan_top<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan.p, ntiles);
             k_scan_apply<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p, ctx->d_pairoff.p);
+            // routing: the slots of single-hadron pairs head the plain list
+            if (route) {
+                k_scan_tiles<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lscnt.p, N, ctx->d_scan.p);
+                k_scan_top<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan.p, ntiles);
+                k_scan_apply<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lscnt.p, N, ctx->d_scan.p, ctx->d_soff.p);
+            }
+            const int64_t* soff = route ? ctx->d_soff.p : nullptr;
             const int fblk = static_cast<int>(std::max<int64_t>(1, (N + kPairCellsPerBlock - 1) / kPairCellsPerBlock));
             k_lpair_fill<<<fblk, 64 * kPairCellsPerBlock, 0, ctx->stream>>>(a, Wb, ctx->d_lbits.p, ctx->d_pairoff.p,
                                                                             ctx->d_sp16.p, ctx->d_lslot.p,
-                                                                            ctx->d_pkey.p);
-            k_zero_u64<<<1, 64, 0, ctx->stream>>>(ctx->d_fb_count.p);
+                                                                            ctx->d_pkey.p, soff, ctx->d_fb_list.p);
+            k_plain_count<<<1, 64, 0, ctx->stream>>>(route ? soff + N : nullptr, ctx->d_fb_count.p,
+                                                     ctx->d_stats.p + 8);
             k_learn_setup<<<lsetup_blocks, kSetupThreads, kMlpBytes * (kSetupThreads / kWave), ctx->stream>>>(
                 L, ctx->d_lrec.p);
             k_learn_trials<<<learn_blocks, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
             k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p,
                                                                  ctx->d_fb_count.p);
-            k_trials<<<ctx->num_cu * per_cu_fb, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            k_trials<<<ctx->num_cu * (route ? per_cu : per_cu_fb), 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
         }
         k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);
     };
@@ -2207,7 +2255,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         }
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    unsigned long long st[8];
+    unsigned long long st[9];
     unsigned long long K = 0;
     HIPCHK(hipMemcpyAsync(st, ctx->d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(&K, ctx->d_bbase.p, sizeof(K), hipMemcpyDeviceToHost, ctx->stream));
@@ -2229,6 +2277,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
     ctx->stats.n_fallback = learned ? static_cast<int64_t>(st[4]) : 0;
     ctx->stats.n_violations = learned ? static_cast<int64_t>(st[5]) : 0;
+    ctx->stats.n_routed = learned ? static_cast<int64_t>(st[8]) : 0;
     ctx->stats.ms_sample = ms;
     ctx->stats.ms_sample_kernel = ms;
     if (st[6] || st[7] || ctx->K > out_cap) {

@@ -773,6 +773,9 @@ struct LearnArgs {
     const uint2* pkey;
     const int64_t* P;
     const LSlot* slot;
+    // with routing, the number of slots sampled by the plain sampler (the
+    // learned slot list holds the others); null: every slot is listed
+    const int64_t* nsingle;
 };
 
 // Per-hadron record of the learned sampler (96 bytes): the band table with
@@ -807,10 +810,14 @@ static_assert(sizeof(LSlot) == 16, "LSlot layout");
 // The pairs of a pass.  One hadron in 1.5 shares its (cell, species) with
 // another in the 200k-cell workload, so the network and the band envelope are
 // evaluated once per pair.  k_pick (the species of every slot) also leaves each
-// cell's species bitmap and its number of pairs; then
-//   scan           the cells' first pair indices
-//   k_lpair_fill   the compact pair list (cell, species) in bitmap order and
-//                  each slot's record with its pair (one wave per cell)
+// cell's species bitmap and its number of pairs (with routing: the species of
+// two or more slots, and the number of the other slots); then
+//   scan           the cells' first pair indices (and first routed slots)
+//   k_lpair_fill   the compact pair list (cell, species) in bitmap order,
+//                  the learned slots' records with their pairs, in slot order
+//                  (one wave per cell), and with routing the routed slots'
+//                  indices, in slot order, at the head of the plain list
+// Every position comes from the scans, so the lists do not depend on timing.
 constexpr int kPairCellsPerBlock = 4;
 constexpr int kPairWords = kMaxSpecies / 32;
 
@@ -819,7 +826,9 @@ __global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_fill(SampleAr
                                                                           const int64_t* __restrict__ pairoff,
                                                                           const uint16_t* __restrict__ sp16,
                                                                           LSlot* __restrict__ slot,
-                                                                          uint2* __restrict__ pkey) {
+                                                                          uint2* __restrict__ pkey,
+                                                                          const int64_t* __restrict__ soff,
+                                                                          int64_t* __restrict__ plain) {
     __shared__ uint32_t s_bits[kPairCellsPerBlock][kPairWords];
     __shared__ uint32_t s_pre[kPairCellsPerBlock][kPairWords];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -844,16 +853,39 @@ __global__ __launch_bounds__(64 * kPairCellsPerBlock) void k_lpair_fill(SampleAr
     auto rank = [&](int s) { return pre[s >> 5] + __popc(bits[s >> 5] & ((1u << (s & 31)) - 1u)); };
     for (int s = lane; s < a.Ns; s += 64)
         if (bits[s >> 5] & (1u << (s & 31))) pkey[base + rank(s)] = make_uint2(static_cast<uint32_t>(c), s);
-    for (int64_t r = lane; r < n; r += 64) {
-        const int s = sp16[o0 + r];
-        LSlot rec;
-        rec.cell = static_cast<uint32_t>(c);
-        rec.j = static_cast<uint32_t>(r);
-        rec.pair = static_cast<uint32_t>(base + rank(s));
-        rec.species = static_cast<uint16_t>(s);
-        rec.pad = 0;
-        slot[o0 + r] = rec;
+    // this cell's first learned and first routed positions
+    const int64_t s0 = soff ? soff[c] : 0;
+    int64_t lpos = o0 - s0, spos = s0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int64_t r0 = 0; r0 < n; r0 += 64) {
+        const int64_t r = r0 + lane;
+        const int s = r < n ? static_cast<int>(sp16[o0 + r]) : 0;
+        const bool pair = r < n && ((bits[s >> 5] >> (s & 31)) & 1u);
+        const bool routed = r < n && !pair;
+        const unsigned long long mp = __ballot(pair), mr = __ballot(routed);
+        if (pair) {
+            LSlot rec;
+            rec.cell = static_cast<uint32_t>(c);
+            rec.j = static_cast<uint32_t>(r);
+            rec.pair = static_cast<uint32_t>(base + rank(s));
+            rec.species = static_cast<uint16_t>(s);
+            rec.pad = 0;
+            slot[lpos + __popcll(mp & below)] = rec;
+        }
+        if (routed) plain[spos + __popcll(mr & below)] = o0 + r;
+        lpos += __popcll(mp);
+        spos += __popcll(mr);
     }
+}
+
+// the plain list's length before the learned trials append their fallbacks:
+// the routed slots (none without routing); acc (batched rounds) sums them
+__global__ void k_plain_count(const int64_t* __restrict__ nrouted, unsigned long long* __restrict__ count,
+                              unsigned long long* __restrict__ acc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const unsigned long long n = nrouted ? static_cast<unsigned long long>(*nrouted) : 0ull;
+    *count = n;
+    if (acc) *acc += n;
 }
 
 
@@ -930,6 +962,8 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
     const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
     int64_t Kn = a.Kdev ? static_cast<int64_t>(*a.Kdev) : a.K;
     if (a.setup_cap > 0 && Kn > a.setup_cap) Kn = a.setup_cap;
+    // (the learned slot list: every slot but the routed ones)
+    if (L.nsingle) Kn = max<int64_t>(0, Kn - *L.nsingle);
     const int64_t base = Kn / nwaves, rem = Kn % nwaves;
     const int64_t beg = wid * base + min(wid, rem);
     const int64_t end = beg + base + (wid < rem ? 1 : 0);
@@ -938,19 +972,24 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
     const uint32_t tw1 = tag_word(TAG_LEARN, chunk);
     // one proposal is one outer trial of the reference (src/FSSW.cpp:1872)
     const int max_it = a.max_trials;
+    // launch-uniform forms of the shear and bulk terms (scalar registers):
+    // shear Ps(E) = ps0 (1) or ps0 / E (2); bulk qi / E + q1 E (RTA, CE) or
+    // q0 + q1 E + q2 E^2, so qi and q0 share the lane's register qa
+    const int smode_u = a.shear ? (a.neos == 1 ? 2 : 1) : 0;
+    const bool bulk_inv = a.bulk == BULK_RTA || a.bulk == BULK_CE;
 
     int64_t next = beg;
-    unsigned long long n_it = 0, n_fail = 0, n_range = 0, n_fb = 0, n_viol = 0;
+    // per-lane counters (32 bits: a lane's share of one launch), summed in 64
+    uint32_t n_it = 0, n_fail = 0, n_range = 0, n_fb = 0, n_viol = 0;
 
     bool active = false;
-    int64_t c = 0, cur = 0;
+    uint32_t c = 0;
+    int64_t cur = 0;
     uint32_t jj = 0;
     int s = 0, it = 0, klast = 0;
     uint64_t outp = 0;
     // band table in registers: thresholds (2 x u16 per word), lines, kappas (2 x fp16), betas (4 x u8)
     uint32_t Qw[4], CS[kBands], KW[4], BW[3];
-    float4 eu = make_float4(0.f, 0.f, 0.f, 0.f), ep = eu;  // the cell's flow and position (3+1D)
-    int pid = 0;
     for (int k = 0; k < 4; k++) { Qw[k] = 0u; KW[k] = 0u; }
     for (int k = 0; k < kBands; k++) CS[k] = 0u;
     for (int k = 0; k < 3; k++) BW[k] = 0u;
@@ -962,12 +1001,12 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const int64_t navail = end - next;
             const int rank = __popcll(need & ((1ull << lane) - 1));
             if (!active && rank < navail) {
-                cur = next + rank;
-                const LSlot sl = L.slot[cur];
+                const LSlot sl = L.slot[next + rank];
                 const uint4* r = reinterpret_cast<const uint4*>(rec + sl.pair);
                 const uint4 h0 = r[0], h1 = r[1], h2 = r[2], h3 = r[3], h4 = r[4], h5 = r[5];
                 c = sl.cell;
                 jj = sl.j;
+                cur = a.off[c] + jj;  // the slot (its output position)
                 s = static_cast<int>(h0.z & 0xFFFFu);
                 const int flags = static_cast<int>((h0.z >> 16) & 0xFFu);
                 klast = static_cast<int>(h0.z >> 24);
@@ -988,11 +1027,7 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
                     KW[0] = h4.x; KW[1] = h4.y; KW[2] = h4.z; KW[3] = h4.w;
                     BW[0] = h5.x; BW[1] = h5.y; BW[2] = h5.z;
                     lparams_fast(a, L.aux, c, s, __uint_as_float(h5.w), P);
-                    if (a.ecell) {
-                        eu = a.ecell[2 * c];
-                        ep = a.ecell[2 * c + 1];
-                        pid = a.sp.pid[s];
-                    }
+                    P.qi += P.q0;  // (one of the two is zero: qa of the bulk form)
                     it = 0;
                     active = true;
                 }
@@ -1064,9 +1099,11 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         if (uf >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }
         const float W = P.l1 * n0 * n0 + P.l2 * n1 * n1 + P.l3 * n2 * n2;
         // the exact Cooper-Frye weights of the plain sampler
-        const float w1 = (P.D > 0.f) ? clamp01(fdiv(E * P.d0 + p * P.dn * y, E * P.D)) : 0.f;
-        const float dfs = fe * p2 * W * ps_of(P, E);
-        const float dfb = fe * qb_of(P, E);
+        const float Dn = fabsf(P.d0) + P.dn;
+        const float w1 = (Dn > 0.f) ? clamp01(fdiv(E * P.d0 + p * P.dn * y, E * Dn)) : 0.f;
+        const float rE = __builtin_amdgcn_rcpf(E);
+        const float dfs = fe * p2 * W * (smode_u == 2 ? P.ps0 * rE : (smode_u == 1 ? P.ps0 : 0.f));
+        const float dfb = fe * (bulk_inv ? P.qi * rE + P.q1 * E : P.qi + E * (P.q1 + P.q2 * E));
         const float w2 = clamp01(0.5f * (1.f + dfs + dfb));
         const float racc = fdiv(fexp(G - (ck + sk * tau)) * w1 * w2, (1.f + kk * W) * (1.f + bk * y));
         if (racc > 1.f + 1e-5f) n_viol++;
@@ -1079,7 +1116,10 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
             const float px = p * (R00 * n0 + R01 * n1 + R02 * n2);
             const float py = p * (R10 * n0 + R11 * n1 + R12 * n2);
             const float pz = p * (R20 * n0 + R21 * n1 + R22 * n2);
-            if (a.ecell) emit_cell_lab(a, eu, ep, pid, P.m, px, py, pz, E, outp);
+            if (a.ecell) {
+                const float4* ec = a.ecell + 2 * static_cast<int64_t>(c);
+                emit_cell_lab(a, ec[0], ec[1], a.sp.pid[s], P.m, px, py, pz, E, outp);
+            }
             else emit_lab(a, chunk, c, jj, a.sp.pid[s], P.m, px, py, pz, E, outp);
             done = true;
         } else if (it >= max_it) {
@@ -1089,18 +1129,18 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         }
         if (done) active = false;
     }
-    n_it = wave_sum(n_it);
-    n_fail = wave_sum(n_fail);
-    n_range = wave_sum(n_range);
-    n_fb = wave_sum(n_fb);
-    n_viol = wave_sum(n_viol);
+    const unsigned long long t_it = wave_sum(static_cast<unsigned long long>(n_it));
+    const unsigned long long t_fail = wave_sum(static_cast<unsigned long long>(n_fail));
+    const unsigned long long t_range = wave_sum(static_cast<unsigned long long>(n_range));
+    const unsigned long long t_fb = wave_sum(static_cast<unsigned long long>(n_fb));
+    const unsigned long long t_viol = wave_sum(static_cast<unsigned long long>(n_viol));
     if (lane == 0) {
-        atomicAdd(&s_stats[0], n_it);
-        atomicAdd(&s_stats[1], n_it);
-        atomicAdd(&s_stats[2], n_fail);
-        atomicAdd(&s_stats[3], n_range);
-        atomicAdd(&s_stats[4], n_fb);
-        atomicAdd(&s_stats[5], n_viol);
+        atomicAdd(&s_stats[0], t_it);
+        atomicAdd(&s_stats[1], t_it);
+        atomicAdd(&s_stats[2], t_fail);
+        atomicAdd(&s_stats[3], t_range);
+        atomicAdd(&s_stats[4], t_fb);
+        atomicAdd(&s_stats[5], t_viol);
     }
     __syncthreads();
     if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);

@@ -427,9 +427,6 @@ __global__ __launch_bounds__(kYieldThreads) void k_yield(YieldArgs a, const Spec
 __global__ void k_clear_flag(int* flag) {
     if (threadIdx.x == 0) *flag = 0;
 }
-__global__ void k_zero_u64(unsigned long long* p) {
-    if (threadIdx.x == 0) *p = 0ull;
-}
 
 __global__ void k_species_reduce(const double* __restrict__ partial, int nblk, int Ns,
                                  double* __restrict__ out) {

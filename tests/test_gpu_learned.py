@@ -147,6 +147,8 @@ def trained():
     ctx.sample(nev, seed=11)
     out["plain"] = (ctx.hadrons().copy(), ctx.event_counts().copy(), ctx.stats().as_dict())
     out["n_train"] = ctx.mlp_collect(2_000_000)
+    # the network's gain is measured with every pair on the learned proposal
+    ctx.set_learned_routing(1)
     ctx.mlp_init(0)
     ctx.sample(nev, seed=11, proposal=1)
     out["zero"] = (ctx.hadrons().copy(), ctx.event_counts().copy(), ctx.stats().as_dict())
@@ -156,11 +158,18 @@ def trained():
     out["learned"] = (ctx.hadrons().copy(), ctx.event_counts().copy(), ctx.stats().as_dict())
     ctx.sample(nev, seed=11, proposal=1)
     out["learned2"] = ctx.hadrons().copy()
+    # the default: pairs of one hadron take the plain sampler
+    ctx.set_learned_routing(2)
+    ctx.sample(nev, seed=11, proposal=1)
+    out["routed"] = (ctx.hadrons().copy(), ctx.event_counts().copy(), ctx.stats().as_dict())
+    ctx.sample(nev, seed=11, proposal=1)
+    out["routed2"] = ctx.hadrons().copy()
     print("train", out["report"], "set", out["n_train"])
-    for k in ("plain", "zero", "learned"):
+    for k in ("plain", "zero", "learned", "routed"):
         st = out[k][2]
         print(k, "hadrons", st["n_hadrons"], "proposals/hadron %.3f" % (st["n_inner"] / st["n_hadrons"]),
-              "fallback", st["n_fallback"], "violations", st["n_violations"], "ms %.3f" % st["ms_sample_kernel"])
+              "fallback", st["n_fallback"], "routed", st["n_routed"], "pairs", st["n_pairs"],
+              "violations", st["n_violations"], "ms %.3f" % st["ms_sample_kernel"])
     return out
 
 
@@ -170,15 +179,16 @@ def test_training_lowers_envelope_mass(trained):
     assert rep["loss_last"] < rep["loss_first"] - 0.02
 
 
-def test_learned_is_exact_and_deterministic(trained):
+@pytest.mark.parametrize("which", ["learned", "routed"])
+def test_learned_is_exact_and_deterministic(trained, which):
     hp, cp, sp_ = trained["plain"]
-    hl, cl, sl = trained["learned"]
+    hl, cl, sl = trained[which]
     assert sl["n_violations"] == 0 and trained["zero"][2]["n_violations"] == 0
     assert sl["n_fallback"] == 0 and sl["n_failed"] == 0
     # the same species draws and event assignment as the plain sampler
     assert np.array_equal(cp, cl)
     assert np.array_equal(np.sort(hp["pid"]), np.sort(hl["pid"]))
-    assert np.array_equal(hl, trained["learned2"])
+    assert np.array_equal(hl, trained[which + "2"])
     # momenta: a different, independent sample of the same distribution
     tot_chi, tot_ndf = 0.0, 0
     nev = trained["nev"]
@@ -209,6 +219,25 @@ def test_learned_is_exact_and_deterministic(trained):
         tot_ndf += ndf
     print("combined chi2/ndf %.4f ndf %d" % (tot_chi / tot_ndf, tot_ndf))
     assert tot_chi / tot_ndf <= 1.1
+
+
+def test_routing_counts_single_hadron_pairs(trained):
+    """Routing sends exactly the (cell, species) pairs of one hadron in the
+    pass to the plain sampler: recounted from the output, where a cell is
+    its position (unique on this surface) and a species its pid."""
+    h, _, st = trained["routed"]
+    _, _, st1 = trained["learned"]
+    key = np.stack([h["x"], h["y"], h["z"], h["t"]], axis=1)
+    _, cell_of = np.unique(key, axis=0, return_inverse=True)
+    pair = np.stack([cell_of.ravel().astype(np.int64), h["pid"].astype(np.int64)], axis=1)
+    _, mult = np.unique(pair, axis=0, return_counts=True)
+    print("pairs", mult.size, "of one hadron", int(np.sum(mult == 1)), "routed", st["n_routed"],
+          "learned pairs", st["n_pairs"], "(all pairs without routing:", st1["n_pairs"], ")")
+    assert st1["n_routed"] == 0 and st1["n_pairs"] == mult.size
+    assert st["n_routed"] == np.sum(mult == 1)
+    assert st["n_pairs"] == np.sum(mult >= 2)
+    # the routed hadrons go through the plain sampler's set-up, not the fallback count
+    assert st["n_fallback"] == 0
 
 
 def test_learned_needs_fewer_proposals(trained):
