@@ -1028,6 +1028,8 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
                     BW[0] = h5.x; BW[1] = h5.y; BW[2] = h5.z;
                     lparams_fast(a, L.aux, c, s, __uint_as_float(h5.w), P);
                     P.qi += P.q0;  // (one of the two is zero: qa of the bulk form)
+                    P.D = P.D > 0.f ? __builtin_amdgcn_rcpf(P.D) : 0.f;  // (its reciprocal)
+
                     it = 0;
                     active = true;
                 }
@@ -1076,7 +1078,10 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         const float p = __builtin_amdgcn_sqrtf(p2);
         const float et = fexp(-t);
         const float fe = fdiv(1.f, 1.f + P.sgn * et);     // e^t f0 = 1 - sign f0
-        const float G = flog(p * E * fe) - tau;
+        // the target over the envelope line, exp(log(p E fe) - tau - (c + s tau)),
+        // as a product (the log envelope's intercept c is an fp16 of magnitude
+        // below 6e4, so the exponent is finite; p = 0 gives 0)
+        const float pEf = p * E * fe;
         // direction in the shear eigenframe
         const float P0 = (1.f + kk * P.l1) * (1.f / 3.f), P1 = P0 + (1.f + kk * P.l2) * (1.f / 3.f);
         const float uy = u24(w1r.z);
@@ -1099,13 +1104,12 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         if (uf >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }
         const float W = P.l1 * n0 * n0 + P.l2 * n1 * n1 + P.l3 * n2 * n2;
         // the exact Cooper-Frye weights of the plain sampler
-        const float Dn = fabsf(P.d0) + P.dn;
-        const float w1 = (Dn > 0.f) ? clamp01(fdiv(E * P.d0 + p * P.dn * y, E * Dn)) : 0.f;
         const float rE = __builtin_amdgcn_rcpf(E);
+        const float w1 = clamp01((P.d0 + p * rE * P.dn * y) * P.D);
         const float dfs = fe * p2 * W * (smode_u == 2 ? P.ps0 * rE : (smode_u == 1 ? P.ps0 : 0.f));
         const float dfb = fe * (bulk_inv ? P.qi * rE + P.q1 * E : P.qi + E * (P.q1 + P.q2 * E));
         const float w2 = clamp01(0.5f * (1.f + dfs + dfb));
-        const float racc = fdiv(fexp(G - (ck + sk * tau)) * w1 * w2, (1.f + kk * W) * (1.f + bk * y));
+        const float racc = fdiv(pEf * fexp(-(tau + ck + sk * tau)) * w1 * w2, (1.f + kk * W) * (1.f + bk * y));
         if (racc > 1.f + 1e-5f) n_viol++;
         bool done = false;
         const uint32_t ua = (w1r.y & 0xFFu) | ((w1r.z & 0xFFu) << 8) | ((w1r.w & 0xFFu) << 16);

@@ -605,10 +605,23 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
 // depend on the launch shape.
 // idle lanes that trigger a refill, unless the autotuner chose otherwise
 constexpr int kRefillMin = 16;
+// Every kCoarse-th row of the two |p| tables below t = 30 (Bose-Einstein 0,
+// Fermi-Dirac 3), which nearly every hadron uses, is copied into LDS (e0, e1,
+// e2 per row, 12 KB) by each workgroup: the search narrows the bracket among
+// those rows first, then ends among at most kCoarse rows in global memory
+// (two dependent loads instead of about seven).
+constexpr int kCoarse = 8;
+constexpr int kCoarseRows = (kPTabLen + kCoarse - 1) / kCoarse;
 
 __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) {
     __shared__ unsigned long long s_stats[4];
+    __shared__ double s_ctab[2][kCoarseRows * 3];
     if (threadIdx.x < 4) s_stats[threadIdx.x] = 0ull;
+    for (int i = threadIdx.x; i < 2 * kCoarseRows * 3; i += blockDim.x) {
+        const int t = i / (kCoarseRows * 3), g = (i / 3) % kCoarseRows, k = i % 3;
+        const int tbl = t == 0 ? 0 : 3, row = g * kCoarse;
+        s_ctab[t][g * 3 + k] = row < a.pt.len[tbl] ? a.pt.C[(static_cast<int64_t>(tbl) * kPTabLen + row) * 4 + k] : 0.;
+    }
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
@@ -725,10 +738,38 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
         // search issues its three loads together
         int lo = i0, hi = imax;
         double rlo = rmin0, rhi = cmax;
+        auto comb_of = [&](double e0, double e1, double e2) { return e2 + 2. * q.mut * e1 + q.aa * e0 - mterm; };
         auto comb = [&](int qq) {
             const double* e = C + 4 * qq;
-            return e[2] + 2. * q.mut * e[1] + q.aa * e[0] - mterm;
+            return comb_of(e[0], e[1], e[2]);
         };
+        if (tb == 0 || tb == 3) {
+            // LDS rows g kCoarse strictly inside (lo, hi): g in [ga, gb]; find
+            // L < H with comb(L kCoarse) <= r < comb(H kCoarse), L = ga - 1
+            // and H = gb + 1 standing for lo and hi
+            const double* Cc = s_ctab[tb == 0 ? 0 : 1];
+            auto combc = [&](int g) { return comb_of(Cc[3 * g], Cc[3 * g + 1], Cc[3 * g + 2]); };
+            const int ga = lo / kCoarse + 1, gb = (hi - 1) / kCoarse;
+            int L = ga - 1, H = gb + 1;
+            double vl = rlo, vh = rhi;
+            while (H - L > 4) {
+                const int qd = (H - L) >> 2;
+                const int m1 = L + qd, m2 = L + 2 * qd, m3 = L + 3 * qd;
+                const double v1 = combc(m1), v2 = combc(m2), v3 = combc(m3);
+                if (r < v2) {
+                    if (r < v1) { H = m1; vh = v1; } else { L = m1; vl = v1; H = m2; vh = v2; }
+                } else {
+                    if (r < v3) { L = m2; vl = v2; H = m3; vh = v3; } else { L = m3; vl = v3; }
+                }
+            }
+            while (H - L > 1) {
+                const int mid = (H + L) >> 1;
+                const double vm = combc(mid);
+                if (r < vm) { H = mid; vh = vm; } else { L = mid; vl = vm; }
+            }
+            if (L >= ga) { lo = L * kCoarse; rlo = vl; }
+            if (H <= gb) { hi = H * kCoarse; rhi = vh; }
+        }
         while (hi - lo > 3) {
             const int qd = (hi - lo) >> 2;
             const int m1 = lo + qd, m2 = lo + 2 * qd, m3 = lo + 3 * qd;
