@@ -101,13 +101,17 @@ struct DBuf {
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
+    // grows with 1/8 headroom above a previous allocation, so that sizes
+    // that vary from pass to pass (hadron counts) do not reallocate each
+    // time a new maximum comes up
     hipError_t ensure(size_t n) {
         if (n <= cap && p) return hipSuccess;
+        const size_t want = cap > 0 ? n + n / 8 : n;
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) cap = n;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) cap = want;
         return e;
     }
     void release() {

@@ -1064,8 +1064,15 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         const float kk = (k < kBands - 1) ? h2f(static_cast<uint16_t>((kwk >> ((k & 1) * 16)) & 0xFFFFu)) : 0.f;
         const float bk = static_cast<float>((bwk >> ((k & 3) * 8)) & 0xFFu) * (1.f / 255.f);
         const float tau_max = P.tmax - P.x;
-        const float ba = fminf(band_edge(k), tau_max);
-        const float bb = (k < kBands - 1) ? fminf(band_edge(k + 1), tau_max) : tau_max;
+        // the band's edges by selects (a switch on the lane's band would branch)
+        float ea = 0.f, eb = band_edge(1);
+#pragma unroll
+        for (int j = 1; j < kBands; j++) {
+            ea = k >= j ? band_edge(j) : ea;
+            eb = k >= j ? band_edge(j + 1) : eb;
+        }
+        const float ba = fminf(ea, tau_max);
+        const float bb = (k < kBands - 1) ? fminf(eb, tau_max) : tau_max;
         const float bw = bb - ba;
         // tau from exp(s tau) on [ba, bb]
         const float ut = u24(w1r.y);
@@ -1085,20 +1092,22 @@ __global__ __launch_bounds__(256) void k_learn_trials(LearnArgs L, const LRec* _
         // direction in the shear eigenframe
         const float P0 = (1.f + kk * P.l1) * (1.f / 3.f), P1 = P0 + (1.f + kk * P.l2) * (1.f / 3.f);
         const float uy = u24(w1r.z);
-        int ax;
-        float rr;
-        if (uy < P0) { ax = 0; rr = fdiv(uy, P0); }
-        else if (uy < P1) { ax = 1; rr = fdiv(uy - P0, P1 - P0); }
-        else { ax = 2; rr = fdiv(uy - P1, 1.f - P1); }
+        // the axis with probabilities P0, P1 - P0, 1 - P1 and the position
+        // within its interval (one division for every lane)
+        const bool ax0 = uy < P0, ax1 = !ax0 && uy < P1;
+        const int ax = ax0 ? 0 : (ax1 ? 1 : 2);
+        const float ylo = ax0 ? 0.f : (ax1 ? P0 : P1), yhi = ax0 ? P0 : (ax1 ? P1 : 1.f);
+        float rr = fdiv(uy - ylo, yhi - ylo);
         rr = fminf(fmaxf(rr, 0.f), 1.f);
         const float cz = fcbrt(2.f * rr - 1.f);
         const float sz = __builtin_amdgcn_sqrtf(fmaxf(1.f - cz * cz, 0.f));
         float sph, cph;
         __sincosf(2.f * static_cast<float>(kPi) * u24(w1r.w), &sph, &cph);
-        float n0, n1, n2;
-        if (ax == 0) { n0 = cz; n1 = sz * cph; n2 = sz * sph; }
-        else if (ax == 1) { n1 = cz; n2 = sz * cph; n0 = sz * sph; }
-        else { n2 = cz; n0 = sz * cph; n1 = sz * sph; }
+        // (cz along the axis, then sz cos, sz sin along the next two, cyclically)
+        const float nc = sz * cph, ns = sz * sph;
+        float n0 = ax == 0 ? cz : (ax == 1 ? ns : nc);
+        float n1 = ax == 0 ? nc : (ax == 1 ? cz : ns);
+        float n2 = ax == 0 ? ns : (ax == 1 ? nc : cz);
         float y = n0 * P.dh0 + n1 * P.dh1 + n2 * P.dh2;
         const float uf = (static_cast<float>(w1r.x & 0xFFFFu) + 0.5f) * (1.f / 65536.f);
         if (uf >= 0.5f * (1.f + bk * y)) { n0 = -n0; n1 = -n1; n2 = -n2; y = -y; }

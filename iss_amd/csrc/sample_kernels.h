@@ -607,10 +607,10 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
 constexpr int kRefillMin = 16;
 // Every kCoarse-th row of the two |p| tables below t = 30 (Bose-Einstein 0,
 // Fermi-Dirac 3), which nearly every hadron uses, is copied into LDS (e0, e1,
-// e2 per row, 12 KB) by each workgroup: the search narrows the bracket among
+// e2 per row, 24 KB) by each workgroup: the search narrows the bracket among
 // those rows first, then ends among at most kCoarse rows in global memory
 // (two dependent loads instead of about seven).
-constexpr int kCoarse = 8;
+constexpr int kCoarse = 4;
 constexpr int kCoarseRows = (kPTabLen + kCoarse - 1) / kCoarse;
 
 __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) {
@@ -639,7 +639,8 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     const int max_it = a.max_trials * 64;
 
     int64_t next = beg;
-    unsigned long long n_trial = 0, n_inner = 0, n_fail = 0, n_range = 0;
+    // per-lane counters (32 bits: a lane's share of one launch), summed in 64
+    uint32_t n_trial = 0, n_inner = 0, n_fail = 0, n_range = 0;
 
     bool active = false;
     int64_t c = 0;
@@ -852,15 +853,15 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
         }
         if (done) active = false;
     }
-    n_trial = wave_sum(n_trial);
-    n_inner = wave_sum(n_inner);
-    n_fail = wave_sum(n_fail);
-    n_range = wave_sum(n_range);
+    const unsigned long long t_trial = wave_sum(static_cast<unsigned long long>(n_trial));
+    const unsigned long long t_inner = wave_sum(static_cast<unsigned long long>(n_inner));
+    const unsigned long long t_fail = wave_sum(static_cast<unsigned long long>(n_fail));
+    const unsigned long long t_range = wave_sum(static_cast<unsigned long long>(n_range));
     if (lane == 0) {
-        atomicAdd(&s_stats[0], n_trial);
-        atomicAdd(&s_stats[1], n_inner);
-        atomicAdd(&s_stats[2], n_fail);
-        atomicAdd(&s_stats[3], n_range);
+        atomicAdd(&s_stats[0], t_trial);
+        atomicAdd(&s_stats[1], t_inner);
+        atomicAdd(&s_stats[2], t_fail);
+        atomicAdd(&s_stats[3], t_range);
     }
     __syncthreads();
     if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
