@@ -372,7 +372,7 @@ double host_pcdf(int fermion, int quirk, int trunc, double m, int k, double E) {
 
 int build_ptables(iss_ctx* ctx, int quirk) {
     const double m0s[6] = {0.05, 30., 50., 0., 30., 50.};
-    const int tr[6] = {10, 2, 1, 10, 2, 1};
+    const int tr[6] = {10, 2, 1, 10, 2, 1};  // (at most kMaxTrunc terms on the device)
     std::vector<double> h(6ull * kPTabLen * 4, 0.);
     PTabDev& pt = ctx->pt;
     pt.quirk = quirk;

@@ -264,8 +264,17 @@ __device__ __forceinline__ void emit_cell_lab(const SampleArgs& a, float4 eu, fl
 // CDF_0, CDF_1, CDF_2 of one |p| table at E~ together: the per-term
 // exponentials become powers of exp(m0 - E~) and exp(-m0), so one hadron's
 // normalisation costs two exponentials and a logarithm.
+// 1/n, 1/n^2 and 1/n^3 of the series terms n = 1..kMaxTrunc: folded
+// constants are correctly rounded, so s / n^k (s = +-1, n^k exact) equals
+// s * kInvPow[k-1][n-1] bit for bit, without FP64 divisions
+constexpr int kMaxTrunc = 10;
+__constant__ double kInvPow[3][kMaxTrunc] = {
+    {1. / 1, 1. / 2, 1. / 3, 1. / 4, 1. / 5, 1. / 6, 1. / 7, 1. / 8, 1. / 9, 1. / 10},
+    {1. / 1, 1. / 4, 1. / 9, 1. / 16, 1. / 25, 1. / 36, 1. / 49, 1. / 64, 1. / 81, 1. / 100},
+    {1. / 1, 1. / 8, 1. / 27, 1. / 64, 1. / 125, 1. / 216, 1. / 343, 1. / 512, 1. / 729, 1. / 1000}};
+
 __device__ void pcdf_all(const PTabDev& pt, int tb, double E, double out[3]) {
-    const int fer = pt.fermion[tb], trunc = pt.trunc[tb], quirk = pt.quirk;
+    const int fer = pt.fermion[tb], trunc = min(pt.trunc[tb], kMaxTrunc), quirk = pt.quirk;
     const double m = pt.m0[tb];
     const double r = exp(m - E), em1 = pt.em1[tb];
     double c0 = 0., c1 = 0., c2 = 0., sign = 1., em = 1., ee = r;
@@ -273,11 +282,11 @@ __device__ void pcdf_all(const PTabDev& pt, int tb, double E, double out[3]) {
         const int n1 = n + 1;
         const double dn = static_cast<double>(n1);
         if (trunc <= 5)
-            c0 += (fer ? static_cast<double>(static_cast<int>(sign) / n1) : 1. / dn) * em * (1. - ee);
+            c0 += (fer ? static_cast<double>(static_cast<int>(sign) / n1) : kInvPow[0][n]) * em * (1. - ee);
         const double s1 = fer ? sign : 1.;
-        c1 += s1 / (dn * dn) * em * ((m * dn + 1) - ee * (E * dn + 1));
+        c1 += s1 * kInvPow[1][n] * em * ((m * dn + 1) - ee * (E * dn + 1));
         const double s2 = (fer && !quirk) ? sign : 1.;
-        c2 += s2 / (dn * dn * dn) * em * ((m * dn * (m * dn + 2) + 2) - ee * (E * dn * (E * dn + 2) + 2));
+        c2 += s2 * kInvPow[2][n] * em * ((m * dn * (m * dn + 2) + 2) - ee * (E * dn * (E * dn + 2) + 2));
         sign = -sign;
         em *= em1;
         ee *= r;
