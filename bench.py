@@ -258,8 +258,9 @@ def run_batched(args):
     if not args.no_tune:
         ctx.compute_yields(precision=args.precision)
         tune["plain"] = ctx.autotune(proposal=0, n_events=2, seed=7, repeats=1)
-        small = np.maximum(ev // 10, 1).astype(np.int32)
-        tune["batch_plain"] = ctx.autotune_batch(small, seed=7, precision=args.precision, proposal=0)
+        # (whole batches of the workload: the round size and the learned
+        # routing depend on the hadrons per (cell, species) pair)
+        tune["batch_plain"] = ctx.autotune_batch(ev, seed=7, precision=args.precision, proposal=0)
     # the network, trained on ~1e7 hadrons accepted by a plain pass over the
     # whole record (outside the timed region)
     per_event = float(ctx.compute_yields(precision=args.precision).sum())
@@ -270,8 +271,7 @@ def run_batched(args):
     rep = ctx.mlp_train(batch=8192, steps=args.train_steps, lr=3e-3, seed=5)
     if not args.no_tune:
         tune["learned"] = ctx.autotune(proposal=1, n_events=2, seed=7, repeats=1)
-        tune["batch_learned"] = ctx.autotune_batch(np.maximum(ev // 10, 1).astype(np.int32), seed=7,
-                                                   precision=args.precision, proposal=1)
+        tune["batch_learned"] = ctx.autotune_batch(ev, seed=7, precision=args.precision, proposal=1)
 
     def run(proposal):
         def step(i):
@@ -347,6 +347,7 @@ def run_batched(args):
         "envelope_violations": learned["violations"],
         "fallback_hadrons": learned["fallback"],
         "routed_hadrons_per_step": learned["routed"] / args.steps,
+        "routing_min_hadrons": ctx.learned_routing(),
         "training": rep,
         "autotune": tune,
     }
@@ -401,14 +402,16 @@ def main():
     tune = {}
     if not args.no_tune:
         ctx.compute_yields(precision=args.precision)
-        tune["plain"] = ctx.autotune(proposal=0, n_events=args.events // 4, seed=7, repeats=1)
+        # (passes of the workload's size: the learned routing depends on the
+        # hadrons per (cell, species) pair of a pass)
+        tune["plain"] = ctx.autotune(proposal=0, n_events=args.events, seed=7, repeats=1)
     plain = run_steps(ctx, args, 0, 0)
     # training set: the hadrons accepted by the last plain pass
     n_train = ctx.mlp_collect(10_000_000)
     ctx.mlp_init(1)
     rep = ctx.mlp_train(batch=8192, steps=args.train_steps, lr=3e-3, seed=5)
     if not args.no_tune:
-        tune["learned"] = ctx.autotune(proposal=1, n_events=args.events // 4, seed=7, repeats=1)
+        tune["learned"] = ctx.autotune(proposal=1, n_events=args.events, seed=7, repeats=1)
     learned = run_steps(ctx, args, 1, (args.warmup + args.steps) * args.events)
 
     v_learn = learned["hadrons"] / learned["elapsed"]
@@ -461,6 +464,7 @@ def main():
         "learned_pairs_per_step": pairs,
         "hadrons_per_pair": (hl - routed) / max(pairs, 1.0),
         "routed_hadrons_per_step": routed,
+        "routing_min_hadrons": ctx.learned_routing(),
         "fallback_hadrons": learned["fallback"],
         "training": rep,
         "autotune": tune,

@@ -210,6 +210,8 @@ typedef struct {
     double ms_best;          /* trial-kernel time of the chosen shape */
     double ms_default;       /* trial-kernel time of the default shape */
     int32_t n_tried;
+    int32_t learned_min_hadrons;  /* learned proposal: the routing threshold chosen on whole
+                                     sampling passes (iss_set_learned_routing); 0 for plain */
 } iss_tune_report;
 
 int iss_autotune(iss_ctx* ctx, const iss_tune_opts* opts, iss_tune_report* report);
@@ -229,11 +231,13 @@ int iss_mlp_set_params(iss_ctx* ctx, const float* params, int n);
 int iss_mlp_get_params(iss_ctx* ctx, float* params, int n);
 /* Routing of the learned passes (iss_sample, iss_sample_batch): a (cell,
  * species) pair with fewer than min_hadrons hadrons in the pass takes the
- * plain sampler, whose per-hadron set-up costs less than one network
- * evaluation and envelope.  1: every pair takes the learned proposal;
- * 2 (the default): the pairs of one hadron take the plain sampler
- * (iss_stats.n_routed counts their hadrons).  The output law is the same. */
+ * plain sampler, whose per-hadron set-up and trials cost less than one
+ * network evaluation and envelope shared by few hadrons.  1: every pair
+ * takes the learned proposal; default 4; at most 64.  iss_stats.n_routed
+ * counts the routed hadrons; iss_autotune (proposal 1) picks the threshold
+ * on timed passes.  The output law is the same for every threshold. */
 int iss_set_learned_routing(iss_ctx* ctx, int32_t min_hadrons);
+int iss_get_learned_routing(const iss_ctx* ctx);
 
 typedef struct {
     int64_t batch;           /* samples per step, a multiple of 64 */
@@ -326,8 +330,10 @@ int iss_batch_rounds(const iss_ctx* ctx);
 int iss_batch_events_per_round(const iss_ctx* ctx);
 double iss_batch_ms(const iss_ctx* ctx);   /* device time of the last batch (all rounds) */
 int iss_get_batch_layout(iss_ctx* ctx, int64_t* segments, int64_t* event_sizes);
-/* times whole batches for R in {4, 8, 16, 32, 64}, keeps the fastest as the
- * default events_per_round; report: refill_min = R chosen, ms_* batch times */
+/* times whole batches for R in {4, 16, 64, 256, 1024, 4096}, keeps the
+ * fastest as the default events_per_round of that proposal (the learned one
+ * also times its routing thresholds at that R); report: refill_min = R
+ * chosen, ms_* batch times, learned_min_hadrons */
 int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* opts, const int32_t* n_events, iss_tune_report* report);
 
 /* ---- resonance decays: particle_decay (src/particle_decay.cpp:33-533) driven

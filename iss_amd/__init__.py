@@ -102,7 +102,7 @@ class _TuneOpts(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("refill_min", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms_best", C.c_double),
-                ("ms_default", C.c_double), ("n_tried", C.c_int32)]
+                ("ms_default", C.c_double), ("n_tried", C.c_int32), ("learned_min_hadrons", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -156,6 +156,7 @@ _SIGNATURES = {
     "iss_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "iss_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "iss_set_learned_routing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "iss_get_learned_routing": (C.c_int, [C.c_void_p]),
     "iss_mlp_collect": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "iss_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(_TrainOpts), C.POINTER(TrainReport)]),
     "iss_mlp_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -370,10 +371,13 @@ class Context:
         self._check(self.lib.iss_mlp_get_params(self.h, _ptr(out), out.size), "iss_mlp_get_params")
         return out
 
-    def set_learned_routing(self, min_hadrons: int = 2):
-        """Pairs with fewer hadrons than min_hadrons (1 or 2) in a learned
-        pass take the plain sampler."""
+    def set_learned_routing(self, min_hadrons: int = 4):
+        """(cell, species) pairs with fewer hadrons than min_hadrons (1..64)
+        in a learned pass take the plain sampler."""
         self._check(self.lib.iss_set_learned_routing(self.h, min_hadrons), "iss_set_learned_routing")
+
+    def learned_routing(self) -> int:
+        return int(self.lib.iss_get_learned_routing(self.h))
 
     def mlp_collect(self, max_samples: int = 10_000_000) -> int:
         n = C.c_int64()

@@ -149,6 +149,14 @@ struct PinBuf {
 };
 constexpr size_t PIN_OOB = 0, PIN_K = 8, PIN_STATS = 64, PIN_AUX = 128, PIN_TOT = 256;
 
+// Learned passes: (cell, species) pairs of fewer hadrons than the routing
+// threshold take the plain sampler.  A pair's network and envelope cost
+// about as much as one plain hadron's set-up and trials, and each learned
+// hadron saves about 40 % of those, so pairs of fewer than about four
+// hadrons are cheaper on the plain sampler; the autotuner times the choice.
+constexpr int kLearnRouteDefault = 4;
+constexpr int kLearnRouteMax = 64;
+
 // one generation of the resonance feed-down: its particles and per-particle
 // bookkeeping (kept by the context and reused across calls)
 struct DecayGen {
@@ -260,8 +268,8 @@ struct iss_ctx {
     DBuf<uint2> d_pkey;
     DBuf<uint32_t> d_lbits, d_lcnt, d_lscnt;
     DBuf<int64_t> d_pairoff, d_soff;
-    // pairs of fewer hadrons than this take the plain sampler (1 or 2)
-    int learn_route = 2;
+    // pairs of fewer hadrons than this take the plain sampler (1: none)
+    int learn_route = kLearnRouteDefault;
     DBuf<uint16_t> d_sp16;
     DBuf<LSlot> d_lslot;
     DBuf<unsigned long long> d_fb_count;
@@ -1527,13 +1535,13 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         uint32_t* gc = learned ? ctx->d_lcnt.p : nullptr;
         uint32_t* gs = route ? ctx->d_lscnt.p : nullptr;
         if (f64 && learned)
-            k_pick<double, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
+            k_pick<double, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         else if (f64)
-            k_pick<double, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
+            k_pick<double, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         else if (learned)
-            k_pick<float, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
+            k_pick<float, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         else
-            k_pick<float, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs);
+            k_pick<float, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         HIPCHK(hipGetLastError());
     }
     int per_cu_trials = 0;
@@ -1740,12 +1748,33 @@ int iss_autotune(iss_ctx* ctx, const iss_tune_opts* o, iss_tune_report* rep) {
         }
     ctx->refill_min[p] = best_rf;
     ctx->blocks_per_cu[p] = best_bp;
+    // learned proposal: the routing threshold, on the whole sampling pass
+    // (pair bookkeeping, set-up and trials of both samplers)
+    int best_route = ctx->learn_route;
+    if (p == 1) {
+        const int routes[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        double best_pass = 1e30;
+        for (int rt : routes) {
+            ctx->learn_route = rt;
+            double ms = 1e30;
+            for (int r = 0; r < o->repeats; r++) {
+                int64_t n = 0;
+                rc = iss_sample(ctx, &so, &n);
+                if (rc) return rc;
+                ms = std::min(ms, ctx->stats.ms_sample_kernel);
+            }
+            tried++;
+            if (ms < best_pass) { best_pass = ms; best_route = rt; }
+        }
+        ctx->learn_route = best_route;
+    }
     if (rep) {
         rep->refill_min = best_rf;
         rep->blocks_per_cu = best_bp;
         rep->ms_best = best;
         rep->ms_default = ms_def;
         rep->n_tried = tried;
+        rep->learned_min_hadrons = p == 1 ? best_route : 0;
     }
     return ISS_OK;
 }
@@ -1754,11 +1783,14 @@ int iss_mlp_num_params(void) { return kNumParams; }
 
 int iss_set_learned_routing(iss_ctx* ctx, int32_t min_hadrons) {
     if (!ctx) return ISS_ERR_ARG;
-    if (min_hadrons != 1 && min_hadrons != 2)
-        return set_err(ctx, ISS_ERR_ARG, "learned routing: min_hadrons must be 1 or 2");
+    if (min_hadrons < 1 || min_hadrons > kLearnRouteMax)
+        return set_err(ctx, ISS_ERR_ARG, "learned routing: min_hadrons must be in [1, " +
+                                             std::to_string(kLearnRouteMax) + "]");
     ctx->learn_route = min_hadrons;
     return ISS_OK;
 }
+
+int iss_get_learned_routing(const iss_ctx* ctx) { return ctx ? ctx->learn_route : ISS_ERR_ARG; }
 
 int iss_mlp_init(iss_ctx* ctx, uint64_t seed) {
     if (!ctx) return ISS_ERR_ARG;
@@ -2203,7 +2235,8 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         } else {
             const int pblk = static_cast<int>(std::max<int64_t>(1, (N + kPickCells - 1) / kPickCells));
             k_batch_pick<<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, ctx->d_sp16.p, ctx->d_lbits.p,
-                                                                       ctx->d_lcnt.p, route ? ctx->d_lscnt.p : nullptr);
+                                                                       ctx->d_lcnt.p, route ? ctx->d_lscnt.p : nullptr,
+                                                                       ctx->learn_route);
             k_scan_tiles<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p);
             k_scan_top<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan.p, ntiles);
             k_scan_apply<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p, ctx->d_pairoff.p);
@@ -2350,12 +2383,32 @@ int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_e
         if (t < best) { best = t; best_R = R; }
     }
     ctx->batch_R_tuned[learned ? 1 : 0] = best_R;
+    // learned proposal: the routing threshold at the chosen round size
+    int best_route = ctx->learn_route;
+    if (learned) {
+        const int routes[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        bo.events_per_round = best_R;
+        for (int rt : routes) {
+            ctx->learn_route = rt;
+            double t = 1e30;
+            for (int rep_i = 0; rep_i < 2; rep_i++) {
+                int64_t n = 0;
+                int rc = iss_sample_batch(ctx, &bo, n_events, &n);
+                if (rc) return rc;
+                t = std::min(t, ctx->batch_ms);
+            }
+            tried++;
+            if (t < best) { best = t; best_route = rt; }
+        }
+        ctx->learn_route = best_route;
+    }
     if (rep) {
         rep->refill_min = best_R;   // events per round
         rep->blocks_per_cu = ctx->blocks_per_cu[0];
         rep->ms_best = best;
         rep->ms_default = ms_def;
         rep->n_tried = tried;
+        rep->learned_min_hadrons = learned ? best_route : 0;
     }
     return ISS_OK;
 }

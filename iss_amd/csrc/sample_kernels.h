@@ -383,34 +383,38 @@ __global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ 
 // species of zero yield is never taken (the nearest species of positive
 // yield replaces one that the roundings of the scan leave an interval of
 // measure ~1e-16).  Writes each slot's species and cell; with PAIRS also the
-// cell's species bitmap and its number of distinct species (the learned
-// sampler's (cell, species) pairs; with scnt, routing, mark_species).  Launch with ceil(Nc / kPickCells)
-// workgroups of kPickCells waves.
+// cell's pair bitmap and its number of pairs (the learned sampler's (cell,
+// species) pairs, count_pairs).  Launch with ceil(Nc / kPickCells) workgroups
+// of kPickCells waves.
 constexpr int kPickCells = 4;
 constexpr int kPickWords = kMaxSpecies / 32;
 
-// The learned sampler's pairs of one cell, from the species of its slots
-// (marked into `bits` as the slots are drawn, `multi` holding the species met
-// twice).  With scnt (routing) a pair is a species with at least two slots
-// and the slots of the species met once are counted in scnt[c]: they take the
-// plain sampler, whose per-hadron set-up costs less than the network and
-// envelope of a pair of one hadron.  Without it every species is a pair.
-__device__ __forceinline__ void mark_species(uint32_t* bits, uint32_t* multi, int sp, bool route) {
-    const uint32_t bit = 1u << (sp & 31);
-    const uint32_t old = atomicOr(&bits[sp >> 5], bit);
-    if (route && (old & bit)) atomicOr(&multi[sp >> 5], bit);
-}
-__device__ __forceinline__ void write_pairs(const uint32_t* bits, const uint32_t* multi, int64_t c, int Wb,
+// The learned sampler's pairs of one cell from its slots' species, counted
+// per species in LDS (`hits`) as the slots are drawn.  A pair is a species of
+// at least `thr` slots; with scnt (routing, thr > 1) the slots of the other
+// species are counted in scnt[c]: they take the plain sampler, whose
+// per-hadron set-up costs less than the network and envelope of a pair of
+// few hadrons.  Writes the pair bitmap (Wb words) and the number of pairs.
+__device__ __forceinline__ void count_pairs(const uint32_t* hits, int Ns, int thr, int64_t c, int Wb,
                                             uint32_t* __restrict__ gbits, uint32_t* __restrict__ cnt,
                                             uint32_t* __restrict__ scnt) {
     const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t all = lane < Wb ? bits[lane] : 0u;
-    const uint32_t wd = scnt ? (lane < Wb ? multi[lane] : 0u) : all;
-    if (lane < Wb) gbits[c * Wb + lane] = wd;
-    const uint32_t np = wave_sum(static_cast<uint32_t>(__popc(wd)));
+    uint32_t np = 0, ns = 0;
+    for (int s0 = 0; s0 < Ns; s0 += kWave) {
+        const int s = s0 + lane;
+        const uint32_t k = s < Ns ? hits[s] : 0u;
+        const bool pair = k >= static_cast<uint32_t>(thr);
+        if (k > 0 && !pair) ns += k;
+        const unsigned long long m = __ballot(pair);
+        if (lane == 0) {
+            gbits[c * Wb + (s0 >> 5)] = static_cast<uint32_t>(m);
+            if ((s0 >> 5) + 1 < Wb) gbits[c * Wb + (s0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
+        }
+        np += static_cast<uint32_t>(__popcll(m));
+    }
     if (lane == 0) cnt[c] = np;
     if (scnt) {
-        const uint32_t ns = wave_sum(static_cast<uint32_t>(__popc(all & ~wd)));
+        ns = wave_sum(ns);
         if (lane == 0) scnt[c] = ns;
     }
 }
@@ -419,11 +423,10 @@ template <typename R, bool PAIRS>
 __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const R* __restrict__ rows, int stride,
                                                              uint16_t* __restrict__ sp16, uint32_t* __restrict__ cellidx,
                                                              uint32_t* __restrict__ gbits, uint32_t* __restrict__ cnt,
-                                                             uint32_t* __restrict__ scnt) {
+                                                             uint32_t* __restrict__ scnt, int thr) {
     __shared__ double s_pre[kPickCells][kMaxSpecies];
     __shared__ uint32_t s_pos[kPickCells][kPickWords];
-    __shared__ uint32_t s_bits[kPickCells][kPickWords];
-    __shared__ uint32_t s_multi[kPickCells][kPickWords];
+    __shared__ uint32_t s_hits[kPickCells][kMaxSpecies];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
     if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
@@ -439,9 +442,9 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     }
     double* pre = s_pre[w];
     uint32_t* pos = s_pos[w];
-    uint32_t* bits = s_bits[w];
-    uint32_t* multi = s_multi[w];
-    if (PAIRS && lane < Wb) bits[lane] = multi[lane] = 0u;
+    uint32_t* hits = s_hits[w];
+    if (PAIRS)
+        for (int s = lane; s < Ns; s += kWave) hits[s] = 0u;
     const R* row = rows + c * stride;
     double carry = 0.;
     for (int s0 = 0; s0 < Ns; s0 += kWave) {
@@ -475,11 +478,11 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
         }
         sp16[o0 + r] = static_cast<uint16_t>(sp);
         cellidx[o0 + r] = static_cast<uint32_t>(c);
-        if (PAIRS) mark_species(bits, multi, sp, scnt != nullptr);
+        if (PAIRS) atomicAdd(&hits[sp], 1u);
     }
     if (PAIRS) {
         wave_lds_sync();
-        write_pairs(bits, multi, c, Wb, gbits, cnt, scnt);
+        count_pairs(hits, Ns, thr, c, Wb, gbits, cnt, scnt);
     }
 }
 
@@ -504,17 +507,15 @@ __device__ __forceinline__ uint64_t batch_out(const SampleArgs& a, int64_t i, in
 
 // Batched rounds with the learned proposal: each slot's species drawn from its
 // cell's FP64 CDF row exactly as k_setup draws it (the same stream, so the
-// species of every slot equal the plain rounds'), and the cell's species
-// bitmap and number of distinct species (the (cell, species) pairs; with
-// scnt only the species of two or more slots, as k_pick).  Slots at or beyond
+// species of every slot equal the plain rounds'), and the cell's pair bitmap
+// and number of pairs (count_pairs, as k_pick).  Slots at or beyond
 // the round's set-up records are left out.  One wave per cell;
 // launch with ceil(Nc / kPickCells) workgroups of kPickCells waves.
 __global__ __launch_bounds__(kWave * kPickCells) void k_batch_pick(SampleArgs a, uint16_t* __restrict__ sp16,
                                                                    uint32_t* __restrict__ gbits,
                                                                    uint32_t* __restrict__ cnt,
-                                                                   uint32_t* __restrict__ scnt) {
-    __shared__ uint32_t s_bits[kPickCells][kPickWords];
-    __shared__ uint32_t s_multi[kPickCells][kPickWords];
+                                                                   uint32_t* __restrict__ scnt, int thr) {
+    __shared__ uint32_t s_hits[kPickCells][kMaxSpecies];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
     if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
@@ -523,9 +524,8 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_batch_pick(SampleArgs a,
     const int64_t o0 = a.off[c];
     int64_t n = a.off[c + 1] - o0;
     if (a.setup_cap > 0) n = max<int64_t>(0, min<int64_t>(n, a.setup_cap - o0));
-    uint32_t* bits = s_bits[w];
-    uint32_t* multi = s_multi[w];
-    if (lane < Wb) bits[lane] = multi[lane] = 0u;
+    uint32_t* hits = s_hits[w];
+    for (int s = lane; s < Ns; s += kWave) hits[s] = 0u;
     wave_lds_sync();
     const double* row = a.cdf + c * cdf_stride(Ns);
     for (int64_t r = lane; r < n; r += kWave) {
@@ -533,10 +533,10 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_batch_pick(SampleArgs a,
         int sp = upper_bound(row, 0, Ns, st.uniform() * a.ctot[c]);
         if (sp >= Ns) sp = Ns - 1;
         sp16[o0 + r] = static_cast<uint16_t>(sp);
-        mark_species(bits, multi, sp, scnt != nullptr);
+        atomicAdd(&hits[sp], 1u);
     }
     wave_lds_sync();
-    write_pairs(bits, multi, c, Wb, gbits, cnt, scnt);
+    count_pairs(hits, Ns, thr, c, Wb, gbits, cnt, scnt);
 }
 
 // the species-dependent part of a set-up record: the |p| table, its first

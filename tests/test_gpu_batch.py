@@ -183,15 +183,20 @@ def test_surface_in_batch_matches_surface_alone(species, surfaces):
 
 
 def test_autotuned_round_size_is_used(species, surfaces):
-    """one tuned round size per proposal, each used by its own batches"""
+    """one tuned round size per proposal, each used by its own batches; the
+    learned proposal also times its routing thresholds and keeps one"""
     ctx = _ctx(species, surfaces)
     cand = (4, 16, 64, 256, 1024, 4096)
+    routes = (1, 2, 3, 4, 6, 8, 12, 16)
     rep = ctx.autotune_batch(EVENTS, seed=2)
     ctx.mlp_init(1)
     rep_l = ctx.autotune_batch(EVENTS, seed=2, proposal=1)
     print("batch autotune", rep, "learned", rep_l)
+    assert rep["n_tried"] == len(cand) and rep["learned_min_hadrons"] == 0
+    assert rep_l["n_tried"] == len(cand) + len(routes)
+    assert rep_l["learned_min_hadrons"] in routes and ctx.learned_routing() == rep_l["learned_min_hadrons"]
     for r in (rep, rep_l):
-        assert r["refill_min"] in cand and r["n_tried"] == len(cand)
+        assert r["refill_min"] in cand
     for proposal, r in ((0, rep), (1, rep_l)):
         ctx.sample_batch(EVENTS, seed=2, proposal=proposal)
         seg, ev = ctx.batch_layout()

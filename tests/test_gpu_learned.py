@@ -33,6 +33,9 @@ import mlp_oracle as mo  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
+# routing threshold of the routed learned pass (hadrons per (cell, species))
+ROUTE = 3
+
 
 def _ctx(quirk=0, shear=1, bulk=1, bulk_kind=11):
     ctx = iss_amd.Context(0)
@@ -158,8 +161,8 @@ def trained():
     out["learned"] = (ctx.hadrons().copy(), ctx.event_counts().copy(), ctx.stats().as_dict())
     ctx.sample(nev, seed=11, proposal=1)
     out["learned2"] = ctx.hadrons().copy()
-    # the default: pairs of one hadron take the plain sampler
-    ctx.set_learned_routing(2)
+    # routing: pairs of fewer than ROUTE hadrons take the plain sampler
+    ctx.set_learned_routing(ROUTE)
     ctx.sample(nev, seed=11, proposal=1)
     out["routed"] = (ctx.hadrons().copy(), ctx.event_counts().copy(), ctx.stats().as_dict())
     ctx.sample(nev, seed=11, proposal=1)
@@ -221,21 +224,22 @@ def test_learned_is_exact_and_deterministic(trained, which):
     assert tot_chi / tot_ndf <= 1.1
 
 
-def test_routing_counts_single_hadron_pairs(trained):
-    """Routing sends exactly the (cell, species) pairs of one hadron in the
-    pass to the plain sampler: recounted from the output, where a cell is
-    its position (unique on this surface) and a species its pid."""
+def test_routing_counts_small_pairs(trained):
+    """Routing sends exactly the (cell, species) pairs of fewer than ROUTE
+    hadrons in the pass to the plain sampler: recounted from the output,
+    where a cell is its position (unique on this surface) and a species its
+    pid."""
     h, _, st = trained["routed"]
     _, _, st1 = trained["learned"]
     key = np.stack([h["x"], h["y"], h["z"], h["t"]], axis=1)
     _, cell_of = np.unique(key, axis=0, return_inverse=True)
     pair = np.stack([cell_of.ravel().astype(np.int64), h["pid"].astype(np.int64)], axis=1)
     _, mult = np.unique(pair, axis=0, return_counts=True)
-    print("pairs", mult.size, "of one hadron", int(np.sum(mult == 1)), "routed", st["n_routed"],
-          "learned pairs", st["n_pairs"], "(all pairs without routing:", st1["n_pairs"], ")")
+    print("pairs", mult.size, "of fewer than", ROUTE, "hadrons", int(np.sum(mult < ROUTE)), "routed hadrons",
+          st["n_routed"], "learned pairs", st["n_pairs"], "(all pairs without routing:", st1["n_pairs"], ")")
     assert st1["n_routed"] == 0 and st1["n_pairs"] == mult.size
-    assert st["n_routed"] == np.sum(mult == 1)
-    assert st["n_pairs"] == np.sum(mult >= 2)
+    assert st["n_routed"] == np.sum(mult[mult < ROUTE])
+    assert st["n_pairs"] == np.sum(mult >= ROUTE)
     # the routed hadrons go through the plain sampler's set-up, not the fallback count
     assert st["n_fallback"] == 0
 
@@ -285,13 +289,17 @@ def test_learned_reference_tmunu(case):
 
 
 def test_autotuned_launch_shape_keeps_the_samples():
-    """The autotuner only changes launch shapes; with counter-based random
-    numbers the hadrons of a pass are bit-identical before and after."""
+    """The autotuner's launch shapes leave the samples alone: with
+    counter-based random numbers the hadrons of a pass are bit-identical
+    before and after (at the same routing threshold, which the learned
+    proposal's tuning also chooses: it decides which sampler draws a hadron's
+    momentum, not its law)."""
     ctx, sp = _ctx()
     cells, _ = synth.viscous_lrf(3000, sp, seed=8, target_hadrons=600.0)
     ctx.set_surface_lrf(cells)
     ctx.compute_yields(precision=32)
     ctx.mlp_init(0)
+    route0 = ctx.learned_routing()
     ref = {}
     for prop in (0, 1):
         ctx.sample(200, seed=21, proposal=prop)
@@ -300,5 +308,9 @@ def test_autotuned_launch_shape_keeps_the_samples():
         rep = ctx.autotune(proposal=prop, n_events=100, seed=3, repeats=1)
         print("proposal", prop, rep)
         assert rep["n_tried"] >= 10 and rep["ms_best"] <= rep["ms_default"] + 1e-9
+        if prop == 1:
+            assert rep["learned_min_hadrons"] in (1, 2, 3, 4, 6, 8, 12, 16)
+            assert ctx.learned_routing() == rep["learned_min_hadrons"]
+            ctx.set_learned_routing(route0)
         ctx.sample(200, seed=21, proposal=prop)
         assert np.array_equal(ctx.hadrons(), ref[prop])
