@@ -1535,13 +1535,13 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         uint32_t* gc = learned ? ctx->d_lcnt.p : nullptr;
         uint32_t* gs = route ? ctx->d_lscnt.p : nullptr;
         if (f64 && learned)
-            k_pick<double, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
+            k_pick<double, true><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, true), ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         else if (f64)
-            k_pick<double, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
+            k_pick<double, false><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, false), ctx->stream>>>(a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         else if (learned)
-            k_pick<float, true><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
+            k_pick<float, true><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, true), ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         else
-            k_pick<float, false><<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
+            k_pick<float, false><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, false), ctx->stream>>>(a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, ctx->learn_route);
         HIPCHK(hipGetLastError());
     }
     int per_cu_trials = 0;

@@ -385,9 +385,14 @@ __global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ 
 // measure ~1e-16).  Writes each slot's species and cell; with PAIRS also the
 // cell's pair bitmap and its number of pairs (the learned sampler's (cell,
 // species) pairs, count_pairs).  Launch with ceil(Nc / kPickCells) workgroups
-// of kPickCells waves.
+// of kPickCells waves and pick_lds_bytes(Ns, PAIRS) bytes of dynamic LDS (the
+// prefix rows and hit counts, sized to the species table so that the
+// latency-bound kernel keeps its full occupancy).
 constexpr int kPickCells = 4;
 constexpr int kPickWords = kMaxSpecies / 32;
+__host__ __device__ constexpr size_t pick_lds_bytes(int Ns, bool pairs) {
+    return static_cast<size_t>(kPickCells) * Ns * (sizeof(double) + (pairs ? sizeof(uint32_t) : 0));
+}
 
 // The learned sampler's pairs of one cell from its slots' species, counted
 // per species in LDS (`hits`) as the slots are drawn.  A pair is a species of
@@ -424,9 +429,8 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
                                                              uint16_t* __restrict__ sp16, uint32_t* __restrict__ cellidx,
                                                              uint32_t* __restrict__ gbits, uint32_t* __restrict__ cnt,
                                                              uint32_t* __restrict__ scnt, int thr) {
-    __shared__ double s_pre[kPickCells][kMaxSpecies];
+    extern __shared__ double s_pick[];  // [kPickCells][Ns] prefix, then (PAIRS) [kPickCells][Ns] hits
     __shared__ uint32_t s_pos[kPickCells][kPickWords];
-    __shared__ uint32_t s_hits[kPickCells][kMaxSpecies];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
     if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
@@ -440,9 +444,9 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
         }
         return;
     }
-    double* pre = s_pre[w];
+    double* pre = s_pick + w * Ns;
     uint32_t* pos = s_pos[w];
-    uint32_t* hits = s_hits[w];
+    uint32_t* hits = reinterpret_cast<uint32_t*>(s_pick + kPickCells * Ns) + w * Ns;
     if (PAIRS)
         for (int s = lane; s < Ns; s += kWave) hits[s] = 0u;
     const R* row = rows + c * stride;
