@@ -84,7 +84,7 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def cpu_baseline(cells, seconds_target=10.0):
+def cpu_baseline(cells, seconds_target=10.0, total_cells=None):
     """The CPU oracle (a restatement of the reference algorithm: yields,
     Poisson multiplicities, |p| tables and rejection, boost) on a bounded
     sample of the same workload: the first 2000 cells of the surface, as many
@@ -113,7 +113,7 @@ def cpu_baseline(cells, seconds_target=10.0):
     dtall = time.perf_counter() - t0
     host = f"host {cpu_model()}, nproc {os.cpu_count()}"
     return {"value": nall / dtall, "unit": "hadrons/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (yields + sampling) on the first 2000 of the 200,000 cells: {cores} threads x "
+            "sample": f"oracle (yields + sampling) on the first 2000 of the {total_cells or cells.shape[0]:,} cells: {cores} threads x "
                       f"{nev} events, {nall} hadrons in {dtall:.1f} s; {host}",
             "single_thread": {"value": n1 / dt1, "cores": 1,
                               "sample": f"{nev} events, {n1} hadrons in {dt1:.1f} s"}}
@@ -353,7 +353,7 @@ def run_batched(args):
     }
     ctx.close()
     if c5 and not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sub)
+        out["cpu_baseline"] = cpu_baseline(sub, total_cells=int(nc))
     print(json.dumps(out))
 
 

@@ -233,9 +233,12 @@ int iss_mlp_get_params(iss_ctx* ctx, float* params, int n);
  * species) pair with fewer than min_hadrons hadrons in the pass takes the
  * plain sampler, whose per-hadron set-up and trials cost less than one
  * network evaluation and envelope shared by few hadrons.  1: every pair
- * takes the learned proposal; default 4; at most 64.  iss_stats.n_routed
- * counts the routed hadrons; iss_autotune (proposal 1) picks the threshold
- * on timed passes.  The output law is the same for every threshold. */
+ * takes the learned proposal; default 4; 1..64, or ISS_LEARNED_ROUTE_ALL:
+ * every pair routed, the pass runs as the plain one without the pair
+ * bookkeeping.  iss_stats.n_routed counts the routed hadrons; iss_autotune
+ * and iss_autotune_batch (proposal 1) pick the threshold on timed passes.
+ * The output law is the same for every threshold. */
+#define ISS_LEARNED_ROUTE_ALL 65
 int iss_set_learned_routing(iss_ctx* ctx, int32_t min_hadrons);
 int iss_get_learned_routing(const iss_ctx* ctx);
 

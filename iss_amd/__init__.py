@@ -373,7 +373,8 @@ class Context:
 
     def set_learned_routing(self, min_hadrons: int = 4):
         """(cell, species) pairs with fewer hadrons than min_hadrons (1..64)
-        in a learned pass take the plain sampler."""
+        in a learned pass take the plain sampler; 65 routes every pair (the
+        pass runs as the plain one)."""
         self._check(self.lib.iss_set_learned_routing(self.h, min_hadrons), "iss_set_learned_routing")
 
     def learned_routing(self) -> int:

@@ -156,6 +156,10 @@ constexpr size_t PIN_OOB = 0, PIN_K = 8, PIN_STATS = 64, PIN_AUX = 128, PIN_TOT 
 // hadrons are cheaper on the plain sampler; the autotuner times the choice.
 constexpr int kLearnRouteDefault = 4;
 constexpr int kLearnRouteMax = 64;
+// every pair routed: the learned pass runs as the plain pass, without the
+// pair bookkeeping (what the autotuner keeps where no pair pays)
+constexpr int kLearnRouteAll = kLearnRouteMax + 1;
+static_assert(kLearnRouteAll == ISS_LEARNED_ROUTE_ALL, "routing constant of the C ABI");
 
 // one generation of the resonance feed-down: its particles and per-particle
 // bookkeeping (kept by the context and reused across calls)
@@ -1506,14 +1510,15 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     a.yLB = static_cast<float>(o->y_LB);
     a.yRB = static_cast<float>(o->y_RB);
     a.max_trials = o->max_trials;
-    a.refill_min = ctx->refill_min[o->proposal == 1 ? 1 : 0];
+    // (the learned proposal with every pair routed runs the plain pass)
+    const bool learned = o->proposal == 1 && ctx->learn_route < kLearnRouteAll;
+    a.refill_min = ctx->refill_min[learned ? 1 : 0];
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
     a.Kdev = nullptr;
     a.anti = ctx->d_anti.p;
     // each slot's cell and species from its cell's yield row; the learned
-    // pass also gets each cell's species bitmap and pair count
-    const bool learned = o->proposal == 1;
+    // pass also gets each cell's pair bitmap and pair count
     const bool route = learned && ctx->learn_route > 1;
     const int Wb = (ctx->Ns + 31) / 32;
     HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(K, 1)));
@@ -1550,7 +1555,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         per_cu_trials = 1;
     const int per_cu_plain = ctx->blocks_per_cu[0] > 0 ? std::min(ctx->blocks_per_cu[0], per_cu_trials) : per_cu_trials;
     HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
-    if (o->proposal == 0) {
+    if (!learned) {
         HIPCHK(ctx->d_setup.ensure(std::max<int64_t>(K, 1)));
         if (K > 0) {
             HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
@@ -1636,7 +1641,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     unsigned long long* h_st = ctx->h_pin.at<unsigned long long>(PIN_STATS);
     HIPCHK(hipMemcpyAsync(h_st, ctx->d_stats.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     int64_t npairs = 0, nrouted = 0;
-    if (o->proposal == 1 && K > 0)
+    if (learned && K > 0)
         HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_AUX), ctx->d_pairoff.p + Nc, sizeof(int64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
     if (route && K > 0)
@@ -1645,7 +1650,8 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     unsigned long long st[6];
     std::memcpy(st, h_st, sizeof(st));
-    if (o->proposal == 1 && K > 0) npairs = *ctx->h_pin.at<int64_t>(PIN_AUX);
+    if (learned && K > 0) npairs = *ctx->h_pin.at<int64_t>(PIN_AUX);
+    if (o->proposal == 1 && !learned) nrouted = K;
     if (route && K > 0) nrouted = *ctx->h_pin.at<int64_t>(PIN_AUX + 8);
     float ms_all = 0.f, ms_k = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_all, ctx->ev[3], ctx->ev[5]));
@@ -1752,7 +1758,7 @@ int iss_autotune(iss_ctx* ctx, const iss_tune_opts* o, iss_tune_report* rep) {
     // (pair bookkeeping, set-up and trials of both samplers)
     int best_route = ctx->learn_route;
     if (p == 1) {
-        const int routes[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        const int routes[] = {1, 2, 3, 4, 6, 8, 12, 16, kLearnRouteAll};
         double best_pass = 1e30;
         for (int rt : routes) {
             ctx->learn_route = rt;
@@ -1783,9 +1789,9 @@ int iss_mlp_num_params(void) { return kNumParams; }
 
 int iss_set_learned_routing(iss_ctx* ctx, int32_t min_hadrons) {
     if (!ctx) return ISS_ERR_ARG;
-    if (min_hadrons < 1 || min_hadrons > kLearnRouteMax)
+    if (min_hadrons < 1 || min_hadrons > kLearnRouteAll)
         return set_err(ctx, ISS_ERR_ARG, "learned routing: min_hadrons must be in [1, " +
-                                             std::to_string(kLearnRouteMax) + "]");
+                                             std::to_string(kLearnRouteAll) + "]");
     ctx->learn_route = min_hadrons;
     return ISS_OK;
 }
@@ -2078,8 +2084,9 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     if (ctx->S <= 0) return set_err(ctx, ISS_ERR_STATE, "no batched surfaces (iss_set_surfaces_lrf)");
     if (o->max_trials <= 1) return set_err(ctx, ISS_ERR_ARG, "max_trials must exceed 1");
     if (o->proposal != 0 && o->proposal != 1) return set_err(ctx, ISS_ERR_ARG, "proposal must be 0 or 1");
-    const bool learned = o->proposal == 1;
-    if (learned && !ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters");
+    if (o->proposal == 1 && !ctx->mlp_ready) return set_err(ctx, ISS_ERR_STATE, "no network parameters");
+    // (the learned proposal with every pair routed runs the plain rounds)
+    const bool learned = o->proposal == 1 && ctx->learn_route < kLearnRouteAll;
     const int S = ctx->S;
     int maxE = 0;
     for (int k = 0; k < S; k++) {
@@ -2314,7 +2321,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
     ctx->stats.n_fallback = learned ? static_cast<int64_t>(st[4]) : 0;
     ctx->stats.n_violations = learned ? static_cast<int64_t>(st[5]) : 0;
-    ctx->stats.n_routed = learned ? static_cast<int64_t>(st[8]) : 0;
+    ctx->stats.n_routed = learned ? static_cast<int64_t>(st[8]) : (o->proposal == 1 ? ctx->K : 0);
     ctx->stats.ms_sample = ms;
     ctx->stats.ms_sample_kernel = ms;
     if (st[6] || st[7] || ctx->K > out_cap) {
@@ -2386,7 +2393,7 @@ int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_e
     // learned proposal: the routing threshold at the chosen round size
     int best_route = ctx->learn_route;
     if (learned) {
-        const int routes[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        const int routes[] = {1, 2, 3, 4, 6, 8, 12, 16, kLearnRouteAll};
         bo.events_per_round = best_R;
         for (int rt : routes) {
             ctx->learn_route = rt;

@@ -309,7 +309,7 @@ def test_autotuned_launch_shape_keeps_the_samples():
         print("proposal", prop, rep)
         assert rep["n_tried"] >= 10 and rep["ms_best"] <= rep["ms_default"] + 1e-9
         if prop == 1:
-            assert rep["learned_min_hadrons"] in (1, 2, 3, 4, 6, 8, 12, 16)
+            assert rep["learned_min_hadrons"] in (1, 2, 3, 4, 6, 8, 12, 16, 65)  # 65: every pair routed
             assert ctx.learned_routing() == rep["learned_min_hadrons"]
             ctx.set_learned_routing(route0)
         ctx.sample(200, seed=21, proposal=prop)
