@@ -154,7 +154,8 @@ def run_steps(ctx, args, proposal, first_event):
     for i in range(args.warmup):
         step(i)
     acc = {"hadrons": 0, "trials": 0, "inner": 0, "ms_yield": [], "ms_sample": [], "ms_setup": [],
-           "ms_trials": [], "violations": 0, "fallback": 0, "pairs": 0, "routed": 0}
+           "ms_trials": [], "ms_ltrials": [], "ms_rsetup": [], "ms_rtrials": [], "violations": 0, "fallback": 0,
+           "pairs": 0, "routed": 0, "routed_trials": 0}
     t0 = time.perf_counter()
     for i in range(args.steps):
         n, st = step(args.warmup + i)
@@ -169,8 +170,12 @@ def run_steps(ctx, args, proposal, first_event):
         acc["fallback"] += st.n_fallback
         acc["pairs"] += st.n_pairs if proposal == 1 else 0
         acc["routed"] += st.n_routed if proposal == 1 else 0
+        acc["routed_trials"] += st.n_routed_trials
+        acc["ms_ltrials"].append(st.ms_learned_trials_kernel)
+        acc["ms_rsetup"].append(st.ms_routed_setup_kernel)
+        acc["ms_rtrials"].append(st.ms_routed_trials_kernel)
     acc["elapsed"] = time.perf_counter() - t0
-    for k in ("ms_yield", "ms_sample", "ms_setup", "ms_trials"):
+    for k in ("ms_yield", "ms_sample", "ms_setup", "ms_trials", "ms_ltrials", "ms_rsetup", "ms_rtrials"):
         acc[k] = float(np.mean(acc[k]))
     return acc
 
@@ -191,14 +196,14 @@ def hw_from_profiles(kernel: str):
 
 
 def roofline_valu(kernel, ops, ms):
-    tf = ops / (ms * 1e-3) / 1e12
+    tf = ops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     return {"bound": "valu", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": tf / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms,
             "hardware": hw_from_profiles(kernel)}
 
 
 def roofline_mfma(kernel, flop, ms):
-    tf = flop / (ms * 1e-3) / 1e12
+    tf = flop / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     return {"bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": tf / BF16_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel), "kernel": kernel, "ms": ms,
             "hardware": hw_from_profiles(kernel)}
@@ -360,8 +365,8 @@ def run_batched(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=200000)
     ap.add_argument("--events", type=int, default=1000)
     ap.add_argument("--precision", type=int, default=32)
@@ -404,14 +409,14 @@ def main():
         ctx.compute_yields(precision=args.precision)
         # (passes of the workload's size: the learned routing depends on the
         # hadrons per (cell, species) pair of a pass)
-        tune["plain"] = ctx.autotune(proposal=0, n_events=args.events, seed=7, repeats=1)
+        tune["plain"] = ctx.autotune(proposal=0, n_events=args.events, seed=7, repeats=3)
     plain = run_steps(ctx, args, 0, 0)
     # training set: the hadrons accepted by the last plain pass
     n_train = ctx.mlp_collect(10_000_000)
     ctx.mlp_init(1)
     rep = ctx.mlp_train(batch=8192, steps=args.train_steps, lr=3e-3, seed=5)
     if not args.no_tune:
-        tune["learned"] = ctx.autotune(proposal=1, n_events=args.events, seed=7, repeats=1)
+        tune["learned"] = ctx.autotune(proposal=1, n_events=args.events, seed=7, repeats=3)
     learned = run_steps(ctx, args, 1, (args.warmup + args.steps) * args.events)
 
     v_learn = learned["hadrons"] / learned["elapsed"]
@@ -428,12 +433,24 @@ def main():
     # sampling pass and its trial kernels against the FP32 vector peak
     l_setup = roofline_mfma("k_learn_setup", NET_FLOP_PER_PAIR * pairs, learned["ms_setup"])
     lt, pt_ = learned["trials"] / args.steps, plain["trials"] / args.steps
-    l_trials = roofline_valu("k_learn_trials", OPS_PER_TRIAL * lt + OPS_PER_ACCEPTED * hl, learned["ms_trials"])
+    # the learned step's trial kernels apart: the learned one on the pairs'
+    # hadrons, the plain one on the routed and envelope-less hadrons
+    rt = learned["routed_trials"] / args.steps
+    hr = routed + learned["fallback"] / args.steps
+    l_trials = roofline_valu("k_learn_trials", OPS_PER_TRIAL * (lt - rt) + OPS_PER_ACCEPTED * (hl - hr),
+                             learned["ms_ltrials"])
+    r_trials = roofline_valu("k_trials_routed", OPS_PER_TRIAL * rt + OPS_PER_ACCEPTED * hr, learned["ms_rtrials"])
     p_trials = roofline_valu("k_trials", OPS_PER_TRIAL * pt_ + OPS_PER_ACCEPTED * hp, plain["ms_trials"])
     pass_l = roofline_valu("sampling pass (learned)", OPS_PER_TRIAL * lt + OPS_PER_ACCEPTED * hl, learned["ms_sample"])
     pass_p = roofline_valu("sampling pass (plain)", OPS_PER_TRIAL * pt_ + OPS_PER_ACCEPTED * hp, plain["ms_sample"])
     p_setup = {"kernel": "k_setup", "ms": plain["ms_setup"], "hardware": hw_from_profiles("k_setup")}
-    cands = [(learned["ms_yield"], roof_y), (learned["ms_setup"], l_setup), (learned["ms_trials"], l_trials)]
+    # the dominant kernel of the learned step (the value's step); with every
+    # pair routed that step is the plain pass
+    cands = [(learned["ms_yield"], roof_y), (learned["ms_setup"], l_setup), (learned["ms_ltrials"], l_trials),
+             (learned["ms_rtrials"], r_trials)]
+    if learned["ms_ltrials"] == 0 and learned["ms_rtrials"] == 0:
+        cands.append((learned["ms_trials"], roofline_valu("k_trials", OPS_PER_TRIAL * lt + OPS_PER_ACCEPTED * hl,
+                                                          learned["ms_trials"])))
     dominant = max(cands, key=lambda t: t[0])[1]
     out = {
         "metric": "sampled hadrons/s (learned proposal vs plain GPU) at fixed chi2; yield-pass GB/s",
@@ -469,11 +486,13 @@ def main():
         "training": rep,
         "autotune": tune,
         "kernels_ms": {"k_yield_rows": learned["ms_yield"], "k_learn_setup": learned["ms_setup"],
-                       "k_learn_trials": learned["ms_trials"], "k_setup": plain["ms_setup"],
+                       "k_learn_trials": learned["ms_ltrials"], "k_setup (routed)": learned["ms_rsetup"],
+                       "k_trials_routed": learned["ms_rtrials"], "k_setup": plain["ms_setup"],
                        "k_trials": plain["ms_trials"]},
         "roofline_yield": roof_y,
         "roofline_sample": {"pass_learned": pass_l, "pass_plain": pass_p, "k_learn_setup": l_setup,
-                            "k_learn_trials": l_trials, "k_setup": p_setup, "k_trials": p_trials},
+                            "k_learn_trials": l_trials, "k_trials_routed": r_trials, "k_setup": p_setup,
+                            "k_trials": p_trials},
         "yield_GBs": y_gbs,
     }
     # resonance feed-down of the last learned step's hadrons (outside the timed region)

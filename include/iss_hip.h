@@ -136,6 +136,15 @@ typedef struct {
     int64_t n_pairs;         /* learned proposal: (cell, species) pairs of the last pass */
     int64_t n_routed;        /* learned proposal: hadrons routed to the plain sampler
                                 (iss_set_learned_routing) */
+    /* learned iss_sample: device time of the learned trial kernel alone, and
+     * of the plain set-up and trials of the routed and envelope-less hadrons
+     * (ms_trials_kernel holds all three); the plain trials' outer trials and
+     * |p| proposals (included in n_trials, n_inner) */
+    double ms_learned_trials_kernel;
+    double ms_routed_setup_kernel;
+    double ms_routed_trials_kernel;
+    int64_t n_routed_trials;
+    int64_t n_routed_inner;
 } iss_stats;
 
 /* ---- lifecycle (the reference: iSS ctor/dtor, src/iSS.cpp:14-30) ---- */

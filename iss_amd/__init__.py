@@ -78,7 +78,10 @@ class Stats(C.Structure):
                 ("n_fallback", C.c_int64), ("n_violations", C.c_int64),
                 ("ms_setup_kernel", C.c_double), ("ms_trials_kernel", C.c_double),
                 ("ms_decay", C.c_double), ("n_decay_generations", C.c_int64),
-                ("n_pairs", C.c_int64), ("n_routed", C.c_int64)]
+                ("n_pairs", C.c_int64), ("n_routed", C.c_int64),
+                ("ms_learned_trials_kernel", C.c_double), ("ms_routed_setup_kernel", C.c_double),
+                ("ms_routed_trials_kernel", C.c_double), ("n_routed_trials", C.c_int64),
+                ("n_routed_inner", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

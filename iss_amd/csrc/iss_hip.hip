@@ -124,7 +124,7 @@ struct DBuf {
 // Pinned host memory for the read-backs of counts, flags and totals: a copy
 // from device memory into pageable memory goes through the runtime's staging
 // buffer and costs tens of microseconds more than one into pinned memory.
-// Slots: PIN_OOB int, PIN_K int64, PIN_STATS 8 x u64, PIN_AUX int64, then
+// Slots: PIN_OOB int, PIN_K int64, PIN_STATS 16 x u64, PIN_AUX 2 x int64, then
 // PIN_TOT [Ns] doubles.
 struct PinBuf {
     void* p = nullptr;
@@ -147,7 +147,11 @@ struct PinBuf {
     template <typename T>
     T* at(size_t off) { return reinterpret_cast<T*>(static_cast<char*>(p) + off); }
 };
-constexpr size_t PIN_OOB = 0, PIN_K = 8, PIN_STATS = 64, PIN_AUX = 128, PIN_TOT = 256;
+constexpr size_t PIN_OOB = 0, PIN_K = 8, PIN_STATS = 64, PIN_AUX = 192, PIN_TOT = 256;
+// device counters: [0..3] trials, |p| proposals, failed, out of table; [4..5]
+// learned fallback and envelope violations; [6..7] batch overflow flags; [8]
+// routed hadrons of batches; [9..12] the routed plain trials' [0..3]
+constexpr int kStatRouted = 9, kStatSlots = 13;
 
 // Learned passes: (cell, species) pairs of fewer hadrons than the routing
 // threshold take the plain sampler.  A pair's network and envelope cost
@@ -185,7 +189,7 @@ struct iss_ctx {
     double* scratch = nullptr;  // this device's `scratch` (allocation-free entry point)
     std::string scratch_err;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[11] = {};
     std::string err;
     int num_cu = 256;
     // species
@@ -342,6 +346,15 @@ int check_output(iss_ctx* ctx) {
     if (ctx->K < 0 || (ctx->K > 0 && (!ctx->d_out.p || static_cast<size_t>(ctx->K) > ctx->d_out.cap)))
         return set_err(ctx, ISS_ERR_STATE, "hadron count exceeds the output buffer");
     return ISS_OK;
+}
+
+// the learned pass's per-kernel figures, which only iss_sample fills
+void clear_routed_stats(iss_ctx* ctx) {
+    ctx->stats.ms_learned_trials_kernel = 0.;
+    ctx->stats.ms_routed_setup_kernel = 0.;
+    ctx->stats.ms_routed_trials_kernel = 0.;
+    ctx->stats.n_routed_trials = 0;
+    ctx->stats.n_routed_inner = 0;
 }
 
 int launch_scan(iss_ctx* ctx, const uint32_t* in, int64_t n, int64_t* out) {
@@ -878,7 +891,7 @@ int iss_ctx_create(int device, iss_ctx** out) {
     ctx->scratch = scratch_address(device, &ctx->scratch_err);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
-    if (ctx->d_stats.ensure(9) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
+    if (ctx->d_stats.ensure(kStatSlots) != hipSuccess || ctx->d_fb_count.ensure(1) != hipSuccess) {
         delete ctx;
         return ISS_ERR_NOMEM;
     }
@@ -1335,7 +1348,7 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     HIPCHK(ctx->d_mcnt.ensure(nseg));
     HIPCHK(ctx->d_msegoff.ensure(nseg + 1));
     HIPCHK(ctx->d_evoff.ensure(E + 1));
-    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 6 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, kStatSlots * sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if (Nc > 0) k_row_scan<<<Ns, 256, 0, ctx->stream>>>(ctx->d_ycol.p, Nc);
     k_model_counts<<<(nseg + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_stot.p, Ns, E, mult, model, o->sampling_para1,
@@ -1412,6 +1425,7 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     ctx->stats.n_fallback = 0;
     ctx->stats.n_violations = 0;
     ctx->stats.n_routed = 0;
+    clear_routed_stats(ctx);
     if (n_hadrons) *n_hadrons = K;
     if (st[3]) return set_err(ctx, ISS_ERR_RANGE, "m/T - mu/T outside the momentum-sampler tables");
     return ISS_OK;
@@ -1449,7 +1463,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     while ((1ll << D) < E) D++;
     HIPCHK(ctx->d_tree0.ensure(1ll << D));
     HIPCHK(ctx->d_tree1.ensure(1ll << D));
-    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 6 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, kStatSlots * sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if (Nc > 0)
         k_counts<<<(Nc + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_ctot.p, Nc, mult, o->seed, chunk,
@@ -1621,15 +1635,19 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             const int nblk = static_cast<int>(
                 std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu)));
             k_learn_trials<<<nblk, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
+            HIPCHK(hipEventRecord(ctx->ev[8], ctx->stream));
             // routed hadrons and hadrons without an envelope: the plain
-            // sampler (its own launch shape), sized on the device
+            // sampler (its own launch shape and counters), sized on the device
             const int64_t fb_want = route ? (K + 255) / 256 : ctx->num_cu * 4;
             k_setup<false><<<static_cast<int>(std::max<int64_t>(1, fb_want)), 256, 0, ctx->stream>>>(
                 a, ctx->d_setup.p, ctx->d_fb_list.p, ctx->d_fb_count.p);
+            HIPCHK(hipEventRecord(ctx->ev[9], ctx->stream));
             SampleArgs af = a;
             af.Kdev = ctx->d_fb_count.p;
             af.refill_min = ctx->refill_min[0];
-            k_trials<<<ctx->num_cu * per_cu_plain, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            af.stats = ctx->d_stats.p + kStatRouted;
+            k_trials_routed<<<ctx->num_cu * per_cu_plain, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            HIPCHK(hipEventRecord(ctx->ev[10], ctx->stream));
         }
     }
     HIPCHK(hipGetLastError());
@@ -1639,7 +1657,8 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     }
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     unsigned long long* h_st = ctx->h_pin.at<unsigned long long>(PIN_STATS);
-    HIPCHK(hipMemcpyAsync(h_st, ctx->d_stats.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h_st, ctx->d_stats.p, kStatSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          ctx->stream));
     int64_t npairs = 0, nrouted = 0;
     if (learned && K > 0)
         HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_AUX), ctx->d_pairoff.p + Nc, sizeof(int64_t),
@@ -1648,7 +1667,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         HIPCHK(hipMemcpyAsync(ctx->h_pin.at<int64_t>(PIN_AUX + 8), ctx->d_soff.p + Nc, sizeof(int64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    unsigned long long st[6];
+    unsigned long long st[kStatSlots];
     std::memcpy(st, h_st, sizeof(st));
     if (learned && K > 0) npairs = *ctx->h_pin.at<int64_t>(PIN_AUX);
     if (o->proposal == 1 && !learned) nrouted = K;
@@ -1663,13 +1682,26 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
     HIPCHK(hipEventElapsedTime(&ms_tr, ctx->ev[6], ctx->ev[5]));
     ctx->stats.ms_setup_kernel = ms_su;
     ctx->stats.ms_trials_kernel = ms_tr;
+    // learned pass: its trial kernel, and the plain set-up and trials of the
+    // routed and envelope-less hadrons, apart
+    float ms_lt = 0.f, ms_rs = 0.f, ms_rt = 0.f;
+    if (learned && K > 0) {
+        HIPCHK(hipEventElapsedTime(&ms_lt, ctx->ev[6], ctx->ev[8]));
+        HIPCHK(hipEventElapsedTime(&ms_rs, ctx->ev[8], ctx->ev[9]));
+        HIPCHK(hipEventElapsedTime(&ms_rt, ctx->ev[9], ctx->ev[10]));
+    }
+    ctx->stats.ms_learned_trials_kernel = ms_lt;
+    ctx->stats.ms_routed_setup_kernel = ms_rs;
+    ctx->stats.ms_routed_trials_kernel = ms_rt;
+    ctx->stats.n_routed_trials = static_cast<int64_t>(st[kStatRouted + 0]);
+    ctx->stats.n_routed_inner = static_cast<int64_t>(st[kStatRouted + 1]);
     ctx->K = K;
     ctx->E = E;
     ctx->stats.n_hadrons = K;
-    ctx->stats.n_trials = static_cast<int64_t>(st[0]);
-    ctx->stats.n_inner = static_cast<int64_t>(st[1]);
-    ctx->stats.n_failed = static_cast<int64_t>(st[2]);
-    ctx->stats.n_range_errors = static_cast<int64_t>(st[3]);
+    ctx->stats.n_trials = static_cast<int64_t>(st[0] + st[kStatRouted + 0]);
+    ctx->stats.n_inner = static_cast<int64_t>(st[1] + st[kStatRouted + 1]);
+    ctx->stats.n_failed = static_cast<int64_t>(st[2] + st[kStatRouted + 2]);
+    ctx->stats.n_range_errors = static_cast<int64_t>(st[3] + st[kStatRouted + 3]);
     ctx->stats.ms_sample = ms_all;
     ctx->stats.ms_sample_kernel = ms_k;
     ctx->stats.n_fallback = static_cast<int64_t>(st[4]);
@@ -2143,7 +2175,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     HIPCHK(ctx->d_bbase.ensure(1));
     HIPCHK(hipMemsetAsync(ctx->d_bround.p, 0, sizeof(uint32_t), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->d_bbase.p, 0, sizeof(unsigned long long), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, 9 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_stats.p, 0, kStatSlots * sizeof(unsigned long long), ctx->stream));
     BatchDev b{ctx->d_surf_off.p, ctx->d_bevents.p, S, R, ctx->d_bround.p, ctx->d_bbase.p, ymult};
     SampleArgs a = base_args(ctx);
     a.cdf = ctx->d_cdf.p;
@@ -2265,7 +2297,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
             k_learn_trials<<<learn_blocks, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
             k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p,
                                                                  ctx->d_fb_count.p);
-            k_trials<<<ctx->num_cu * (route ? per_cu : per_cu_fb), 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            k_trials_routed<<<ctx->num_cu * (route ? per_cu : per_cu_fb), 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
         }
         k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);
     };
@@ -2322,6 +2354,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     ctx->stats.n_fallback = learned ? static_cast<int64_t>(st[4]) : 0;
     ctx->stats.n_violations = learned ? static_cast<int64_t>(st[5]) : 0;
     ctx->stats.n_routed = learned ? static_cast<int64_t>(st[8]) : (o->proposal == 1 ? ctx->K : 0);
+    clear_routed_stats(ctx);
     ctx->stats.ms_sample = ms;
     ctx->stats.ms_sample_kernel = ms;
     if (st[6] || st[7] || ctx->K > out_cap) {

@@ -626,7 +626,7 @@ constexpr int kRefillMin = 16;
 constexpr int kCoarse = 4;
 constexpr int kCoarseRows = (kPTabLen + kCoarse - 1) / kCoarse;
 
-__global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) {
+__device__ __forceinline__ void trials_body(const SampleArgs& a, const HSetup* __restrict__ rec) {
     __shared__ unsigned long long s_stats[4];
     __shared__ double s_ctab[2][kCoarseRows * 3];
     if (threadIdx.x < 4) s_stats[threadIdx.x] = 0ull;
@@ -878,6 +878,15 @@ __global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __re
     }
     __syncthreads();
     if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) { trials_body(a, rec); }
+
+// The same trials for the hadrons a learned pass hands to the plain sampler
+// (routed pairs and hadrons without an envelope): its own symbol, so that
+// profiles and timings tell it from the plain pass's launch.
+__global__ __launch_bounds__(256) void k_trials_routed(SampleArgs a, const HSetup* __restrict__ rec) {
+    trials_body(a, rec);
 }
 
 // ============================================= local charge conservation
