@@ -429,9 +429,16 @@ def main():
               "ms": learned["ms_yield"], "bytes": ybytes, "hardware": hw_from_profiles("k_yield_rows")}
     pairs = learned["pairs"] / args.steps
     routed = learned["routed"] / args.steps
+    # every pair routed (the autotuner's choice where no pair pays): the
+    # learned step ran the plain pass, whose set-up is the plain one
+    plain_step = learned["ms_ltrials"] == 0 and learned["ms_rtrials"] == 0
+    if plain_step:
+        learned["ms_lsetup"] = 0.0
+    else:
+        learned["ms_lsetup"] = learned["ms_setup"]
     # the network (one evaluation per pair) against the BF16 MFMA peak; the
     # sampling pass and its trial kernels against the FP32 vector peak
-    l_setup = roofline_mfma("k_learn_setup", NET_FLOP_PER_PAIR * pairs, learned["ms_setup"])
+    l_setup = roofline_mfma("k_learn_setup", NET_FLOP_PER_PAIR * pairs, learned["ms_lsetup"])
     lt, pt_ = learned["trials"] / args.steps, plain["trials"] / args.steps
     # the learned step's trial kernels apart: the learned one on the pairs'
     # hadrons, the plain one on the routed and envelope-less hadrons
@@ -446,9 +453,9 @@ def main():
     p_setup = {"kernel": "k_setup", "ms": plain["ms_setup"], "hardware": hw_from_profiles("k_setup")}
     # the dominant kernel of the learned step (the value's step); with every
     # pair routed that step is the plain pass
-    cands = [(learned["ms_yield"], roof_y), (learned["ms_setup"], l_setup), (learned["ms_ltrials"], l_trials),
+    cands = [(learned["ms_yield"], roof_y), (learned["ms_lsetup"], l_setup), (learned["ms_ltrials"], l_trials),
              (learned["ms_rtrials"], r_trials)]
-    if learned["ms_ltrials"] == 0 and learned["ms_rtrials"] == 0:
+    if plain_step:
         cands.append((learned["ms_trials"], roofline_valu("k_trials", OPS_PER_TRIAL * lt + OPS_PER_ACCEPTED * hl,
                                                           learned["ms_trials"])))
     dominant = max(cands, key=lambda t: t[0])[1]
@@ -485,7 +492,8 @@ def main():
         "fallback_hadrons": learned["fallback"],
         "training": rep,
         "autotune": tune,
-        "kernels_ms": {"k_yield_rows": learned["ms_yield"], "k_learn_setup": learned["ms_setup"],
+        "learned_step_is_plain_pass": plain_step,
+        "kernels_ms": {"k_yield_rows": learned["ms_yield"], "k_learn_setup": learned["ms_lsetup"],
                        "k_learn_trials": learned["ms_ltrials"], "k_setup (routed)": learned["ms_rsetup"],
                        "k_trials_routed": learned["ms_rtrials"], "k_setup": plain["ms_setup"],
                        "k_trials": plain["ms_trials"]},
