@@ -1301,8 +1301,8 @@ static int lcc_partners(iss_ctx* ctx, const SampleArgs& a, int E, const HSetup* 
         ap.K = P;
         ap.Kdev = nullptr;
         const int nblk = static_cast<int>(
-            std::max<int64_t>(1, std::min<int64_t>((P + 255) / 256, int64_t(ctx->num_cu) * per_cu_plain)));
-        k_trials<<<nblk, 256, 0, ctx->stream>>>(ap, ctx->d_lcc_rec.p);
+            std::max<int64_t>(1, std::min<int64_t>((P + kTrialThreads - 1) / kTrialThreads, int64_t(ctx->num_cu) * per_cu_plain)));
+        k_trials<<<nblk, kTrialThreads, 0, ctx->stream>>>(ap, ctx->d_lcc_rec.p);
         HIPCHK(hipGetLastError());
     }
     k_lcc_events<<<(E + 1 + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_evoff.p, E, ctx->d_lcc_g.p, ctx->d_lcc_evoff.p);
@@ -1388,13 +1388,13 @@ static int sample_model(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadro
     }
     HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, 256, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, kTrialThreads, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     if (ctx->blocks_per_cu[0] > 0) per_cu = std::min(ctx->blocks_per_cu[0], per_cu);
     if (K > 0) {
-        const int nblk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((K + 255) / 256,
-                                                                                  int64_t(ctx->num_cu) * per_cu)));
-        k_trials<<<nblk, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+        const int nblk = static_cast<int>(std::max<int64_t>(
+            1, std::min<int64_t>((K + kTrialThreads - 1) / kTrialThreads, int64_t(ctx->num_cu) * per_cu)));
+        k_trials<<<nblk, kTrialThreads, 0, ctx->stream>>>(a, ctx->d_setup.p);
         HIPCHK(hipGetLastError());
     }
     ctx->last_plain = true;  // the records' output slots index d_out
@@ -1564,7 +1564,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
         HIPCHK(hipGetLastError());
     }
     int per_cu_trials = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_trials, k_trials, kTrialThreads, 0) != hipSuccess ||
         per_cu_trials < 1)
         per_cu_trials = 1;
     const int per_cu_plain = ctx->blocks_per_cu[0] > 0 ? std::min(ctx->blocks_per_cu[0], per_cu_trials) : per_cu_trials;
@@ -1575,10 +1575,10 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
             k_setup<false><<<(K + 255) / 256, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
             HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
-            const int64_t want = (K + 255) / 256;
+            const int64_t want = (K + kTrialThreads - 1) / kTrialThreads;
             const int nblk = static_cast<int>(
                 std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->num_cu) * per_cu_plain)));
-            k_trials<<<nblk, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+            k_trials<<<nblk, kTrialThreads, 0, ctx->stream>>>(a, ctx->d_setup.p);
         }
         ctx->last_plain = true;
     } else {
@@ -1646,7 +1646,7 @@ int iss_sample(iss_ctx* ctx, const iss_sample_opts* o, int64_t* n_hadrons) {
             af.Kdev = ctx->d_fb_count.p;
             af.refill_min = ctx->refill_min[0];
             af.stats = ctx->d_stats.p + kStatRouted;
-            k_trials_routed<<<ctx->num_cu * per_cu_plain, 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            k_trials_routed<<<ctx->num_cu * per_cu_plain, kTrialThreads, 0, ctx->stream>>>(af, ctx->d_setup.p);
             HIPCHK(hipEventRecord(ctx->ev[10], ctx->stream));
         }
     }
@@ -2203,7 +2203,8 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     a.out = ctx->d_out.p;
     a.stats = ctx->d_stats.p;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trials, kTrialThreads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
     const int per_cu_fb = per_cu;  // (the learned rounds' plain fallback)
     if (ctx->blocks_per_cu[0] > 0) per_cu = std::min(ctx->blocks_per_cu[0], per_cu);
     const int trial_blocks = ctx->num_cu * per_cu;
@@ -2270,7 +2271,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         if (N > 0) k_cell_index<<<(N + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, N, setup_cap, ctx->d_cellidx.p);
         if (!learned) {
             k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
-            k_trials<<<trial_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p);
+            k_trials<<<trial_blocks, kTrialThreads, 0, ctx->stream>>>(a, ctx->d_setup.p);
         } else {
             const int pblk = static_cast<int>(std::max<int64_t>(1, (N + kPickCells - 1) / kPickCells));
             k_batch_pick<<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, ctx->d_sp16.p, ctx->d_lbits.p,
@@ -2297,7 +2298,7 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
             k_learn_trials<<<learn_blocks, 256, 0, ctx->stream>>>(L, ctx->d_lrec.p);
             k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, ctx->d_fb_list.p,
                                                                  ctx->d_fb_count.p);
-            k_trials_routed<<<ctx->num_cu * (route ? per_cu : per_cu_fb), 256, 0, ctx->stream>>>(af, ctx->d_setup.p);
+            k_trials_routed<<<ctx->num_cu * (route ? per_cu : per_cu_fb), kTrialThreads, 0, ctx->stream>>>(af, ctx->d_setup.p);
         }
         k_batch_advance<<<1, 64, 0, ctx->stream>>>(b, ctx->d_off.p, N);
     };

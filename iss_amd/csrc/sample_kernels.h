@@ -618,12 +618,15 @@ __global__ __launch_bounds__(256) void k_setup(SampleArgs a, HSetup* __restrict_
 // depend on the launch shape.
 // idle lanes that trigger a refill, unless the autotuner chose otherwise
 constexpr int kRefillMin = 16;
-// Every kCoarse-th row of the two |p| tables below t = 30 (Bose-Einstein 0,
-// Fermi-Dirac 3), which nearly every hadron uses, is copied into LDS (e0, e1,
-// e2 per row, 24 KB) by each workgroup: the search narrows the bracket among
-// those rows first, then ends among at most kCoarse rows in global memory
-// (two dependent loads instead of about seven).
-constexpr int kCoarse = 4;
+// The two |p| tables below t = 30 (Bose-Einstein 0, Fermi-Dirac 3), which
+// nearly every hadron uses, are copied into LDS (e0, e1, e2 per row, 96 KB)
+// by each workgroup, one workgroup of kTrialThreads per CU (4 waves per SIMD
+// at the kernel's registers): the bracket search runs in LDS, without
+// dependent global loads.  (The search over LDS rows g kCoarse narrows the
+// bracket; with every row there, kCoarse = 1, it finds it.)  Other tables
+// are searched in global memory.
+constexpr int kTrialThreads = 1024;
+constexpr int kCoarse = 1;
 constexpr int kCoarseRows = (kPTabLen + kCoarse - 1) / kCoarse;
 
 __device__ __forceinline__ void trials_body(const SampleArgs& a, const HSetup* __restrict__ rec) {
@@ -880,12 +883,14 @@ __device__ __forceinline__ void trials_body(const SampleArgs& a, const HSetup* _
     if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) { trials_body(a, rec); }
+__global__ __launch_bounds__(kTrialThreads) void k_trials(SampleArgs a, const HSetup* __restrict__ rec) {
+    trials_body(a, rec);
+}
 
 // The same trials for the hadrons a learned pass hands to the plain sampler
 // (routed pairs and hadrons without an envelope): its own symbol, so that
 // profiles and timings tell it from the plain pass's launch.
-__global__ __launch_bounds__(256) void k_trials_routed(SampleArgs a, const HSetup* __restrict__ rec) {
+__global__ __launch_bounds__(kTrialThreads) void k_trials_routed(SampleArgs a, const HSetup* __restrict__ rec) {
     trials_body(a, rec);
 }
 
