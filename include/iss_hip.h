@@ -327,7 +327,7 @@ typedef struct {
     uint64_t seed;
     int32_t hydro_mode;          /* 1 boost invariant (y in [y_LB, y_RB]), 2 3+1D */
     double y_LB, y_RB;
-    int32_t events_per_round;    /* R; 0: the autotuned value (default 16) */
+    int32_t events_per_round;    /* R (at most 16384); 0: the autotuned value (default 16) */
     int32_t max_trials;
     int32_t precision;           /* yields: 64 or 32 */
     int32_t use_graph;           /* 1: one captured round replayed; 0: launched per round */
@@ -342,7 +342,7 @@ int iss_batch_rounds(const iss_ctx* ctx);
 int iss_batch_events_per_round(const iss_ctx* ctx);
 double iss_batch_ms(const iss_ctx* ctx);   /* device time of the last batch (all rounds) */
 int iss_get_batch_layout(iss_ctx* ctx, int64_t* segments, int64_t* event_sizes);
-/* times whole batches for R in {4, 16, 64, 256, 1024, 4096}, keeps the
+/* times whole batches for R in {4, 16, 64, 256, 1024, 4096, 16384}, keeps the
  * fastest as the default events_per_round of that proposal (the learned one
  * also times its routing thresholds at that R); report: refill_min = R
  * chosen, ms_* batch times, learned_min_hadrons */

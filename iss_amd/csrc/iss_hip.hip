@@ -152,6 +152,9 @@ constexpr size_t PIN_OOB = 0, PIN_K = 8, PIN_STATS = 64, PIN_AUX = 192, PIN_TOT 
 // learned fallback and envelope violations; [6..7] batch overflow flags; [8]
 // routed hadrons of batches; [9..12] the routed plain trials' [0..3]
 constexpr int kStatRouted = 9, kStatSlots = 13;
+// events per surface and round of a batch at most (one round covers up to
+// this many events of every surface)
+constexpr int kMaxEventsPerRound = 16384;
 
 // Learned passes: (cell, species) pairs of fewer hadrons than the routing
 // threshold take the plain sampler.  A pair's network and envelope cost
@@ -1199,10 +1202,8 @@ int iss_get_surface_lrf(iss_ctx* ctx, float* cells28) {
 int iss_compute_yields(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals, void* per_cell) {
     if (!ctx || !opts) return ISS_ERR_ARG;
     (void)hipSetDevice(ctx->device);
-    // batched surfaces are sampled from FP64 species CDF rows, one surface
-    // from its yield rows
-    return run_yields(ctx, opts->precision, per_cell, species_totals, 0, nullptr,
-                      ctx->S > 0 ? LAYOUT_CDF : LAYOUT_ROWS);
+    // (one surface and batched surfaces alike are sampled from the yield rows)
+    return run_yields(ctx, opts->precision, per_cell, species_totals, 0, nullptr, LAYOUT_ROWS);
 }
 
 int iss_compute_yields_noalloc(iss_ctx* ctx, const iss_yield_opts* opts, double* species_totals, void* per_cell,
@@ -2126,11 +2127,12 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         maxE = std::max(maxE, n_events[k]);
     }
     const int R = o->events_per_round > 0 ? o->events_per_round : ctx->batch_R_tuned[o->proposal == 1 ? 1 : 0];
-    if (R > 4096) return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most 4096");
+    if (R > kMaxEventsPerRound)
+        return set_err(ctx, ISS_ERR_ARG, "events_per_round must be at most " + std::to_string(kMaxEventsPerRound));
     invalidate_output(ctx);
-    if (!ctx->yields_valid || ctx->yields_lcc || ctx->yield_layout != LAYOUT_CDF ||
+    if (!ctx->yields_valid || ctx->yields_lcc || ctx->yield_layout != LAYOUT_ROWS ||
         (o->precision != ctx->yield_precision && (o->precision == 32 || o->precision == 64))) {
-        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, 0, nullptr, LAYOUT_CDF);
+        int rc = run_yields(ctx, o->precision == 64 ? 64 : 32, nullptr, nullptr, 0, nullptr, LAYOUT_ROWS);
         if (rc) return rc;
     }
     const int64_t N = ctx->Nc;
@@ -2193,7 +2195,9 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     a.out_cap = static_cast<uint64_t>(out_cap);
     a.setup_cap = setup_cap;
     HIPCHK(ctx->d_cellidx.ensure(std::max<int64_t>(setup_cap, 1)));
+    HIPCHK(ctx->d_sp16.ensure(std::max<int64_t>(setup_cap, 1)));
     a.cellidx = ctx->d_cellidx.p;
+    a.slot_species = ctx->d_sp16.p;  // (every slot's species from the round's pick)
     a.boost_invariant = binv ? 1 : 0;
     a.ecell = binv ? nullptr : ctx->d_ecell.p;  // 3+1D: the cells' flow and position words
     a.yLB = static_cast<float>(o->y_LB);
@@ -2220,7 +2224,6 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     const bool route = learned && ctx->learn_route > 1;
     if (learned) {
         const int64_t cap = std::max<int64_t>(setup_cap, 1);
-        HIPCHK(ctx->d_sp16.ensure(cap));
         HIPCHK(ctx->d_lbits.ensure(std::max<int64_t>(N, 1) * Wb));
         HIPCHK(ctx->d_lcnt.ensure(std::max<int64_t>(N, 1)));
         HIPCHK(ctx->d_lscnt.ensure(std::max<int64_t>(N, 1)));
@@ -2230,7 +2233,6 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         HIPCHK(ctx->d_lslot.ensure(cap));
         HIPCHK(ctx->d_lrec.ensure(cap));
         HIPCHK(ctx->d_fb_list.ensure(cap));
-        a.slot_species = ctx->d_sp16.p;  // the fallback set-up reads the drawn species
         L.a = a;
         L.aux = ctx->d_aux.p;
         L.W = MlpDev{ctx->d_mlp_frag.p, ctx->d_mlp_par.p};
@@ -2268,15 +2270,33 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
         }
         k_batch_multi_leaves<<<((static_cast<int64_t>(S) << D) + 255) / 256, 256, 0, ctx->stream>>>(
             b, cur, D, ctx->d_bevcount.p);
-        if (N > 0) k_cell_index<<<(N + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_off.p, N, setup_cap, ctx->d_cellidx.p);
+        // every slot's cell and species from its cell's yield row (the FP64
+        // prefix in LDS, as one surface's pass); the learned rounds' pairs
+        if (N > 0) {
+            const int pblk = static_cast<int>((N + kPickCells - 1) / kPickCells);
+            const bool f64 = ctx->yield_precision == 64;
+            const void* rows = ctx->d_rows.p;
+            uint32_t* gb = learned ? ctx->d_lbits.p : nullptr;
+            uint32_t* gc = learned ? ctx->d_lcnt.p : nullptr;
+            uint32_t* gs = route ? ctx->d_lscnt.p : nullptr;
+            const int thr = ctx->learn_route;
+            if (f64 && learned)
+                k_pick<double, true><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, true), ctx->stream>>>(
+                    a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, thr);
+            else if (f64)
+                k_pick<double, false><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, false), ctx->stream>>>(
+                    a, static_cast<const double*>(rows), row_stride(ctx->Ns, 8), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, thr);
+            else if (learned)
+                k_pick<float, true><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, true), ctx->stream>>>(
+                    a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, thr);
+            else
+                k_pick<float, false><<<pblk, kWave * kPickCells, pick_lds_bytes(ctx->Ns, false), ctx->stream>>>(
+                    a, static_cast<const float*>(rows), row_stride(ctx->Ns, 4), ctx->d_sp16.p, ctx->d_cellidx.p, gb, gc, gs, thr);
+        }
         if (!learned) {
             k_setup<true><<<setup_blocks, 256, 0, ctx->stream>>>(a, ctx->d_setup.p, nullptr, nullptr);
             k_trials<<<trial_blocks, kTrialThreads, 0, ctx->stream>>>(a, ctx->d_setup.p);
         } else {
-            const int pblk = static_cast<int>(std::max<int64_t>(1, (N + kPickCells - 1) / kPickCells));
-            k_batch_pick<<<pblk, kWave * kPickCells, 0, ctx->stream>>>(a, ctx->d_sp16.p, ctx->d_lbits.p,
-                                                                       ctx->d_lcnt.p, route ? ctx->d_lscnt.p : nullptr,
-                                                                       ctx->learn_route);
             k_scan_tiles<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p);
             k_scan_top<<<1, kScanBlock, 0, ctx->stream>>>(ctx->d_scan.p, ntiles);
             k_scan_apply<<<ntiles, kScanBlock, 0, ctx->stream>>>(ctx->d_lcnt.p, N, ctx->d_scan.p, ctx->d_pairoff.p);
@@ -2404,8 +2424,8 @@ int iss_autotune_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_e
     // network and envelopes once per (cell, species) pair of a round, so
     // longer rounds share them among more hadrons)
     const bool learned = o->proposal == 1;
-    const int cand[] = {4, 16, 64, 256, 1024, 4096};
-    const int ncand = 6, R_def = learned ? 256 : 16;
+    const int cand[] = {4, 16, 64, 256, 1024, 4096, kMaxEventsPerRound};
+    const int ncand = 7, R_def = learned ? 256 : 16;
     iss_batch_opts bo = *o;
     double best = 1e30, ms_def = 0.;
     int best_R = R_def, tried = 0;

@@ -154,7 +154,7 @@ struct SampleArgs {
     // local charge conservation: the antiparticle's species index ([Ns], -1
     // none); partner records carry kPartnerBit in j
     const int* anti;
-    // the cell of every slot of the pass (k_cell_index or k_pick); null: search the offsets
+    // the cell of every slot of the pass (k_pick); null: search the offsets
     const uint32_t* cellidx;
     // the species of every slot (k_pick); null: drawn from the FP64 species CDF rows
     const uint16_t* slot_species;

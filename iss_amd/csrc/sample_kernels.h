@@ -336,41 +336,6 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
     q.fsg = static_cast<float>(a.sp.sign[s]);
 }
 
-// The cell of every hadron slot of a pass (the set-up then reads it instead
-// of searching the cell offsets per slot).  One wave per 64 consecutive
-// cells; the lanes load the cells' offsets.  With many slots per cell (more
-// than kCellIdxCoop per lane on average) the wave writes each cell's run of
-// slots in turn, lanes side by side, so the stores are contiguous; with few
-// (batched rounds: about one slot per cell) each lane writes its own cell's
-// slots, which skips the per-cell steps of empty cells.  Slots at or beyond
-// `cap` (> 0) are not written.  Launch with ceil(Nc / 64) waves.
-constexpr int kCellIdxCoop = 4;
-__global__ __launch_bounds__(256) void k_cell_index(const int64_t* __restrict__ off, int64_t Nc, int64_t cap,
-                                                    uint32_t* __restrict__ cellidx) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t c0 = (static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave;
-    if (c0 >= Nc) return;  // whole waves
-    const int64_t c = c0 + lane;
-    int64_t o0 = 0, o1 = 0;
-    if (c < Nc) {
-        o0 = off[c];
-        o1 = off[c + 1];
-        if (cap > 0) {
-            o0 = min(o0, cap);
-            o1 = min(o1, cap);
-        }
-    }
-    const int nc = static_cast<int>(min<int64_t>(kWave, Nc - c0));
-    const int64_t first = __shfl(o0, 0), last = __shfl(o1, nc - 1);
-    if (last - first <= static_cast<int64_t>(kCellIdxCoop) * kWave) {
-        for (int64_t i = o0; i < o1; i++) cellidx[i] = static_cast<uint32_t>(c);
-        return;
-    }
-    for (int j = 0; j < nc; j++) {
-        const int64_t b = __shfl(o0, j), e = __shfl(o1, j);
-        for (int64_t i = b + lane; i < e; i += kWave) cellidx[i] = static_cast<uint32_t>(c0 + j);
-    }
-}
 
 // Species of every hadron slot of a pass, one wave per cell.  By Poisson
 // splitting a hadron of cell c is of species s with probability
@@ -434,8 +399,13 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
     if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
+    // (batched rounds: the round index lives on the device)
+    const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
     const int Ns = a.Ns, Wb = (Ns + 31) / 32;
-    const int64_t o0 = a.off[c], n = a.off[c + 1] - o0;
+    const int64_t o0 = a.off[c];
+    int64_t n = a.off[c + 1] - o0;
+    // (batched rounds: only the slots with set-up records)
+    if (a.setup_cap > 0) n = max<int64_t>(0, min<int64_t>(n, a.setup_cap - o0));
     if (n == 0) {
         if (PAIRS) {
             if (lane < Wb) gbits[c * Wb + lane] = 0u;
@@ -468,7 +438,7 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     const double tot = carry;
     auto positive = [&](int s) { return (pos[s >> 5] >> (s & 31)) & 1u; };
     for (int64_t r = lane; r < n; r += kWave) {
-        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, a.chunk));
+        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, chunk));
         int sp = upper_bound(pre, 0, Ns, st.uniform() * tot);
         if (sp >= Ns) sp = Ns - 1;
         if (!positive(sp)) {
@@ -507,40 +477,6 @@ __device__ __forceinline__ uint64_t batch_out(const SampleArgs& a, int64_t i, in
     const int k = upper_bound(a.surf_off + 1, 0, a.S, c);
     const int64_t h0 = a.off[a.surf_off[k]];
     return *a.out_base + static_cast<uint64_t>(h0) + a.perms[k](static_cast<uint64_t>(i - h0));
-}
-
-// Batched rounds with the learned proposal: each slot's species drawn from its
-// cell's FP64 CDF row exactly as k_setup draws it (the same stream, so the
-// species of every slot equal the plain rounds'), and the cell's pair bitmap
-// and number of pairs (count_pairs, as k_pick).  Slots at or beyond
-// the round's set-up records are left out.  One wave per cell;
-// launch with ceil(Nc / kPickCells) workgroups of kPickCells waves.
-__global__ __launch_bounds__(kWave * kPickCells) void k_batch_pick(SampleArgs a, uint16_t* __restrict__ sp16,
-                                                                   uint32_t* __restrict__ gbits,
-                                                                   uint32_t* __restrict__ cnt,
-                                                                   uint32_t* __restrict__ scnt, int thr) {
-    __shared__ uint32_t s_hits[kPickCells][kMaxSpecies];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kPickCells + w;
-    if (c >= a.Nc) return;  // whole waves; no workgroup barrier below
-    const uint32_t chunk = a.chunk_dev ? *a.chunk_dev : a.chunk;
-    const int Ns = a.Ns, Wb = (Ns + 31) / 32;
-    const int64_t o0 = a.off[c];
-    int64_t n = a.off[c + 1] - o0;
-    if (a.setup_cap > 0) n = max<int64_t>(0, min<int64_t>(n, a.setup_cap - o0));
-    uint32_t* hits = s_hits[w];
-    for (int s = lane; s < Ns; s += kWave) hits[s] = 0u;
-    wave_lds_sync();
-    const double* row = a.cdf + c * cdf_stride(Ns);
-    for (int64_t r = lane; r < n; r += kWave) {
-        Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, chunk));
-        int sp = upper_bound(row, 0, Ns, st.uniform() * a.ctot[c]);
-        if (sp >= Ns) sp = Ns - 1;
-        sp16[o0 + r] = static_cast<uint16_t>(sp);
-        atomicAdd(&hits[sp], 1u);
-    }
-    wave_lds_sync();
-    count_pairs(hits, Ns, thr, c, Wb, gbits, cnt, scnt);
 }
 
 // the species-dependent part of a set-up record: the |p| table, its first

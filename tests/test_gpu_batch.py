@@ -186,7 +186,7 @@ def test_autotuned_round_size_is_used(species, surfaces):
     """one tuned round size per proposal, each used by its own batches; the
     learned proposal also times its routing thresholds and keeps one"""
     ctx = _ctx(species, surfaces)
-    cand = (4, 16, 64, 256, 1024, 4096)
+    cand = (4, 16, 64, 256, 1024, 4096, 16384)
     routes = (1, 2, 3, 4, 6, 8, 12, 16, 65)  # 65: every pair routed
     rep = ctx.autotune_batch(EVENTS, seed=2)
     ctx.mlp_init(1)
