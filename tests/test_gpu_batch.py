@@ -97,11 +97,14 @@ def test_layout_tiles_the_output(species, surfaces):
     ctx.close()
 
 
-def test_every_surface_follows_its_yields(species, surfaces):
+@pytest.mark.parametrize("precision", [32, 64])
+def test_every_surface_follows_its_yields(species, surfaces, precision):
+    """Rounds draw every slot's species from its cell's yield row in the
+    working precision (FP32 rows, or FP64 rows with precision 64)."""
     ctx = _ctx(species, surfaces)
-    _, per = ctx.compute_yields(precision=32, per_cell=True)
+    _, per = ctx.compute_yields(precision=precision, per_cell=True)
     ev = np.array([400, 60, 1000, 30, 7, 0, 90], dtype=np.int32)
-    n = ctx.sample_batch(ev, seed=21, events_per_round=16)
+    n = ctx.sample_batch(ev, seed=21, events_per_round=16, precision=precision)
     h = ctx.hadrons()
     layout = ctx.batch_layout()
     off = np.concatenate([[0], np.cumsum(SIZES)])
