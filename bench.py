@@ -50,7 +50,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector, packed (78.6 unpacked)
 
-# Algorithmic units (the survey of the path's measurement): a sampling pass
+# Algorithmic units of the path's measurement: a sampling pass
 # costs 200 FP32 ops per trial (|p| proposal ~115 incl. inner tries, outer
 # trial ~85) plus 250 per accepted hadron (CDF refresh ~200, boost and record
 # ~50), FMA = 2; one network evaluation 2 (14 x 64 + 64 x 64 + 64 x 8) FLOP

@@ -2,7 +2,7 @@
 //
 // A CPU restatement, in FP64 with std::mt19937, of the iSS "FSSW" Cooper-Frye
 // sampling path (chunshen1987/iSS, MC_sampling = 4).  It is the checker that the
-// HIP product is judged against.  Only tests/, __graft_entry__.smoke() and the
+// HIP product is checked against.  Only tests/, __graft_entry__.smoke() and the
 // cpu_baseline leg of bench.py may load it; the product never links, imports or
 // calls it.
 //
