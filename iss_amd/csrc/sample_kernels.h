@@ -364,10 +364,11 @@ __host__ __device__ constexpr size_t pick_lds_bytes(int Ns, bool pairs) {
 // at least `thr` slots; with scnt (routing, thr > 1) the slots of the other
 // species are counted in scnt[c]: they take the plain sampler, whose
 // per-hadron set-up costs less than the network and envelope of a pair of
-// few hadrons.  Writes the pair bitmap (Wb words) and the number of pairs.
+// few hadrons.  Writes the pair bitmap (Wb words, gathered in the wave's
+// LDS words `wbuf` and stored at once) and the number of pairs.
 __device__ __forceinline__ void count_pairs(const uint32_t* hits, int Ns, int thr, int64_t c, int Wb,
-                                            uint32_t* __restrict__ gbits, uint32_t* __restrict__ cnt,
-                                            uint32_t* __restrict__ scnt) {
+                                            uint32_t* wbuf, uint32_t* __restrict__ gbits,
+                                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ scnt) {
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t np = 0, ns = 0;
     for (int s0 = 0; s0 < Ns; s0 += kWave) {
@@ -377,11 +378,13 @@ __device__ __forceinline__ void count_pairs(const uint32_t* hits, int Ns, int th
         if (k > 0 && !pair) ns += k;
         const unsigned long long m = __ballot(pair);
         if (lane == 0) {
-            gbits[c * Wb + (s0 >> 5)] = static_cast<uint32_t>(m);
-            if ((s0 >> 5) + 1 < Wb) gbits[c * Wb + (s0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
+            wbuf[s0 >> 5] = static_cast<uint32_t>(m);
+            if ((s0 >> 5) + 1 < Wb) wbuf[(s0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
         }
         np += static_cast<uint32_t>(__popcll(m));
     }
+    wave_lds_sync();
+    if (lane < Wb) gbits[c * Wb + lane] = wbuf[lane];
     if (lane == 0) cnt[c] = np;
     if (scnt) {
         ns = wave_sum(ns);
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     }
     if (PAIRS) {
         wave_lds_sync();
-        count_pairs(hits, Ns, thr, c, Wb, gbits, cnt, scnt);
+        count_pairs(hits, Ns, thr, c, Wb, pos, gbits, cnt, scnt);  // (pos is free after the slots)
     }
 }
 
