@@ -7,8 +7,9 @@ T in [0.135, 0.160] GeV, mu_B in [0, 0.3] GeV, |u_perp| <= 3, random
 traceless pi with |pi|/(e+P) <= 0.3, bulk Pi/(e+P) in [-0.2, 0], OSU
 14-moment bulk + shear delta f), the 321-species UrQMD hadron list,
 normalised to 1e4 hadrons per event.  One step = one yield pass (FP32
-series, FP64 species CDF) + one sampling pass of 1000 oversampled events
-(~1e7 hadrons), surface resident in HBM.  The plain steps run first; the
+series and yield rows; FP64 species totals) + one sampling pass of 1000
+oversampled events (~1e7 hadrons; each cell's FP64 species prefix is built
+at pick time), surface resident in HBM.  The plain steps run first; the
 network is then trained on the 1e7 hadrons the last plain pass accepted
 (outside the timed region), and the learned steps run on the same surface.
 `value` = hadrons/s of the learned-proposal steps (yields included); the
@@ -305,7 +306,8 @@ def run_batched(args):
     plain = run(0)
     learned = run(1)
     nc, ns = ctx.num_cells(), ctx.n_species
-    ybytes = yield_bytes(nc, ns, 8)  # (the batched pass stores FP64 species CDF rows)
+    # (the batched pass stores yield rows in the working precision)
+    ybytes = yield_bytes(nc, ns, 4 if args.precision == 32 else 8)
     my = float(np.mean(learned["ms_yield"]))
     y_gbs = ybytes / (my * 1e-3) / 1e9
     v_learn = learned["hadrons"] / learned["elapsed"]
@@ -341,7 +343,7 @@ def run_batched(args):
         "data": "synthetic",
         "config": config,
         "roofline": {"bound": "hbm", "achieved": y_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": y_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_yield", "ms": my,
+                     "frac": y_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_yield_rows", "ms": my,
                      "bytes": ybytes},
         "cpu_baseline": None,
         "plain_hadrons_per_s": v_plain,

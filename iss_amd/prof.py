@@ -2,6 +2,12 @@
 
     python -m iss_amd.prof stats <kernel_stats.csv>
         kernel-trace summary: calls, total, average and share per kernel
+    python -m iss_amd.prof window <kernel_trace.csv> <anchor> <steps> [<stop>]
+        the timed steps of a run: the dispatches before the first dispatch
+        of the kernel `stop` (the first kernel of the work after the steps;
+        without it, the whole trace), from the start of the steps-th last
+        dispatch of the kernel `anchor` (the first kernel of a step) among
+        them; per kernel calls, ms per step and share
     python -m iss_amd.prof traffic <fetch_dir> <write_dir> [out.json]
         HBM bytes of each kernel's largest dispatch (the full-size launch: the
         autotuner's shorter passes dispatch the same kernels) from two
@@ -50,6 +56,33 @@ def stats(path):
     for r in rows:
         print(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
               f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} % |")
+
+
+def window(path, anchor, steps, stop=None):
+    steps = int(steps)
+    rows = [(_short(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+            for r in csv.DictReader(open(path))]
+    rows.sort(key=lambda r: r[1])
+    if stop:
+        rows = rows[:next((i for i, r in enumerate(rows) if r[0] == stop), len(rows))]
+    starts = [t0 for k, t0, _ in rows if k == anchor]
+    if len(starts) < steps:
+        raise SystemExit(f"{anchor}: {len(starts)} dispatches, fewer than {steps} steps")
+    t_begin = starts[-steps]
+    win = [r for r in rows if r[1] >= t_begin]
+    agg = collections.defaultdict(lambda: [0, 0])
+    for k, t0, t1 in win:
+        agg[k][0] += 1
+        agg[k][1] += t1 - t0
+    busy = sum(v[1] for v in agg.values())
+    span = max(t1 for _, _, t1 in win) - t_begin
+    print(f"{len(win)} dispatches from the {steps}-th last `{anchor}`: kernels busy {busy / 1e6 / steps:.3f} ms "
+          f"per step, span {span / 1e6 / steps:.3f} ms per step (with the host's gaps between launches)")
+    print()
+    print("| kernel | calls per step | ms per step | average us | share of busy |")
+    print("|---|---|---|---|---|")
+    for k, (n, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{k}` | {n / steps:g} | {ns / 1e6 / steps:.3f} | {ns / n / 1e3:.1f} | {100.0 * ns / busy:.1f} % |")
 
 
 def traffic(fetch_dir, write_dir, out=None, copy_mib=256):
@@ -161,4 +194,4 @@ def util(*args):
 
 if __name__ == "__main__":
     cmd, args = sys.argv[1], sys.argv[2:]
-    {"stats": stats, "traffic": traffic, "counters": counters, "util": util}[cmd](*args)
+    {"stats": stats, "window": window, "traffic": traffic, "counters": counters, "util": util}[cmd](*args)
