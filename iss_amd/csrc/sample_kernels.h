@@ -340,10 +340,13 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 // Species of every hadron slot of a pass, one wave per cell.  By Poisson
 // splitting a hadron of cell c is of species s with probability
 // N_{c,s} / N_c.  The cell's yield row (k_yield_rows, working precision)
-// becomes its FP64 inclusive species prefix in LDS: a wave scan per chunk of
-// 64 species, made non-decreasing by a running maximum (the scan's tree of
-// roundings could otherwise step down where a yield is zero), plus the
-// previous chunks' total.  Slot j draws u from its stream (c, j, TAG_SETUP)
+// becomes its FP64 inclusive species prefix in LDS: lane l sums the G =
+// ceil(Ns / 64) consecutive species from l G in order, one wave scan of the
+// lanes' totals gives each lane its base, and a running maximum of the lanes'
+// last values keeps the prefix non-decreasing across lanes (the scan's tree
+// of roundings could otherwise step down where a lane's yields are zero).
+// Within a lane the prefix of a zero-yield species equals its predecessor's.
+// Slot j draws u from its stream (c, j, TAG_SETUP)
 // and takes the first species whose prefix exceeds u times the cell total; a
 // species of zero yield is never taken (the nearest species of positive
 // yield replaces one that the roundings of the scan leave an interval of
@@ -355,6 +358,7 @@ __device__ __forceinline__ void pstate_of(const SampleArgs& a, int64_t c, int s,
 // latency-bound kernel keeps its full occupancy).
 constexpr int kPickCells = 4;
 constexpr int kPickWords = kMaxSpecies / 32;
+constexpr int kPickPerLane = kMaxSpecies / kWave;
 __host__ __device__ constexpr size_t pick_lds_bytes(int Ns, bool pairs) {
     return static_cast<size_t>(kPickCells) * Ns * (sizeof(double) + (pairs ? sizeof(uint32_t) : 0));
 }
@@ -423,22 +427,44 @@ __global__ __launch_bounds__(kWave * kPickCells) void k_pick(SampleArgs a, const
     if (PAIRS)
         for (int s = lane; s < Ns; s += kWave) hits[s] = 0u;
     const R* row = rows + c * stride;
-    double carry = 0.;
-    for (int s0 = 0; s0 < Ns; s0 += kWave) {
-        const int s = s0 + lane;
-        const double y = s < Ns ? static_cast<double>(row[s]) : 0.;
-        // (DPP scans: no LDS round trips)
-        const double incl = wave_max_scan_dpp(wave_scan_dpp(y)) + carry;
-        if (s < Ns) pre[s] = incl;
-        const unsigned long long m = __ballot(y > 0.);
-        if (lane == 0) {
-            pos[s0 >> 5] = static_cast<uint32_t>(m);
-            if ((s0 >> 5) + 1 < Wb) pos[(s0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
-        }
-        carry = wave_last(incl);
+    // the positive-yield bitmap, set bit by bit below
+    if (lane < Wb) pos[lane] = 0u;
+    wave_lds_sync();
+    const int G = (Ns + kWave - 1) / kWave, sl = lane * G;
+    // (the loads issued together: indices clamped into the row, values
+    // beyond the lane's species zeroed)
+    R yr[kPickPerLane];
+#pragma unroll
+    for (int k = 0; k < kPickPerLane; k++) yr[k] = row[min(sl + k, Ns - 1)];
+    double v[kPickPerLane];
+    double acc = 0.;
+    uint32_t mb = 0u;  // bit k: species sl + k has a positive yield
+#pragma unroll
+    for (int k = 0; k < kPickPerLane; k++) {
+        const double y = (k < G && sl + k < Ns) ? static_cast<double>(yr[k]) : 0.;
+        acc += y;
+        v[k] = acc;
+        mb |= (y > 0.) ? (1u << k) : 0u;
+    }
+    // the lane's bits span at most two bitmap words (G <= 8); a word past
+    // the species is only touched with bits of valid species
+    const int b0 = sl & 31;
+    atomicOr(&pos[sl >> 5], mb << b0);
+    const uint32_t mhi = b0 ? (mb >> (32 - b0)) : 0u;
+    if (mhi) atomicOr(&pos[(sl >> 5) + 1], mhi);
+    // the earlier lanes' sum (DPP scan), and the running maximum of the
+    // earlier lanes' last values
+    double base = __shfl_up(wave_scan_dpp(acc), 1, kWave);
+    if (lane == 0) base = 0.;
+    double lmax = __shfl_up(wave_max_scan_dpp(base + acc), 1, kWave);
+    if (lane == 0) lmax = 0.;
+#pragma unroll
+    for (int k = 0; k < kPickPerLane; k++) {
+        const int s = sl + k;
+        if (k < G && s < Ns) pre[s] = fmax(base + v[k], lmax);
     }
     wave_lds_sync();
-    const double tot = carry;
+    const double tot = pre[Ns - 1];
     auto positive = [&](int s) { return (pos[s >> 5] >> (s & 31)) & 1u; };
     for (int64_t r = lane; r < n; r += kWave) {
         Stream st(a.seed, static_cast<uint32_t>(c), static_cast<uint32_t>(r), tag_word(TAG_SETUP, chunk));
