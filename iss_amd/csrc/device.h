@@ -206,22 +206,29 @@ __device__ inline int64_t binomial(int64_t n, double p, Stream& s) {
 }
 
 // ------------------------------------------------------------ permutation
-// A keyed pseudo-random permutation of [0, n): an unbalanced Feistel network
-// on the smallest power-of-two domain >= n, six rounds, with cycle walking.
+// A keyed pseudo-random permutation of [0, n), n < 2^32: a mixed-radix
+// Feistel network on [0, a b), a = ceil(sqrt n), b = ceil(n / a) (both at
+// least 2), six rounds, with cycle walking.  x = L b + R with L in [0, a),
+// R in [0, b); a round maps (L, R) to (R, (L + F(R)) mod a) and swaps the
+// radices, so after an even number of rounds the radices are (a, b) again.
+// F(R) is a keyed 32-bit hash brought into [0, a) by a multiply-high.  Since
+// a b - n < a, an output falls outside [0, n) with probability below
+// 1 / sqrt(n): the walk almost never takes a second pass, so the lanes of a
+// wave do not wait on each other's walks (with a power-of-two domain up to
+// half of the outputs walk, and a wave runs several passes).
 struct Permutation {
     uint64_t n;
-    int lbits, rbits;
+    uint64_t cb;  // ceil(2^64 / b): x / b = mulhi(cb, x) for every x < 2^32
+    uint32_t a, b;
     uint32_t key0, key1;
-    __host__ __device__ static int bits_for(uint64_t n) {
-        int b = 1;
-        while ((1ull << b) < n) b++;
-        return b < 2 ? 2 : b;
-    }
     __host__ __device__ void init(uint64_t n_in, uint64_t seed) {
         n = n_in;
-        const int b = bits_for(n_in);
-        lbits = (b + 1) / 2;
-        rbits = b / 2;
+        uint64_t s = 1;
+        while (s * s < n_in) s++;
+        a = static_cast<uint32_t>(s < 2 ? 2 : s);
+        const uint64_t bb = (n_in + a - 1) / a;
+        b = static_cast<uint32_t>(bb < 2 ? 2 : bb);
+        cb = ~0ull / b + 1;
         key0 = static_cast<uint32_t>(seed * 0x9E3779B97F4A7C15ull >> 32);
         key1 = static_cast<uint32_t>(seed);
     }
@@ -234,25 +241,28 @@ struct Permutation {
         x ^= x >> 16;
         return x;
     }
-    __device__ __forceinline__ uint64_t once(uint64_t v) const {
-        uint64_t L = v >> rbits, R = v & ((1ull << rbits) - 1);
-        int lw = lbits, rw = rbits;
+    __device__ __forceinline__ uint32_t once(uint32_t v) const {
+        // (L, R) = (v / b, v mod b) by the multiply-high of cb
+        const uint64_t t = static_cast<uint64_t>(static_cast<uint32_t>(cb)) * v;
+        const uint64_t u = static_cast<uint64_t>(static_cast<uint32_t>(cb >> 32)) * v + (t >> 32);
+        uint32_t L = static_cast<uint32_t>(u >> 32), R = v - L * b;
+        uint32_t A = a, B = b;
 #pragma unroll
         for (int r = 0; r < 6; r++) {
-            const uint32_t f = mix(static_cast<uint32_t>(R) ^ (static_cast<uint32_t>(R >> 32) * 0x9E3779B9u),
-                                   key0 + 0x632BE5ABu * r) ^ key1;
-            const uint64_t nl = R;
-            const uint64_t nr = (L ^ static_cast<uint64_t>(f)) & ((1ull << lw) - 1);
-            L = nl;
+            const uint32_t f = mix(R, key0 + 0x632BE5ABu * r) ^ key1;
+            const uint32_t add = static_cast<uint32_t>((static_cast<uint64_t>(f) * A) >> 32);  // in [0, A)
+            uint32_t nr = L + add;
+            nr = nr >= A ? nr - A : nr;
+            L = R;
             R = nr;
-            const int t = lw;
-            lw = rw;
-            rw = t;
+            const uint32_t tmp = A;
+            A = B;
+            B = tmp;
         }
-        return (L << rw) | R;
+        return L * b + R;
     }
     __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
-        uint64_t x = once(v);
+        uint32_t x = once(static_cast<uint32_t>(v));
         while (x >= n) x = once(x);
         return x;
     }

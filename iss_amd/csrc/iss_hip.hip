@@ -2161,6 +2161,9 @@ int iss_sample_batch(iss_ctx* ctx, const iss_batch_opts* o, const int32_t* n_eve
     }
     const int64_t out_cap = static_cast<int64_t>(total + 10. * std::sqrt(total) + 4096.);
     const int64_t setup_cap = static_cast<int64_t>(maxr + 10. * std::sqrt(maxr) + 4096.);
+    // (a round's slots are placed by 32-bit permutations)
+    if (setup_cap >= (1ll << 31))
+        return set_err(ctx, ISS_ERR_RANGE, "more than 2^31 hadrons in one batched round: lower events_per_round");
     int D = 0;
     while ((1ll << D) < R) D++;
     HIPCHK(ctx->d_out.ensure(out_cap));
