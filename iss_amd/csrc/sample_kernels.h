@@ -281,8 +281,9 @@ __device__ void pcdf_all(const PTabDev& pt, int tb, double E, double out[3]) {
     for (int n = 0; n < trunc; n++) {
         const int n1 = n + 1;
         const double dn = static_cast<double>(n1);
-        if (trunc <= 5)
-            c0 += (fer ? static_cast<double>(static_cast<int>(sign) / n1) : kInvPow[0][n]) * em * (1. - ee);
+        // (Fermi-Dirac: the integer quotient sign / n of the series, which is
+        // sign for n = 1 and 0 beyond, as a select)
+        if (trunc <= 5) c0 += (fer ? (n == 0 ? sign : 0.) : kInvPow[0][n]) * em * (1. - ee);
         const double s1 = fer ? sign : 1.;
         c1 += s1 * kInvPow[1][n] * em * ((m * dn + 1) - ee * (E * dn + 1));
         const double s2 = (fer && !quirk) ? sign : 1.;
