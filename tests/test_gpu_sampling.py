@@ -181,3 +181,32 @@ def test_event_counts_split_sizes(n_events):
     ctx.sample(n_events, seed=21)
     assert np.array_equal(ctx.event_counts(), c)
     ctx.close()
+
+
+def test_event_placement_writes_every_slot_once():
+    """Slots go to output positions through a keyed permutation of [0, K)
+    (a mixed-radix Feistel network with cycle walking): for totals K from 1
+    up to ~1e5, over many seeds (K squares, one off a square, primes and
+    everything between occur), every output record is written exactly once,
+    so none is left empty, and the event sizes add up to K."""
+    sp = tables.species_arrays(1)
+    ctx = iss_amd.Context(0)
+    ctx.set_species(sp)
+    ctx.set_deltaf(shear=1, bulk=1, bulk_kind=11, k11=tables.read_kind11(1))
+    seen = set()
+    for ncell, target, nev, seeds in ((1, 0.5, 4, range(40)), (3, 4.0, 7, range(40)),
+                                      (50, 30.0, 31, range(10)), (2000, 200.0, 500, range(3))):
+        cells, _ = synth.viscous_lrf(ncell, sp, seed=ncell, target_hadrons=target)
+        ctx.set_surface_lrf(cells)
+        for seed in seeds:
+            n = ctx.sample(nev, seed=1000 + seed)
+            st = ctx.stats()
+            assert st.n_failed == 0 and st.n_range_errors == 0
+            h = ctx.hadrons()
+            assert h.size == n and ctx.event_counts().sum() == n
+            # (an unwritten record is all zeros; a written one has pid != 0 and E > 0)
+            assert np.all(h["pid"] != 0) and np.all(h["E"] > 0.)
+            seen.add(n)
+    print("totals sampled", len(seen), "from", min(seen), "to", max(seen))
+    assert min(seen) <= 3 and max(seen) > 50000
+    ctx.close()
